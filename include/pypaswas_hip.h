@@ -1,0 +1,124 @@
+/*
+ * pypaswas_hip.h — C ABI of the MI355X (gfx950) Smith-Waterman scoring and
+ * traceback library.  Plain pointers and sizes only; every call returns an
+ * int status (PW_OK or a negative PW_E* code) and leaves a message for
+ * pw_last_error().  A context is used by one host thread at a time.
+ *
+ * Each entry point replaces the part of pyPaSWAS's device backend named in
+ * its comment (paths relative to the pyPaSWAS source tree).
+ */
+#ifndef PYPASWAS_HIP_H
+#define PYPASWAS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PW_OK 0
+#define PW_EINVAL (-1)   /* bad argument or option  -> InvalidOptionException */
+#define PW_ENOMEM (-2)   /* device or host allocation failed */
+#define PW_EDEVICE (-3)  /* HIP runtime / kernel error -> HardwareException */
+
+typedef struct pw_ctx pw_ctx;
+
+/* Starting point record, 36 bytes, layout of the StartingPoint struct of
+ * pyPaSWAS/Core/ocl/default_main_cpu.cl:66-76 (parsed by
+ * pyPaSWAS/Core/StartingPoint.py:54-93).  sequence/target are indices into
+ * the call's query/target lists; blockX*8+valueX is the 0-based query
+ * position, blockY*8+valueY the 0-based target position. */
+typedef struct {
+    uint32_t sequence, target, blockX, blockY, valueX, valueY;
+    float score, maxScore, posScore;
+} pw_starting_point;
+
+/* One processed starting point of the host alignment walk
+ * (pyPaSWAS/Core/SmithWaterman.py:703-838), in walk order. */
+typedef struct {
+    int32_t query, target;   /* indices into the call's lists */
+    int32_t shown;           /* 1: the walk produced a hit (a Hit object is built) */
+    int32_t s_start, s_end, t_start, t_end;
+    int32_t matches, mismatches;
+    int32_t length;          /* alignment columns = entries in ops */
+    float score;             /* starting-point score (f32) */
+    int32_t sp_index;        /* index into starting_points */
+    int64_t ops_offset;      /* into ops: 'D' pair column, 'L' query char over '-', 'U' '-' over target char */
+} pw_hit;
+
+typedef struct {
+    int64_t n_pairs;
+    const float *maxima;                 /* [n_pairs] pair maximum (globalMaxima of the last block) */
+    const int32_t *pair_query;           /* [n_pairs] */
+    const int32_t *pair_target;          /* [n_pairs] */
+    int64_t n_starting_points;
+    const pw_starting_point *starting_points; /* per pair, emission order */
+    int64_t n_hits;
+    const pw_hit *hits;                  /* per pair, walk order (score ascending, stable) */
+    int64_t n_ops;
+    const char *ops;
+    int64_t n_traceback_pairs;           /* pairs whose maximum passed the traceback gate */
+    int64_t sp_truncated;                /* starting points dropped by the per-pair cap */
+    double cells;                        /* sum over pairs of |q|*|t| (real lengths) */
+    double ms_score;                     /* device time of the scoring kernel(s), HIP events */
+    double ms_directions;                /* direction recompute of gated pairs */
+    double ms_traceback;                 /* start selection + marking */
+    double ms_walk;                      /* walk */
+    double ms_total;                     /* wall time of the call, host clock */
+    int64_t score_launches;
+} pw_result;
+
+/* SmithWatermanOcl._initialize_device (pyPaSWAS/Core/SmithWatermanOcl.py:116-129)
+ * and SmithWatermanCuda._initialize_device: select a device, create the
+ * stream and the persistent device state. */
+int pw_init(int device, pw_ctx **out);
+
+/* Release everything the context owns (SmithWatermanOcl._clear_memory, :137-147). */
+int pw_destroy(pw_ctx *ctx);
+
+/* SmithWatermanOcl._device_global_mem_size (:131-134). */
+int pw_device_mem_bytes(pw_ctx *ctx, uint64_t *bytes);
+
+/* Scoring constants, replacing the kernel-template substitution of
+ * pyPaSWAS/Core/PaSWAS.py:55-91 (default_score.cl).  subst is dim x dim,
+ * row = query residue index (char - char_offset), FP32.  gap_ext == 0 selects
+ * the linear kernels (Scores.py:58, SmithWaterman.py:149). */
+int pw_set_scoring(pw_ctx *ctx, const float *subst, int dim, int char_offset, float gap, float gap_ext,
+                   float lower_limit, float minimum_score);
+
+/* Align explicit (query, target) pairs; replaces one pass of
+ * SmithWaterman.align_sequences (pyPaSWAS/Core/SmithWaterman.py:441-560):
+ * copy_sequences, _calculate_score, _traceback_host and the walk of
+ * _print_alignments, with the per-pair minimum start score of
+ * SmithWatermanOcl._fill_max_possible_score (:302-307).
+ *   qres/qoff/qlen  nq query sequences (uppercase residues), tres/... nt targets
+ *   pq/pt           n_pairs pair indices into the two lists
+ *   min_start_score [n_pairs] f32 (maxPossibleScore of default_main_cpu.cl:412)
+ *   same_id         [n_pairs] 1 when the two record ids are equal (SmithWaterman.py:752), may be NULL
+ *   minimum         settings.minimum_score as a double (SmithWaterman.py:758)
+ *   sp_cap          starting points kept per pair (SmithWaterman.py:144, 683-685)
+ *   mode            0 = plain per-batch launch, 1 = batched persistent kernel
+ * The result is owned by the library until pw_free_result. */
+int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                   const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                   const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
+                   const uint8_t *same_id, double minimum, int32_t sp_cap, int32_t mode, pw_result **out);
+
+/* Score pass only: pair maxima, the quantity the traceback gate of
+ * default_main_cpu.cl:408-410 reads.  maxima_out is [n_pairs]. */
+int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                   const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                   const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
+                   double *ms_out);
+
+/* Free a result of pw_align_pairs. */
+int pw_free_result(pw_ctx *ctx, pw_result *res);
+
+/* Last error message of the context (or of pw_init when ctx is NULL). */
+const char *pw_last_error(pw_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYPASWAS_HIP_H */

@@ -1,0 +1,515 @@
+"""ORACLE — test infrastructure only.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+import this module, and only as the checker.  The product (pypaswas_amd) never
+imports it.
+
+Python side of the CPU restatement of pyPaSWAS's OpenCL-CPU path:
+
+* ctypes bindings of ``sw_oracle.c`` (schedule-faithful restatement of
+  ``pyPaSWAS/Core/ocl/default_main_cpu.cl`` and an independent serial one);
+* the scoring matrices restated from ``pyPaSWAS/Core/Scores.py:101-361``;
+* the per-pair minimum start score of ``SmithWatermanOcl.py:302-307``;
+* a pure-Python restatement of the host walk ``SmithWaterman.py:665-841``
+  (cross-checks ``oracle_walk`` in C);
+* ``HitList.append`` (``pyPaSWAS/Core/HitList.py:28-52``), the hit metrics of
+  ``Hit.set_scores`` (``Hit.py:91-119``), the filter
+  (``SmithWaterman.py:843-848``) and the TXT record
+  (``Formatters.py:19-39``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+# pyPaSWAS/Core/__init__.py:7-13
+UPPER_LEFT_DIRECTION = 1
+UPPER_DIRECTION = 2
+LEFT_DIRECTION = 3
+NO_DIRECTION = 32
+STOP_DIRECTION = 64
+IN_ALIGNMENT = 96
+
+# --------------------------------------------------------------------------
+# Scoring matrices (Scores.py)
+# --------------------------------------------------------------------------
+
+# Scores.py:216-241 (BLOSUM62, rows/cols 'A'..'Z')
+BLOSUM62 = [
+    [4, -2, 0, -2, -1, -2, 0, -2, -1, -1, -1, -1, -1, -2, -4, -1, -1, -1, 1, 0, -4, 0, -3, -1, -2, -1],
+    [-2, 4, -3, 4, 1, -3, -1, 0, -3, -3, 0, -4, -3, 4, -4, -2, 0, -1, 0, -1, -4, -3, -4, -1, -3, 0],
+    [0, -3, 9, -3, -4, -2, -3, -3, -1, -1, -3, -1, -1, -3, -4, -3, -3, -3, -1, -1, -4, -1, -2, -1, -2, -3],
+    [-2, 4, -3, 6, 2, -3, -1, -1, -3, -3, -1, -4, -3, 1, -4, -1, 0, -2, 0, -1, -4, -3, -4, -1, -3, 1],
+    [-1, 1, -4, 2, 5, -3, -2, 0, -3, -3, 1, -3, -2, 0, -4, -1, 2, 0, 0, -1, -4, -2, -3, -1, -2, 4],
+    [-2, -3, -2, -3, -3, 6, -3, -1, 0, 0, -3, 0, 0, -3, -4, -4, -3, -3, -2, -2, -4, -1, 1, -1, 3, -3],
+    [0, -1, -3, -1, -2, -3, 6, -2, -4, -4, -2, -4, -3, 0, -4, -2, -2, -2, 0, -2, -4, -3, -2, -1, -3, -2],
+    [-2, 0, -3, -1, 0, -1, -2, 8, -3, -3, -1, -3, -2, 1, -4, -2, 0, 0, -1, -2, -4, -3, -2, -1, 2, 0],
+    [-1, -3, -1, -3, -3, 0, -4, -3, 4, 3, -3, 2, 1, -3, -4, -3, -3, -3, -2, -1, -4, 3, -3, -1, -1, -3],
+    [-1, -3, -1, -3, -3, 0, -4, -3, 3, 3, -3, 3, 2, -3, -4, -3, -2, -2, -2, -1, -4, 2, -2, -1, -1, -3],
+    [-1, 0, -3, -1, 1, -3, -2, -1, -3, -3, 5, -2, -1, 0, -4, -1, 1, 2, 0, -1, -4, -2, -3, -1, -2, 1],
+    [-1, -4, -1, -4, -3, 0, -4, -3, 2, 3, -2, 4, 2, -3, -4, -3, -2, -2, -2, -1, -4, 1, -2, -1, -1, -3],
+    [-1, -3, -1, -3, -2, 0, -3, -2, 1, 2, -1, 2, 5, -2, -4, -2, 0, -1, -1, -1, -4, 1, -1, -1, -1, -1],
+    [-2, 4, -3, 1, 0, -3, 0, 1, -3, -3, 0, -3, -2, 6, -4, -2, 0, 0, 1, 0, -4, -3, -4, -1, -2, 0],
+    [-4] * 26,
+    [-1, -2, -3, -1, -1, -4, -2, -2, -3, -3, -1, -3, -2, -2, -4, 7, -1, -2, -1, -1, -4, -2, -4, -1, -3, -1],
+    [-1, 0, -3, 0, 2, -3, -2, 0, -3, -2, 1, -2, 0, 0, -4, -1, 5, 1, 0, -1, -4, -2, -2, -1, -1, 4],
+    [-1, -1, -3, -2, 0, -3, -2, 0, -3, -2, 2, -2, -1, 0, -4, -2, 1, 5, -1, -1, -4, -3, -3, -1, -2, 0],
+    [1, 0, -1, 0, 0, -2, 0, -1, -2, -2, 0, -2, -1, 1, -4, -1, 0, -1, 4, 1, -4, -2, -3, -1, -2, 0],
+    [0, -1, -1, -1, -1, -2, -2, -2, -1, -1, -1, -1, -1, 0, -4, -1, -1, -1, 1, 5, -4, 0, -2, -1, -2, -1],
+    [-4] * 26,
+    [0, -3, -1, -3, -2, -1, -3, -3, 3, 2, -2, 1, 1, -3, -4, -2, -2, -3, -2, 0, -4, 4, -3, -1, -1, -2],
+    [-3, -4, -2, -4, -3, 1, -2, -2, -3, -2, -3, -2, -1, -4, -4, -4, -2, -3, -3, -2, -4, -3, 11, -1, 2, -2],
+    [-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -4, -1, -1, -1, -1, -1, -4, -1, -1, -1, -1, -1],
+    [-2, -3, -2, -3, -2, 3, -3, 2, -1, -1, -2, -1, -1, -2, -4, -3, -1, -2, -2, -2, -4, -1, 2, -1, 7, -2],
+    [-1, 0, -3, 1, 4, -3, -2, 0, -3, -3, 1, -3, -1, 0, -4, -1, 4, 0, 0, -1, -4, -2, -2, -1, -2, 4],
+]
+
+
+def dna_rna_matrix(match, mismatch, other, any_):
+    """Scores.py:317-360: 26x26 over 'A'..'Z'."""
+    o = lambda c: ord(c) - ord("A")
+    ambi = {o("K"): [o("G"), o("T")], o("M"): [o("A"), o("C")], o("R"): [o("A"), o("G")],
+            o("Y"): [o("C"), o("T")], o("S"): [o("C"), o("G")], o("W"): [o("A"), o("T")]}
+    nuc = [o(c) for c in "ATCGU"]
+    n = o("N")
+    m = [[other] * 26 for _ in range(26)]
+    for r in range(26):
+        for c in range(26):
+            if r == c or (r in ambi and c in ambi[r]) or (c in ambi and r in ambi[c]):
+                m[r][c] = match
+            elif r == n or c == n:
+                m[r][c] = any_
+            elif r in nuc or c in nuc:
+                m[r][c] = mismatch
+    return m
+
+
+def basic_matrix(match, mismatch, dim=255):
+    """Scores.py:126-138: 255x255 with char offset '\\1'."""
+    return [[match if r == c else mismatch for c in range(dim)] for r in range(dim)]
+
+
+@dataclass
+class Params:
+    """Scoring constants as the kernel templates receive them (PaSWAS.py:55-91)."""
+    matrix: Sequence[Sequence]
+    char_offset: int = ord("A")
+    gap: float = -5.0
+    gap_ext: Optional[float] = None  # None -> linear kernels (Scores.py:58)
+    lower_limit: float = 1.0
+    minimum: float = 30.0
+    filter_factor: float = 0.2
+    highest: float = 5.0
+    relative_score: float = 0.0
+    base_score: float = 0.0
+    query_coverage: float = 0.0
+    query_identity: float = 0.0
+    dna: bool = True
+
+    def subst_f32(self):
+        # constant float scoringsMatrix[...] initialised from the decimal
+        # literals str(value) (Scores.py:87-98, default_score.cl:24)
+        return np.array([[float(v) for v in row] for row in self.matrix], dtype=np.float32)
+
+
+class _SwParams(ctypes.Structure):
+    _fields_ = [("subst", ctypes.POINTER(ctypes.c_float)), ("dim", ctypes.c_int),
+                ("char_offset", ctypes.c_int), ("gap", ctypes.c_float), ("gap_ext", ctypes.c_float),
+                ("affine", ctypes.c_int), ("lower_limit", ctypes.c_float), ("minimum", ctypes.c_float)]
+
+
+class SpRecord(ctypes.Structure):
+    """default_main_cpu.cl:66-76."""
+    _fields_ = [("sequence", ctypes.c_uint32), ("target", ctypes.c_uint32), ("blockX", ctypes.c_uint32),
+                ("blockY", ctypes.c_uint32), ("valueX", ctypes.c_uint32), ("valueY", ctypes.c_uint32),
+                ("score", ctypes.c_float), ("maxScore", ctypes.c_float), ("posScore", ctypes.c_float)]
+
+
+class _WalkResult(ctypes.Structure):
+    _fields_ = [("shown", ctypes.c_int32), ("s_start", ctypes.c_int32), ("s_end", ctypes.c_int32),
+                ("t_start", ctypes.c_int32), ("t_end", ctypes.c_int32), ("matches", ctypes.c_int32),
+                ("mismatches", ctypes.c_int32), ("length", ctypes.c_int32), ("ops_offset", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_sched_batch.restype = ctypes.c_long
+        L.oracle_sched_batch.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(_SwParams), ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+        L.oracle_serial_pair.restype = ctypes.c_long
+        L.oracle_serial_pair.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_float,
+                                         ctypes.POINTER(_SwParams), ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_long]
+        L.oracle_walk.restype = ctypes.c_long
+        L.oracle_walk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_long,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]
+        _lib = L
+    return _lib
+
+
+def _c_params(p: Params, keep: list) -> _SwParams:
+    sub = np.ascontiguousarray(p.subst_f32())
+    keep.append(sub)
+    cp = _SwParams()
+    cp.subst = sub.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    cp.dim = sub.shape[0]
+    cp.char_offset = p.char_offset
+    cp.gap = np.float32(p.gap)
+    cp.gap_ext = np.float32(p.gap_ext if p.gap_ext is not None else 0.0)
+    cp.affine = 1 if p.gap_ext is not None else 0
+    cp.lower_limit = np.float32(p.lower_limit)
+    cp.minimum = np.float32(p.minimum)
+    return cp
+
+
+def min_start_score(p: Params, qlen: int, tlen: int) -> np.float32:
+    """SmithWatermanOcl.py:302-307: highest * min(len) * filter_factor in double, stored as f32."""
+    return np.float32(float(p.highest) * (qlen if qlen < tlen else tlen) * float(p.filter_factor))
+
+
+def pad8(n: int) -> int:
+    """SmithWaterman.py:325, 459-461: round up to a multiple of SHARED_X/Y = 8."""
+    return int(-(-n // 8) * 8)
+
+
+# --------------------------------------------------------------------------
+# Batch restatement
+# --------------------------------------------------------------------------
+
+@dataclass
+class BatchResult:
+    maxima: np.ndarray        # [N][T] f32
+    directions: np.ndarray    # [N][T][X][Y] u8, after the traceback
+    starting_points: List[SpRecord]  # pair-major, canonical emission order
+    X: int
+    Y: int
+    n_sp_total: int
+
+
+def sched_batch(queries: Sequence[str], targets: Sequence[str], p: Params, nthreads: int = 0,
+                sp_cap: int = 1 << 20) -> BatchResult:
+    """One SmithWatermanCPU batch: all queries x all targets (SmithWaterman.py:441-540)."""
+    N, T = len(queries), len(targets)
+    X = pad8(max(len(q) for q in queries))
+    Y = pad8(max(len(t) for t in targets))
+    seqs = b"".join(q.upper().encode().ljust(X, b"\0") for q in queries)
+    tgts = b"".join(t.upper().encode().ljust(Y, b"\0") for t in targets)
+    minsc = np.zeros(T * N, dtype=np.float32)
+    for ti, t in enumerate(targets):
+        for qi, q in enumerate(queries):
+            minsc[ti * N + qi] = min_start_score(p, len(q), len(t))
+    keep: list = []
+    cp = _c_params(p, keep)
+    maxima = np.zeros(N * T, dtype=np.float32)
+    dirs = np.zeros(N * T * X * Y, dtype=np.uint8)
+    sps = (SpRecord * sp_cap)()
+    n = lib().oracle_sched_batch(seqs, N, X, tgts, T, Y, minsc.ctypes.data, ctypes.byref(cp), nthreads,
+                                 maxima.ctypes.data, dirs.ctypes.data, ctypes.addressof(sps), sp_cap)
+    if n < 0:
+        raise MemoryError("oracle_sched_batch: allocation failed")
+    return BatchResult(maxima.reshape(N, T), dirs.reshape(N, T, X, Y), [sps[i] for i in range(min(n, sp_cap))],
+                       X, Y, n)
+
+
+def serial_pair(q: str, t: str, p: Params, mp: Optional[float] = None, sp_cap: int = 1 << 16):
+    """Independent serial restatement for one pair (no padding)."""
+    q = q.upper(); t = t.upper()
+    if mp is None:
+        mp = min_start_score(p, len(q), len(t))
+    keep: list = []
+    cp = _c_params(p, keep)
+    mx = ctypes.c_float()
+    dirs = np.zeros(len(q) * len(t), dtype=np.uint8)
+    sps = (SpRecord * sp_cap)()
+    n = lib().oracle_serial_pair(q.encode(), len(q), t.encode(), len(t), np.float32(mp), ctypes.byref(cp),
+                                 ctypes.addressof(mx), dirs.ctypes.data, ctypes.addressof(sps), sp_cap)
+    if n < 0:
+        raise MemoryError("oracle_serial_pair: allocation failed")
+    return float(mx.value), dirs.reshape(len(q), len(t)), [sps[i] for i in range(min(n, sp_cap))]
+
+
+# --------------------------------------------------------------------------
+# Host walk
+# --------------------------------------------------------------------------
+
+@dataclass
+class WalkHit:
+    qi: int
+    ti: int
+    shown: bool
+    s_start: int
+    s_end: int
+    t_start: int
+    t_end: int
+    score: float
+    matches: int
+    mismatches: int
+    sequence_match: str = ""
+    alignment: str = ""
+    target_match: str = ""
+
+
+def walk_py(directions: np.ndarray, sps: Sequence[SpRecord], queries: Sequence[str], targets: Sequence[str],
+            qids: Sequence[str], tids: Sequence[str], minimum: float) -> List[WalkHit]:
+    """SmithWaterman._print_alignments (SmithWaterman.py:689-841) in plain Python.
+
+    ``directions`` is [N][T][X][Y] and is modified in place (IN_ALIGNMENT marks).
+    Returns one WalkHit per starting point in processing order.
+    """
+    order = sorted(range(len(sps)), key=lambda i: float(sps[i].score))  # stable, :699
+    out = []
+    for i in order:
+        sp = sps[i]
+        qi, ti = sp.sequence, sp.target
+        D = directions[qi][ti]
+        bx, by, vx, vy = int(sp.blockX), int(sp.blockY), int(sp.valueX), int(sp.valueY)
+        s_end = bx * 8 + vx
+        t_end = by * 8 + vy
+        s_start, t_start = s_end + 1, t_end + 1
+        q, t = queries[qi].upper(), targets[ti].upper()
+        direction = int(D[bx * 8 + vx][by * 8 + vy])
+        show = True
+        if qids[qi] == tids[ti]:
+            direction = STOP_DIRECTION
+            show = False
+        if float(sp.score) < float(minimum):
+            show = False
+        seq, aln, tgt = [], [], []
+        matches = mismatches = 0
+        while (show and bx >= 0 and by >= 0 and vx >= 0 and vy >= 0 and direction != STOP_DIRECTION
+               and direction != NO_DIRECTION):
+            direction = int(D[bx * 8 + vx][by * 8 + vy])
+            D[bx * 8 + vx][by * 8 + vy] = IN_ALIGNMENT
+            if direction == IN_ALIGNMENT:
+                show = False
+            elif direction in (UPPER_LEFT_DIRECTION, STOP_DIRECTION):
+                a, b = q[bx * 8 + vx], t[by * 8 + vy]
+                tgt.append(b); seq.append(a)
+                eq = a.lower() == b.lower()
+                aln.append("|" if eq else ".")
+                matches += eq; mismatches += (not eq)
+                s_start -= 1; t_start -= 1
+                if direction == UPPER_LEFT_DIRECTION:
+                    if vx == 0:
+                        bx -= 1; vx = 7
+                    else:
+                        vx -= 1
+                    if vy == 0:
+                        by -= 1; vy = 7
+                    else:
+                        vy -= 1
+            elif direction == LEFT_DIRECTION:
+                tgt.append("-"); seq.append(q[bx * 8 + vx]); aln.append(" ")
+                s_start -= 1
+                if vx == 0:
+                    bx -= 1; vx = 7
+                else:
+                    vx -= 1
+            elif direction == UPPER_DIRECTION:
+                tgt.append(t[by * 8 + vy]); seq.append("-"); aln.append(" ")
+                t_start -= 1
+                if vy == 0:
+                    by -= 1; vy = 7
+                else:
+                    vy -= 1
+            else:
+                bx = -1
+        out.append(WalkHit(qi, ti, show, s_start, s_end, t_start, t_end, float(sp.score), matches, mismatches,
+                           "".join(reversed(seq)), "".join(reversed(aln)), "".join(reversed(tgt))))
+    return out
+
+
+def walk_c(directions: np.ndarray, sps: Sequence[SpRecord], queries: Sequence[str], targets: Sequence[str],
+           qids: Sequence[str], tids: Sequence[str], minimum: float) -> List[WalkHit]:
+    """Same walk through ``oracle_walk`` (C)."""
+    N, T, X, Y = directions.shape
+    nsp = len(sps)
+    arr = (SpRecord * max(nsp, 1))(*sps)
+    pair_of = np.array([sp.sequence * T + sp.target for sp in sps] or [0], dtype=np.int32)
+    same = np.array([1 if qids[i // T] == tids[i % T] else 0 for i in range(N * T)], dtype=np.uint8)
+    qenc = [queries[sp.sequence].upper().encode() for sp in sps]
+    tenc = [targets[sp.target].upper().encode() for sp in sps]
+    qarr = (ctypes.c_char_p * max(nsp, 1))(*qenc)
+    tarr = (ctypes.c_char_p * max(nsp, 1))(*tenc)
+    res = (_WalkResult * max(nsp, 1))()
+    order = np.zeros(max(nsp, 1), dtype=np.int64)
+    cap = sum(len(queries[sp.sequence]) + len(targets[sp.target]) for sp in sps) + 1
+    ops = ctypes.create_string_buffer(cap)
+    D = np.ascontiguousarray(directions)
+    lib().oracle_walk(D.ctypes.data, X, Y, ctypes.addressof(arr), nsp, pair_of.ctypes.data, same.ctypes.data,
+                      float(minimum), qarr, tarr, ctypes.addressof(res), order.ctypes.data, ops, cap)
+    directions[...] = D
+    out = []
+    raw = ops.raw
+    for s in range(nsp):
+        r = res[s]
+        sp = sps[int(order[s])]
+        q, t = queries[sp.sequence].upper(), targets[sp.target].upper()
+        seq, aln, tgt = [], [], []
+        if r.shown:
+            x = r.s_start
+            y = r.t_start
+            for k in range(r.length):
+                op = chr(raw[r.ops_offset + k])
+                if op == "D":
+                    a, b = q[x], t[y]
+                    seq.append(a); tgt.append(b); aln.append("|" if a.lower() == b.lower() else ".")
+                    x += 1; y += 1
+                elif op == "L":
+                    seq.append(q[x]); tgt.append("-"); aln.append(" "); x += 1
+                else:
+                    seq.append("-"); tgt.append(t[y]); aln.append(" "); y += 1
+        out.append(WalkHit(sp.sequence, sp.target, bool(r.shown), r.s_start, r.s_end, r.t_start, r.t_end,
+                           float(sp.score), r.matches, r.mismatches, "".join(seq), "".join(aln), "".join(tgt)))
+    return out
+
+
+# --------------------------------------------------------------------------
+# Hit list and TXT records
+# --------------------------------------------------------------------------
+
+class InvalidLocation(Exception):
+    """Hit._set_seq_location / _set_target_location raise CudaException (Hit.py:68-89)."""
+
+
+@dataclass
+class OHit:
+    qid: str
+    tid: str
+    qlen: int
+    tlen: int
+    s: tuple
+    t: tuple
+    score: float
+    matches: int
+    mismatches: int
+    sequence_match: str
+    alignment: str
+    target_match: str
+    relative_score: float = 0.0
+    query_identity: float = 0.0
+    query_coverage: float = 0.0
+    base_score: float = 0.0
+
+    def keys(self):
+        return [(self.qid, self.tid, self.s[0], self.t[0]), (self.qid, self.tid, self.s[1], self.t[1]),
+                (self.qid, self.tid, self.s[0], self.t[0])]
+
+
+def _valid(loc, n):
+    return 0 <= loc[0] < n and 0 <= loc[1] < n and loc[0] < loc[1]
+
+
+def make_hit(w: WalkHit, qid: str, tid: str, qlen: int, tlen: int) -> OHit:
+    s, t = (w.s_start, w.s_end), (w.t_start, w.t_end)
+    if not _valid(s, qlen) or not _valid(t, tlen):
+        raise InvalidLocation((s, t))
+    h = OHit(qid, tid, qlen, tlen, s, t, w.score, w.matches, w.mismatches, w.sequence_match, w.alignment,
+             w.target_match)
+    # Hit.set_scores, Hit.py:111-119
+    h.relative_score = float(h.score) / float(tlen) if qlen > tlen else float(h.score) / float(qlen)
+    h.query_identity = float(w.matches) / float(s[1] - s[0] + 1)
+    h.query_coverage = float(s[1] - s[0] + 1) / float(qlen)
+    h.base_score = float(h.score) / float(len(h.alignment))
+    return h
+
+
+def passes_filter(h: OHit, p: Params) -> bool:
+    """SmithWaterman._filter_hit, SmithWaterman.py:843-848."""
+    return (h.relative_score >= float(p.relative_score) and h.query_identity >= float(p.query_identity)
+            and h.query_coverage >= float(p.query_coverage) and h.base_score >= float(p.base_score))
+
+
+class OHitList:
+    """HitList.append, HitList.py:28-52 (logging removed)."""
+
+    def __init__(self):
+        self.hits = {}
+        self.real_hits = {}
+
+    def append(self, hit):
+        keys = hit.keys()
+        keys0 = keys[0] in self.hits and self.hits[keys[0]].score < hit.score
+        keys1 = keys[1] in self.hits and self.hits[keys[1]].score < hit.score
+        neither = keys[0] not in self.hits and keys[1] not in self.hits
+        if keys0 or keys1 or neither:
+            if keys1 and self.hits[keys[1]].keys()[2] in self.real_hits:
+                del self.real_hits[self.hits[keys[1]].keys()[2]]
+            self.hits[keys[0]] = hit
+            self.hits[keys[1]] = hit
+            self.real_hits[keys[2]] = hit
+
+
+def txt_records(hitlist: OHitList) -> str:
+    """DefaultFormatter, Formatters.py:19-47 (distance is None for FASTA/GenBank records)."""
+    lines = []
+    for h in sorted(hitlist.real_hits.values(), key=lambda h: (h.qid, h.tid, h.score)):
+        n = len(h.alignment)
+        rec = ", ".join([h.qid, h.tid, str(h.s[0]), str(h.s[1]), str(h.t[0]), str(h.t[1]), str(h.score),
+                         str(h.matches), str(h.mismatches), str(n - h.matches - h.mismatches), str(n),
+                         str(h.score / n), str(h.qlen), str(h.tlen), str(h.score / h.qlen), str(h.score / h.tlen),
+                         "None"])
+        lines.append("\n".join([rec, h.sequence_match, h.alignment, h.target_match]))
+    return "".join(l + "\n" for l in lines)
+
+
+def reverse_complement(s: str) -> str:
+    """IUPAC reverse complement (Biopython Seq.reverse_complement on DNA letters)."""
+    comp = {"A": "T", "C": "G", "G": "C", "T": "A", "U": "A", "N": "N", "R": "Y", "Y": "R", "S": "S", "W": "W",
+            "K": "M", "M": "K", "B": "V", "V": "B", "D": "H", "H": "D", "X": "X"}
+    return "".join(comp.get(c, c) for c in reversed(s.upper()))
+
+
+def align_all(records_q, records_t, p: Params, walk=walk_c, nthreads: int = 0):
+    """All queries x all targets in one oracle batch, then walk, filter and HitList.
+
+    records_* are (id, sequence) lists in the order the reference would hold
+    them (sorted by length, descending, stable; DNA targets with their _RC
+    copies appended before sorting, pypaswasall.py:98-124).  Returns the
+    OHitList; raises InvalidLocation where the reference raises CudaException.
+    """
+    queries = [s for _, s in records_q]
+    targets = [s for _, s in records_t]
+    br = sched_batch(queries, targets, p, nthreads=nthreads)
+    ws = walk(br.directions, br.starting_points, queries, targets, [i for i, _ in records_q],
+              [i for i, _ in records_t], p.minimum)
+    hl = OHitList()
+    for w in ws:
+        if not w.shown:
+            continue
+        h = make_hit(w, records_q[w.qi][0], records_t[w.ti][0], len(queries[w.qi]), len(targets[w.ti]))
+        if passes_filter(h, p):
+            hl.append(h)
+    return hl, br, ws
+
+
+def prepare_records(queries, targets, dna: bool):
+    """Readers.py:30-118 + pypaswasall.py:98-124: RC append for DNA targets, stable sort by length desc."""
+    q = sorted(queries, key=lambda r: len(r[1]), reverse=True)
+    t = list(targets)
+    if dna:
+        t = t + [(i + "_RC", reverse_complement(s)) for i, s in t]
+    t = sorted(t, key=lambda r: len(r[1]), reverse=True)
+    return q, t

@@ -1,0 +1,239 @@
+"""ctypes binding of libpypaswas_hip.so (C ABI declared in include/pypaswas_hip.h).
+
+There is no CPU fallback: if the library is missing or no HIP device is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from pypaswas_amd.Core.Exceptions import HardwareException, InvalidOptionException
+
+LIB_NAME = "libpypaswas_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+PW_OK = 0
+PW_EINVAL = -1
+PW_ENOMEM = -2
+PW_EDEVICE = -3
+
+# exported entry points, in the order include/pypaswas_hip.h declares them
+EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_score_pairs",
+           "pw_free_result", "pw_last_error")
+
+
+class StartingPointRec(ctypes.Structure):
+    """36-byte StartingPoint record (default_main_cpu.cl:66-76)."""
+    _fields_ = [("sequence", ctypes.c_uint32), ("target", ctypes.c_uint32), ("blockX", ctypes.c_uint32),
+                ("blockY", ctypes.c_uint32), ("valueX", ctypes.c_uint32), ("valueY", ctypes.c_uint32),
+                ("score", ctypes.c_float), ("maxScore", ctypes.c_float), ("posScore", ctypes.c_float)]
+
+
+class HitRec(ctypes.Structure):
+    _fields_ = [("query", ctypes.c_int32), ("target", ctypes.c_int32), ("shown", ctypes.c_int32),
+                ("s_start", ctypes.c_int32), ("s_end", ctypes.c_int32), ("t_start", ctypes.c_int32),
+                ("t_end", ctypes.c_int32), ("matches", ctypes.c_int32), ("mismatches", ctypes.c_int32),
+                ("length", ctypes.c_int32), ("score", ctypes.c_float), ("sp_index", ctypes.c_int32),
+                ("ops_offset", ctypes.c_int64)]
+
+
+class ResultRec(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_int64), ("maxima", ctypes.POINTER(ctypes.c_float)),
+                ("pair_query", ctypes.POINTER(ctypes.c_int32)), ("pair_target", ctypes.POINTER(ctypes.c_int32)),
+                ("n_starting_points", ctypes.c_int64), ("starting_points", ctypes.POINTER(StartingPointRec)),
+                ("n_hits", ctypes.c_int64), ("hits", ctypes.POINTER(HitRec)), ("n_ops", ctypes.c_int64),
+                ("ops", ctypes.POINTER(ctypes.c_char)), ("n_traceback_pairs", ctypes.c_int64),
+                ("sp_truncated", ctypes.c_int64), ("cells", ctypes.c_double), ("ms_score", ctypes.c_double),
+                ("ms_directions", ctypes.c_double), ("ms_traceback", ctypes.c_double), ("ms_walk", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("score_launches", ctypes.c_int64)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Loads the HIP library; raises HardwareException when it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise HardwareException(
+                "{} is missing: build it with `python -m pypaswas_amd.csrc.build` "
+                "(there is no CPU fallback)".format(LIB_PATH))
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.pw_init.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.pw_destroy.argtypes = [vp]
+        L.pw_device_mem_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.pw_set_scoring.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.pw_align_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
+                                     ctypes.c_int64, vp, vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.POINTER(ctypes.POINTER(ResultRec))]
+        L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
+                                     ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
+        L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
+        L.pw_last_error.argtypes = [vp]
+        L.pw_last_error.restype = ctypes.c_char_p
+        for name in EXPORTS:
+            if name != "pw_last_error":
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+        return L
+
+
+@dataclass
+class PackedSeqs:
+    data: np.ndarray    # uint8 residues
+    off: np.ndarray     # int64
+    length: np.ndarray  # int32
+
+    @staticmethod
+    def pack(seqs: Sequence[str]) -> "PackedSeqs":
+        enc = [s.encode("latin-1") for s in seqs]
+        length = np.array([len(e) for e in enc], dtype=np.int32)
+        off = np.zeros(len(enc), dtype=np.int64)
+        if len(enc) > 1:
+            off[1:] = np.cumsum(length[:-1], dtype=np.int64)
+        data = np.frombuffer(b"".join(enc) + b"\0", dtype=np.uint8).copy()
+        return PackedSeqs(data, off, length)
+
+
+@dataclass
+class AlignResult:
+    maxima: np.ndarray          # [n_pairs] f32
+    pair_query: np.ndarray
+    pair_target: np.ndarray
+    starting_points: np.ndarray  # structured, 36-byte records
+    hits: np.ndarray             # structured
+    ops: bytes
+    n_traceback_pairs: int
+    sp_truncated: int
+    cells: float
+    ms_score: float
+    ms_directions: float
+    ms_traceback: float
+    ms_walk: float
+    ms_total: float
+    score_launches: int
+
+
+SP_DTYPE = np.dtype([("sequence", "<u4"), ("target", "<u4"), ("blockX", "<u4"), ("blockY", "<u4"),
+                     ("valueX", "<u4"), ("valueY", "<u4"), ("score", "<f4"), ("maxScore", "<f4"),
+                     ("posScore", "<f4")])
+HIT_DTYPE = np.dtype([("query", "<i4"), ("target", "<i4"), ("shown", "<i4"), ("s_start", "<i4"), ("s_end", "<i4"),
+                      ("t_start", "<i4"), ("t_end", "<i4"), ("matches", "<i4"), ("mismatches", "<i4"),
+                      ("length", "<i4"), ("score", "<f4"), ("sp_index", "<i4"), ("ops_offset", "<i8")])
+assert SP_DTYPE.itemsize == ctypes.sizeof(StartingPointRec) == 36
+assert HIT_DTYPE.itemsize == ctypes.sizeof(HitRec)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class Device:
+    """One library context on one GPU (single host thread, like the reference)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.pw_init(int(device), ctypes.byref(self.ctx))
+        if rc != PW_OK:
+            msg = self.lib.pw_last_error(None).decode(errors="replace")
+            if rc == PW_EINVAL:
+                raise InvalidOptionException(msg)
+            raise HardwareException("HIP device {} unavailable: {}".format(device, msg))
+
+    def _check(self, rc: int, what: str):
+        if rc == PW_OK:
+            return
+        msg = "{}: {}".format(what, self.lib.pw_last_error(self.ctx).decode(errors="replace"))
+        if rc == PW_EINVAL:
+            raise InvalidOptionException(msg)
+        if rc == PW_ENOMEM:
+            raise MemoryError(msg)
+        raise HardwareException(msg)
+
+    def close(self):
+        if self.ctx:
+            self.lib.pw_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def global_mem_size(self) -> int:
+        v = ctypes.c_uint64()
+        self._check(self.lib.pw_device_mem_bytes(self.ctx, ctypes.byref(v)), "pw_device_mem_bytes")
+        return int(v.value)
+
+    def set_scoring(self, subst: np.ndarray, char_offset: int, gap: float, gap_ext: float, lower_limit: float,
+                    minimum_score: float):
+        m = np.ascontiguousarray(subst, dtype=np.float32)
+        self._keep_subst = m
+        self._check(self.lib.pw_set_scoring(self.ctx, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                            int(m.shape[0]), int(char_offset), float(np.float32(gap)),
+                                            float(np.float32(gap_ext)), float(np.float32(lower_limit)),
+                                            float(np.float32(minimum_score))), "pw_set_scoring")
+
+    def align_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
+                    min_start_score: np.ndarray, same_id: Optional[np.ndarray], minimum: float,
+                    sp_cap: int = 100, mode: int = 1) -> AlignResult:
+        pq = np.ascontiguousarray(pq, dtype=np.int32)
+        pt = np.ascontiguousarray(pt, dtype=np.int32)
+        mss = np.ascontiguousarray(min_start_score, dtype=np.float32)
+        sid = None if same_id is None else np.ascontiguousarray(same_id, dtype=np.uint8)
+        out = ctypes.POINTER(ResultRec)()
+        rc = self.lib.pw_align_pairs(self.ctx, _ptr(queries.data), _ptr(queries.off), _ptr(queries.length),
+                                     len(queries.length), _ptr(targets.data), _ptr(targets.off),
+                                     _ptr(targets.length), len(targets.length), _ptr(pq), _ptr(pt), len(pq),
+                                     _ptr(mss), _ptr(sid), float(minimum), int(sp_cap), int(mode),
+                                     ctypes.byref(out))
+        self._check(rc, "pw_align_pairs")
+        try:
+            r = out.contents
+            n = int(r.n_pairs)
+            maxima = np.ctypeslib.as_array(r.maxima, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+            pqa = np.ctypeslib.as_array(r.pair_query, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+            pta = np.ctypeslib.as_array(r.pair_target, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+            nsp = int(r.n_starting_points)
+            sps = (np.frombuffer(ctypes.string_at(r.starting_points, nsp * SP_DTYPE.itemsize), dtype=SP_DTYPE).copy()
+                   if nsp else np.zeros(0, SP_DTYPE))
+            nh = int(r.n_hits)
+            hits = (np.frombuffer(ctypes.string_at(r.hits, nh * HIT_DTYPE.itemsize), dtype=HIT_DTYPE).copy()
+                    if nh else np.zeros(0, HIT_DTYPE))
+            ops = ctypes.string_at(r.ops, int(r.n_ops)) if r.n_ops else b""
+            return AlignResult(maxima, pqa, pta, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
+                               float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
+                               float(r.ms_walk), float(r.ms_total), int(r.score_launches))
+        finally:
+            self.lib.pw_free_result(self.ctx, out)
+
+    def score_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
+                    mode: int = 1):
+        pq = np.ascontiguousarray(pq, dtype=np.int32)
+        pt = np.ascontiguousarray(pt, dtype=np.int32)
+        out = np.zeros(len(pq), dtype=np.float32)
+        ms = ctypes.c_double()
+        rc = self.lib.pw_score_pairs(self.ctx, _ptr(queries.data), _ptr(queries.off), _ptr(queries.length),
+                                     len(queries.length), _ptr(targets.data), _ptr(targets.off),
+                                     _ptr(targets.length), len(targets.length), _ptr(pq), _ptr(pt), len(pq),
+                                     int(mode), _ptr(out), ctypes.byref(ms))
+        self._check(rc, "pw_score_pairs")
+        return out, float(ms.value)

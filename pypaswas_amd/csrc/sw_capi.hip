@@ -1,0 +1,783 @@
+// C ABI of the gfx950 Smith-Waterman library (declared in
+// include/pypaswas_hip.h).  Host orchestration of one call:
+//   1. encode residues into a compact alphabet, upload sequences and the two
+//      transposed substitution tables;
+//   2. scoring pass over all pairs (plain: padded shapes, static schedule;
+//      batched: exact shapes, per-pair tile shape, persistent workers pulling
+//      from a device counter, longest pairs first);
+//   3. the traceback gate of default_main_cpu.cl:408-410 on the pair maxima;
+//   4. direction pass over gated pairs, start selection + marking, walk;
+//   5. gather results into one host record.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/pypaswas_hip.h"
+#include "sw_device.hip"
+
+using namespace pw;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct pw_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[10] = {};
+    std::string err;
+    bool scoring_set = false;
+    std::vector<float> subst;
+    int dim = 0, char_offset = 0;
+    float gap = 0, ext = 0, ll = 1, minimum = 0;
+    bool affine = false;
+    DevBuf res, seqs, items, trans, tables, scratch, queue, maxima, ditems, dirs, ccount, cands, tbitems, sps,
+        spcount, keys, hits, ops, opsused;
+};
+
+static thread_local std::string g_init_err;
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return PW_EDEVICE;                                                              \
+        }                                                                                   \
+    } while (0)
+
+static int ensure(pw_ctx *ctx, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return PW_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipFree: ") + hipGetErrorString(e);
+            return PW_EDEVICE;
+        }
+    }
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        ctx->err = "hipMalloc of " + std::to_string(want) + " bytes: " + hipGetErrorString(e);
+        return PW_ENOMEM;
+    }
+    b.cap = want;
+    return PW_OK;
+}
+
+#define ENSURE(ctx, buf, bytes)                      \
+    do {                                             \
+        int rc_ = ensure((ctx), (buf), (bytes));     \
+        if (rc_ != PW_OK) return rc_;                \
+    } while (0)
+
+extern "C" int pw_init(int device, pw_ctx **out) {
+    if (!out) {
+        g_init_err = "pw_init: out is NULL";
+        return PW_EINVAL;
+    }
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        g_init_err = std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "count is 0");
+        return PW_EDEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_init_err = "device " + std::to_string(device) + " out of range (" + std::to_string(n) + " devices)";
+        return PW_EINVAL;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        g_init_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return PW_EDEVICE;
+    }
+    pw_ctx *ctx = new pw_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (ctx->num_cus <= 0) ctx->num_cus = 256;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_init_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete ctx;
+        return PW_EDEVICE;
+    }
+    for (auto &ev : ctx->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+            g_init_err = std::string("hipEventCreate: ") + hipGetErrorString(e);
+            delete ctx;
+            return PW_EDEVICE;
+        }
+    }
+    *out = ctx;
+    return PW_OK;
+}
+
+extern "C" int pw_destroy(pw_ctx *ctx) {
+    if (!ctx) return PW_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->res, &ctx->seqs, &ctx->items, &ctx->trans, &ctx->tables, &ctx->scratch, &ctx->queue,
+                      &ctx->maxima, &ctx->ditems, &ctx->dirs, &ctx->ccount, &ctx->cands, &ctx->tbitems, &ctx->sps,
+                      &ctx->spcount, &ctx->keys, &ctx->hits, &ctx->ops, &ctx->opsused};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PW_OK;
+}
+
+extern "C" const char *pw_last_error(pw_ctx *ctx) { return ctx ? ctx->err.c_str() : g_init_err.c_str(); }
+
+extern "C" int pw_device_mem_bytes(pw_ctx *ctx, uint64_t *bytes) {
+    if (!ctx || !bytes) return PW_EINVAL;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemGetInfo(&free_b, &total_b));
+    *bytes = total_b;
+    return PW_OK;
+}
+
+extern "C" int pw_set_scoring(pw_ctx *ctx, const float *subst, int dim, int char_offset, float gap, float gap_ext,
+                              float lower_limit, float minimum_score) {
+    if (!ctx) return PW_EINVAL;
+    if (!subst || dim <= 0 || dim > 255) {
+        ctx->err = "pw_set_scoring: substitution matrix must be dim x dim with 0 < dim <= 255";
+        return PW_EINVAL;
+    }
+    if (!(gap <= 0.0f) || !(gap_ext <= 0.0f)) {
+        ctx->err = "pw_set_scoring: gap and gap extension scores must be <= 0";
+        return PW_EINVAL;
+    }
+    if (!(gap > GAP_INIT) || !std::isfinite(lower_limit) || !std::isfinite(minimum_score)) {
+        ctx->err = "pw_set_scoring: scoring constants out of range";
+        return PW_EINVAL;
+    }
+    for (int i = 0; i < dim * dim; ++i)
+        if (!std::isfinite(subst[i]) || std::fabs(subst[i]) >= 1e9f) {
+            ctx->err = "pw_set_scoring: substitution scores must be finite and below 1e9 in magnitude";
+            return PW_EINVAL;
+        }
+    ctx->subst.assign(subst, subst + (size_t)dim * dim);
+    ctx->dim = dim;
+    ctx->char_offset = char_offset;
+    ctx->gap = gap;
+    ctx->ext = gap_ext;
+    ctx->affine = gap_ext != 0.0f;
+    ctx->ll = lower_limit;
+    ctx->minimum = minimum_score;
+    ctx->scoring_set = true;
+    return PW_OK;
+}
+
+namespace {
+
+struct Encoded {
+    int K = 0, KP = 8, lkp = 5;
+    std::vector<uint8_t> codes;     // all residues, queries then targets
+    std::vector<SeqRef> refs;       // nq + nt
+    std::vector<float> tables;      // 2 * KP * KP
+    int64_t max_len_q = 0, max_len_t = 0;
+};
+
+int encode(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+           const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, Encoded &E) {
+    int map[256];
+    for (int &m : map) m = -1;
+    std::vector<int> used;
+    auto scan = [&](const uint8_t *res, const int64_t *off, const int32_t *len, int32_t n) -> int {
+        for (int32_t i = 0; i < n; ++i) {
+            if (len[i] < 0 || off[i] < 0) {
+                ctx->err = "negative sequence length or offset";
+                return PW_EINVAL;
+            }
+            const uint8_t *p = res + off[i];
+            for (int32_t k = 0; k < len[i]; ++k) {
+                const int c = p[k];
+                if (map[c] < 0) {
+                    const int idx = c - ctx->char_offset;
+                    if (idx < 0 || idx >= ctx->dim) {
+                        char buf[160];
+                        snprintf(buf, sizeof buf,
+                                 "character %d ('%c') is outside the scoring alphabet (offset %d, dimension %d)", c,
+                                 (c >= 32 && c < 127) ? c : '?', ctx->char_offset, ctx->dim);
+                        ctx->err = buf;
+                        return PW_EINVAL;
+                    }
+                    map[c] = (int)used.size();
+                    used.push_back(c);
+                }
+            }
+        }
+        return PW_OK;
+    };
+    int rc = scan(qres, qoff, qlen, nq);
+    if (rc) return rc;
+    rc = scan(tres, toff, tlen, nt);
+    if (rc) return rc;
+    E.K = (int)used.size();
+    if (E.K > KPMAX - 1) {
+        ctx->err = "more than 63 distinct residues in one call";
+        return PW_EINVAL;
+    }
+    E.KP = 8;
+    while (E.KP < E.K + 1) E.KP *= 2;
+    E.lkp = 2;
+    for (int v = E.KP; v > 1; v >>= 1) ++E.lkp;  // log2(KP) + 2: byte shift of a table row
+    const int KP = E.KP, FILL = KP - 1;
+    E.tables.assign(2 * KP * KP, FILL_SCORE);
+    for (int a = 0; a < E.K; ++a)
+        for (int b = 0; b < E.K; ++b) {
+            const int ia = used[a] - ctx->char_offset, ib = used[b] - ctx->char_offset;
+            // table 0: A = query  -> entry [b][a] = S(query a, target b)
+            E.tables[b * KP + a] = ctx->subst[(size_t)ia * ctx->dim + ib];
+            // table 1: A = target -> entry [b][a] = S(query b, target a)
+            E.tables[KP * KP + b * KP + a] = ctx->subst[(size_t)ib * ctx->dim + ia];
+        }
+    (void)FILL;
+    int64_t total = 0;
+    for (int32_t i = 0; i < nq; ++i) total += qlen[i];
+    for (int32_t i = 0; i < nt; ++i) total += tlen[i];
+    E.codes.resize((size_t)total + 64);
+    E.refs.resize((size_t)nq + nt);
+    int64_t pos = 0;
+    for (int32_t i = 0; i < nq; ++i) {
+        E.refs[i] = SeqRef{pos, qlen[i], 0};
+        for (int32_t k = 0; k < qlen[i]; ++k) E.codes[pos++] = (uint8_t)map[qres[qoff[i] + k]];
+        E.max_len_q = std::max<int64_t>(E.max_len_q, qlen[i]);
+    }
+    for (int32_t i = 0; i < nt; ++i) {
+        E.refs[nq + i] = SeqRef{pos, tlen[i], 0};
+        for (int32_t k = 0; k < tlen[i]; ++k) E.codes[pos++] = (uint8_t)map[tres[toff[i] + k]];
+        E.max_len_t = std::max<int64_t>(E.max_len_t, tlen[i]);
+    }
+    return PW_OK;
+}
+
+inline int pad8(int n) { return (n + 7) / 8 * 8; }
+
+// Tile shape for the batched pass: orientation and residues per lane that
+// minimise the modelled cost stripes * (columns + 63) * (R + overhead).
+void choose_shape(int qlen, int tlen, int &trans, int &R) {
+    double best = 1e300;
+    trans = 0;
+    R = 4;
+    for (int o = 0; o < 2; ++o) {
+        const int a = o ? tlen : qlen, b = o ? qlen : tlen;
+        for (int r : {1, 2, 4, 8}) {
+            const int rows = 64 * r;
+            const int stripes = std::max(1, (a + rows - 1) / rows);
+            const double cost = (double)stripes * (b + 63) * (r + 1.5) + (stripes > 1 ? stripes * (b + 63) * 0.5 : 0);
+            if (cost < best) {
+                best = cost;
+                trans = o;
+                R = r;
+            }
+        }
+    }
+}
+
+int dir_R(int qlen) {
+    int r = 1;
+    while (r < 8 && 64 * r < qlen) r *= 2;
+    return r;
+}
+
+size_t lds_bytes(int KP) { return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE; }
+
+struct Prepared {
+    Encoded E;
+    int64_t max_b_it = 0;
+    int nitems = 0;
+};
+
+int upload_and_score(pw_ctx *ctx, Encoded &E, int32_t nq, const int32_t *qlen, const int32_t *tlen,
+                     const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, double *ms_score,
+                     int64_t *launches) {
+    const int KP = E.KP;
+    ENSURE(ctx, ctx->res, E.codes.size());
+    ENSURE(ctx, ctx->seqs, E.refs.size() * sizeof(SeqRef));
+    ENSURE(ctx, ctx->tables, E.tables.size() * sizeof(float));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->res.p, E.codes.data(), E.codes.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->seqs.p, E.refs.data(), E.refs.size() * sizeof(SeqRef), hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->tables.p, E.tables.data(), E.tables.size() * sizeof(float),
+                               hipMemcpyHostToDevice, ctx->stream));
+
+    std::vector<ScoreItem> items((size_t)n_pairs);
+    std::vector<uint8_t> trans((size_t)n_pairs, 0);
+    int64_t max_b = 0;
+    if (mode == 0) {
+        // the reference's batch shape: every pair padded to the longest query
+        // and target rounded up to 8 (SmithWaterman.py:459-461, :325)
+        const int X = pad8((int)E.max_len_q), Y = pad8((int)E.max_len_t);
+        for (int64_t i = 0; i < n_pairs; ++i)
+            items[i] = ScoreItem{pq[i], nq + pt[i], (int32_t)i, 4, X, Y};
+        max_b = Y;
+    } else {
+        std::vector<std::pair<double, int64_t>> order((size_t)n_pairs);
+        for (int64_t i = 0; i < n_pairs; ++i) {
+            const int ql = qlen[pq[i]], tl = tlen[pt[i]];
+            int tr, R;
+            choose_shape(ql, tl, tr, R);
+            const int a = tr ? tl : ql, b = tr ? ql : tl;
+            const int aseq = tr ? nq + pt[i] : pq[i], bseq = tr ? pq[i] : nq + pt[i];
+            items[i] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b};
+            trans[i] = (uint8_t)tr;
+            const int rows = 64 * R;
+            order[i] = {-(double)((a + rows - 1) / rows) * (b + 63) * (R + 1.5), i};
+            max_b = std::max<int64_t>(max_b, b);
+        }
+        // longest first for the persistent workers
+        std::stable_sort(order.begin(), order.end(),
+                         [](const std::pair<double, int64_t> &x, const std::pair<double, int64_t> &y) {
+                             return x.first < y.first;
+                         });
+        std::vector<ScoreItem> sorted((size_t)n_pairs);
+        std::vector<uint8_t> strans((size_t)n_pairs);
+        for (int64_t i = 0; i < n_pairs; ++i) {
+            sorted[i] = items[order[i].second];
+            strans[i] = trans[order[i].second];
+        }
+        items.swap(sorted);
+        trans.swap(strans);
+    }
+    ENSURE(ctx, ctx->items, items.size() * sizeof(ScoreItem));
+    ENSURE(ctx, ctx->trans, trans.size());
+    ENSURE(ctx, ctx->maxima, (size_t)n_pairs * sizeof(float));
+    ENSURE(ctx, ctx->queue, 64);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->items.p, items.data(), items.size() * sizeof(ScoreItem), hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->trans.p, trans.data(), trans.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64, ctx->stream));
+
+    const size_t lds = lds_bytes(KP);
+    const int64_t want = (int64_t)ctx->num_cus * 8;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_pairs, want));
+    const int64_t scr_stride = 3 * (max_b + 64);
+    ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float));
+    unsigned *queue = mode == 0 ? nullptr : (unsigned *)ctx->queue.p;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if (n_pairs > 0) {
+        if (ctx->affine)
+            hipLaunchKernelGGL(score_kernel<true>, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)ctx->res.p,
+                               (const SeqRef *)ctx->seqs.p, (const ScoreItem *)ctx->items.p, (int)n_pairs,
+                               (const uint8_t *)ctx->trans.p, (const float *)ctx->tables.p, KP, E.lkp, ctx->gap,
+                               ctx->ext, (float *)ctx->scratch.p, scr_stride, queue, (float *)ctx->maxima.p);
+        else
+            hipLaunchKernelGGL(score_kernel<false>, dim3(grid), dim3(64), lds, ctx->stream,
+                               (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
+                               (const ScoreItem *)ctx->items.p, (int)n_pairs, (const uint8_t *)ctx->trans.p,
+                               (const float *)ctx->tables.p, KP, E.lkp, ctx->gap, ctx->ext, (float *)ctx->scratch.p,
+                               scr_stride, queue, (float *)ctx->maxima.p);
+        HIPCHK(ctx, hipGetLastError());
+        if (launches) *launches += 1;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    float ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (ms_score) *ms_score += ms;
+    return PW_OK;
+}
+
+int check_pairs(pw_ctx *ctx, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
+    if (n_pairs < 0 || n_pairs > (int64_t)INT32_MAX) {
+        ctx->err = "n_pairs out of range";
+        return PW_EINVAL;
+    }
+    for (int64_t i = 0; i < n_pairs; ++i)
+        if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
+            ctx->err = "pair " + std::to_string(i) + " indexes outside the sequence lists";
+            return PW_EINVAL;
+        }
+    return PW_OK;
+}
+
+struct HostResult {
+    pw_result pub;
+    std::vector<float> maxima;
+    std::vector<int32_t> pq, pt;
+    std::vector<pw_starting_point> sps;
+    std::vector<pw_hit> hits;
+    std::vector<char> ops;
+};
+
+}  // namespace
+
+extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                              const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                              const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
+                              double *ms_out) {
+    if (!ctx) return PW_EINVAL;
+    if (!ctx->scoring_set) {
+        ctx->err = "pw_set_scoring has not been called";
+        return PW_EINVAL;
+    }
+    if ((n_pairs > 0 && (!pq || !pt || !maxima_out)) || nq < 0 || nt < 0) {
+        ctx->err = "pw_score_pairs: missing argument";
+        return PW_EINVAL;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = check_pairs(ctx, nq, nt, pq, pt, n_pairs);
+    if (rc) return rc;
+    Encoded E;
+    rc = encode(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, E);
+    if (rc) return rc;
+    double ms = 0;
+    rc = upload_and_score(ctx, E, nq, qlen, tlen, pq, pt, n_pairs, mode, &ms, nullptr);
+    if (rc) return rc;
+    if (n_pairs > 0)
+        HIPCHK(ctx, hipMemcpy(maxima_out, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost));
+    if (ms_out) *ms_out = ms;
+    return PW_OK;
+}
+
+extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                              const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                              const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
+                              const uint8_t *same_id, double minimum, int32_t sp_cap, int32_t mode, pw_result **out) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    if (!ctx) return PW_EINVAL;
+    if (!out) {
+        ctx->err = "pw_align_pairs: out is NULL";
+        return PW_EINVAL;
+    }
+    *out = nullptr;
+    if (!ctx->scoring_set) {
+        ctx->err = "pw_set_scoring has not been called";
+        return PW_EINVAL;
+    }
+    if ((n_pairs > 0 && (!pq || !pt || !min_start_score)) || nq < 0 || nt < 0) {
+        ctx->err = "pw_align_pairs: missing argument";
+        return PW_EINVAL;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = check_pairs(ctx, nq, nt, pq, pt, n_pairs);
+    if (rc) return rc;
+    Encoded E;
+    rc = encode(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, E);
+    if (rc) return rc;
+
+    HostResult *R = new HostResult();
+    std::memset(&R->pub, 0, sizeof(R->pub));
+    auto fail = [&](int code) {
+        delete R;
+        return code;
+    };
+    double ms_score = 0;
+    int64_t launches = 0;
+    rc = upload_and_score(ctx, E, nq, qlen, tlen, pq, pt, n_pairs, mode, &ms_score, &launches);
+    if (rc) return fail(rc);
+    R->maxima.resize((size_t)n_pairs);
+    R->pq.assign(pq, pq + n_pairs);
+    R->pt.assign(pt, pt + n_pairs);
+    if (n_pairs > 0) {
+        hipError_t e = hipMemcpy(R->maxima.data(), ctx->maxima.p, (size_t)n_pairs * sizeof(float),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipMemcpy maxima: ") + hipGetErrorString(e);
+            return fail(PW_EDEVICE);
+        }
+    }
+    double cells = 0;
+    for (int64_t i = 0; i < n_pairs; ++i) cells += (double)qlen[pq[i]] * (double)tlen[pt[i]];
+
+    // traceback gate: maximum >= MINIMUM_SCORE (default_main_cpu.cl:410); a
+    // start also needs a value >= the pair's minimum start score (:427)
+    std::vector<int64_t> gated;
+    for (int64_t i = 0; i < n_pairs; ++i) {
+        const float m = R->maxima[i];
+        if (m >= ctx->minimum && m >= min_start_score[i]) gated.push_back(i);
+    }
+    const int ng = (int)gated.size();
+    double ms_dir = 0, ms_tb = 0, ms_walk = 0;
+    std::vector<TbItem> tb(ng);
+    std::vector<int> spc(ng, 0);
+    std::vector<int64_t> opsused(ng, 0);
+    if (ng > 0) {
+        // --- direction pass ---
+        std::vector<DirItem> di(ng);
+        int64_t dir_total = 0, cand_total = 0;
+        const int CAND0 = 16;
+        int64_t max_t = 0;
+        for (int g = 0; g < ng; ++g) {
+            const int64_t p = gated[g];
+            const int ql = qlen[pq[p]], tl = tlen[pt[p]];
+            const int Rr = dir_R(ql);
+            const int rows = 64 * Rr, stripes = std::max(1, (ql + rows - 1) / rows);
+            DirItem d;
+            d.q_seq = pq[p];
+            d.t_seq = nq + pt[p];
+            d.pair = (int32_t)p;
+            d.R = Rr;
+            d.dir_off = dir_total;
+            d.t1 = ctx->ll * R->maxima[p];  // LOWER_LIMIT_SCORE * maximum, f32
+            d.mp = min_start_score[p];
+            d.cand_off = cand_total;
+            d.cand_cap = CAND0;
+            d.pad = 0;
+            di[g] = d;
+            dir_total += (int64_t)stripes * (tl + 63) * 64 * Rr;
+            dir_total = (dir_total + 15) & ~(int64_t)15;
+            cand_total += CAND0;
+            max_t = std::max<int64_t>(max_t, tl);
+        }
+        const size_t lds = lds_bytes(E.KP);
+        const int64_t scr_stride = 3 * (max_t + 64);
+        auto run_dir = [&](const std::vector<DirItem> &list, double &ms) -> int {
+            const int n = (int)list.size();
+            const int grid = std::max(1, std::min(n, ctx->num_cus * 8));
+            ENSURE(ctx, ctx->ditems, list.size() * sizeof(DirItem));
+            ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->ditems.p, list.data(), list.size() * sizeof(DirItem),
+                                       hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ctx->ccount.p, 0, (size_t)n * sizeof(int), ctx->stream));
+            HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+            if (ctx->affine)
+                hipLaunchKernelGGL(dir_kernel<true>, dim3(grid), dim3(64), lds, ctx->stream,
+                                   (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
+                                   (const DirItem *)ctx->ditems.p, n, (const float *)ctx->tables.p, E.KP, E.lkp,
+                                   ctx->gap, ctx->ext, (float *)ctx->scratch.p, scr_stride, (uint8_t *)ctx->dirs.p,
+                                   (int *)ctx->ccount.p, (Cand *)ctx->cands.p);
+            else
+                hipLaunchKernelGGL(dir_kernel<false>, dim3(grid), dim3(64), lds, ctx->stream,
+                                   (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
+                                   (const DirItem *)ctx->ditems.p, n, (const float *)ctx->tables.p, E.KP, E.lkp,
+                                   ctx->gap, ctx->ext, (float *)ctx->scratch.p, scr_stride, (uint8_t *)ctx->dirs.p,
+                                   (int *)ctx->ccount.p, (Cand *)ctx->cands.p);
+            HIPCHK(ctx, hipGetLastError());
+            HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+            HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
+            float t = 0;
+            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+            ms += t;
+            return PW_OK;
+        };
+        ENSURE(ctx, ctx->dirs, (size_t)dir_total);
+        ENSURE(ctx, ctx->ccount, (size_t)ng * sizeof(int));
+        ENSURE(ctx, ctx->cands, (size_t)cand_total * sizeof(Cand));
+        rc = run_dir(di, ms_dir);
+        if (rc) return fail(rc);
+        std::vector<int> cc(ng);
+        {
+            hipError_t e = hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                ctx->err = std::string("hipMemcpy candidate counts: ") + hipGetErrorString(e);
+                return fail(PW_EDEVICE);
+            }
+        }
+        // pairs whose candidates overflowed the first guess run again with the
+        // exact capacity; the recomputed directions are identical
+        std::vector<int> over;
+        for (int g = 0; g < ng; ++g)
+            if (cc[g] > di[g].cand_cap) over.push_back(g);
+        if (!over.empty()) {
+            int64_t ctot = 0;
+            for (int g = 0; g < ng; ++g) {
+                const int cap = std::max(cc[g], 1);
+                di[g].cand_off = ctot;
+                di[g].cand_cap = cap;
+                ctot += cap;
+            }
+            ENSURE(ctx, ctx->cands, (size_t)ctot * sizeof(Cand));
+            rc = run_dir(di, ms_dir);
+            if (rc) return fail(rc);
+            hipError_t e = hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                ctx->err = std::string("hipMemcpy candidate counts: ") + hipGetErrorString(e);
+                return fail(PW_EDEVICE);
+            }
+        }
+        // --- start selection + marking ---
+        int64_t sp_total = 0;
+        for (int g = 0; g < ng; ++g) {
+            const int64_t p = gated[g];
+            TbItem t;
+            t.pair = (int32_t)p;
+            t.q_seq = pq[p];
+            t.t_seq = nq + pt[p];
+            t.R = di[g].R;
+            t.dir_off = di[g].dir_off;
+            t.cand_off = di[g].cand_off;
+            t.sp_off = sp_total;
+            t.ops_off = 0;
+            t.hit_off = sp_total;
+            t.key_off = sp_total;
+            t.maxv = R->maxima[p];
+            t.mp = min_start_score[p];
+            t.qidx = pq[p];
+            t.tidx = pt[p];
+            t.same_id = same_id ? same_id[p] : 0;
+            t.ncand = std::min(cc[g], di[g].cand_cap);
+            tb[g] = t;
+            sp_total += t.ncand;
+        }
+        ENSURE(ctx, ctx->tbitems, (size_t)ng * sizeof(TbItem));
+        ENSURE(ctx, ctx->sps, (size_t)sp_total * sizeof(SpRec));
+        ENSURE(ctx, ctx->spcount, (size_t)ng * sizeof(int));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->tbitems.p, tb.data(), tb.size() * sizeof(TbItem), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        const int tb_grid = (ng + 63) / 64;
+        if (ctx->affine)
+            hipLaunchKernelGGL(tb_kernel<true>, dim3(tb_grid), dim3(64), 0, ctx->stream, (const SeqRef *)ctx->seqs.p,
+                               (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (Cand *)ctx->cands.p,
+                               (SpRec *)ctx->sps.p, (int *)ctx->spcount.p);
+        else
+            hipLaunchKernelGGL(tb_kernel<false>, dim3(tb_grid), dim3(64), 0, ctx->stream, (const SeqRef *)ctx->seqs.p,
+                               (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (Cand *)ctx->cands.p,
+                               (SpRec *)ctx->sps.p, (int *)ctx->spcount.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(spc.data(), ctx->spcount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        {
+            float t = 0;
+            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[4], ctx->ev[5]));
+            ms_tb += t;
+        }
+        // --- walk ---
+        int64_t ops_total = 0;
+        for (int g = 0; g < ng; ++g) {
+            const int64_t p = gated[g];
+            const int64_t ql = qlen[pq[p]], tl = tlen[pt[p]];
+            const int64_t cap = std::min<int64_t>((int64_t)spc[g] * (ql + tl), ql * tl + spc[g]);
+            tb[g].ops_off = ops_total;
+            ops_total += cap;
+        }
+        ENSURE(ctx, ctx->keys, (size_t)sp_total * sizeof(uint64_t));
+        ENSURE(ctx, ctx->hits, (size_t)sp_total * sizeof(HitRec));
+        ENSURE(ctx, ctx->ops, (size_t)ops_total + 1);
+        ENSURE(ctx, ctx->opsused, (size_t)ng * sizeof(int64_t));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->tbitems.p, tb.data(), tb.size() * sizeof(TbItem), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+        if (ctx->affine)
+            hipLaunchKernelGGL(walk_kernel<true>, dim3(tb_grid), dim3(64), 0, ctx->stream, (const uint8_t *)ctx->res.p,
+                               (const SeqRef *)ctx->seqs.p, (const TbItem *)ctx->tbitems.p, ng,
+                               (uint8_t *)ctx->dirs.p, (const SpRec *)ctx->sps.p, (const int *)ctx->spcount.p,
+                               (uint64_t *)ctx->keys.p, (HitRec *)ctx->hits.p, (char *)ctx->ops.p,
+                               (int64_t *)ctx->opsused.p, minimum);
+        else
+            hipLaunchKernelGGL(walk_kernel<false>, dim3(tb_grid), dim3(64), 0, ctx->stream,
+                               (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
+                               (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (const SpRec *)ctx->sps.p,
+                               (const int *)ctx->spcount.p, (uint64_t *)ctx->keys.p, (HitRec *)ctx->hits.p,
+                               (char *)ctx->ops.p, (int64_t *)ctx->opsused.p, minimum);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
+        // gather
+        std::vector<SpRec> hsp((size_t)sp_total);
+        std::vector<HitRec> hh((size_t)sp_total);
+        std::vector<char> hops((size_t)ops_total);
+        if (sp_total > 0) {
+            HIPCHK(ctx, hipMemcpyAsync(hsp.data(), ctx->sps.p, (size_t)sp_total * sizeof(SpRec),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(hh.data(), ctx->hits.p, (size_t)sp_total * sizeof(HitRec),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (ops_total > 0)
+            HIPCHK(ctx, hipMemcpyAsync(hops.data(), ctx->ops.p, (size_t)ops_total, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(opsused.data(), ctx->opsused.p, (size_t)ng * sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        {
+            float t = 0;
+            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
+            ms_walk += t;
+        }
+        // compact: per gated pair, its starting points (emission order), hits
+        // (walk order) and op characters
+        static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
+        for (int g = 0; g < ng; ++g) {
+            const int64_t sp0 = (int64_t)R->sps.size();
+            const int n = spc[g];
+            for (int k = 0; k < n; ++k) {
+                pw_starting_point s;
+                std::memcpy(&s, &hsp[tb[g].sp_off + k], sizeof s);
+                R->sps.push_back(s);
+            }
+            const int64_t op0 = (int64_t)R->ops.size();
+            R->ops.insert(R->ops.end(), hops.begin() + tb[g].ops_off, hops.begin() + tb[g].ops_off + opsused[g]);
+            for (int k = 0; k < n; ++k) {
+                const HitRec &hr = hh[tb[g].hit_off + k];
+                pw_hit h;
+                h.query = hr.query;
+                h.target = hr.target;
+                h.shown = hr.shown;
+                h.s_start = hr.s_start;
+                h.s_end = hr.s_end;
+                h.t_start = hr.t_start;
+                h.t_end = hr.t_end;
+                h.matches = hr.matches;
+                h.mismatches = hr.mismatches;
+                h.length = hr.length;
+                h.score = hr.score;
+                h.sp_index = (int32_t)(sp0 + hr.sp_index);
+                h.ops_offset = op0 + hr.ops_offset;
+                R->hits.push_back(h);
+            }
+        }
+    }
+    // the reference keeps at most 100 starting points per pair of a batch in
+    // total (SmithWaterman.py:683-685); report what a cap would drop
+    int64_t truncated = 0;
+    if (sp_cap > 0) {
+        const int64_t cap_total = (int64_t)sp_cap * n_pairs;
+        if ((int64_t)R->sps.size() >= cap_total) truncated = (int64_t)R->sps.size() - cap_total;
+    }
+    pw_result &P = R->pub;
+    P.n_pairs = n_pairs;
+    P.maxima = R->maxima.data();
+    P.pair_query = R->pq.data();
+    P.pair_target = R->pt.data();
+    P.n_starting_points = (int64_t)R->sps.size();
+    P.starting_points = R->sps.data();
+    P.n_hits = (int64_t)R->hits.size();
+    P.hits = R->hits.data();
+    P.n_ops = (int64_t)R->ops.size();
+    P.ops = R->ops.data();
+    P.n_traceback_pairs = ng;
+    P.sp_truncated = truncated;
+    P.cells = cells;
+    P.ms_score = ms_score;
+    P.ms_directions = ms_dir;
+    P.ms_traceback = ms_tb;
+    P.ms_walk = ms_walk;
+    P.score_launches = launches;
+    P.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    *out = &R->pub;
+    return PW_OK;
+}
+
+extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res) {
+    (void)ctx;
+    if (!res) return PW_EINVAL;
+    // pub is the first member of HostResult
+    delete reinterpret_cast<HostResult *>(res);
+    return PW_OK;
+}

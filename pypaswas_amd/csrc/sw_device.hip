@@ -1,0 +1,579 @@
+// gfx950 Smith-Waterman kernels: scoring pass, direction pass, start
+// selection + path marking, and the alignment walk.
+//
+// Semantics follow pyPaSWAS's OpenCL-CPU kernels
+// (pyPaSWAS/Core/ocl/default_main_cpu.cl) and its host walk
+// (pyPaSWAS/Core/SmithWaterman.py:665-841); the layout and schedule are this
+// library's own:
+//
+//  * one 64-lane wavefront per tile; lane k owns R consecutive residues of
+//    the tile sequence A (a stripe of 64*R rows) and sweeps the other
+//    sequence B column by column, lane k running one column behind lane k-1
+//    (anti-diagonal wavefront inside the wave);
+//  * the values a lane needs from the row above (H, or T = max(M, I, J) and
+//    the J candidate for affine gaps) arrive from lane k-1 through a DPP
+//    wave_shr:1 move; lane 0 receives the previous stripe's bottom row,
+//    staged 64 columns at a time through a per-wave scratch row;
+//  * the substitution matrix (transposed, KP x KP, FILL row/col -1e10) and a
+//    window of B's residue codes live in LDS;
+//  * all score arithmetic is IEEE FP32 adds and max, in the order that yields
+//    exactly the reference's values: max(a+s, b+s) == max(a, b)+s because FP32
+//    rounding is monotone, and negative gap-state values are never clamped to
+//    -1e10 because no output depends on a negative I or J.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sw_kernels.h"
+
+namespace pw {
+
+__device__ __forceinline__ float shr1(float old_lane0, float v) {
+    // DPP wave_shr:1 — lane k receives lane k-1's v; lane 0 keeps old_lane0.
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old_lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ float wrlane(float v, int l, float old) {
+    // v is wave-uniform; lane l takes it, the others keep old
+    return ((int)(threadIdx.x & 63) == l) ? v : old;
+}
+
+__device__ __forceinline__ float fmax3(float a, float b, float c) { return __builtin_fmaxf(__builtin_fmaxf(a, b), c); }
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = __builtin_fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float load_scr(const float *p) {
+    // the scratch row is rewritten in place by the same wave; bypass the L1
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// One tile job: A along lanes (stripes of 64*R rows), B swept.
+//   DIRS = false: returns the pair maximum (max over all cells of H, or of
+//                 max(M, I, J) for affine gaps), the globalMaxima value of
+//                 default_main_cpu.cl:188-220.
+//   DIRS = true : A must be the query; writes one direction byte per cell
+//                 (tile-major, see cell_offset) and appends candidate starts.
+// ---------------------------------------------------------------------------
+template <bool AFF, int R, bool DIRS>
+__device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int alen_it, int blen_it,
+                      const float *subT, int lkp, uint8_t *bwin, float *__restrict__ scr, float G, float E,
+                      int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
+                      Cand *__restrict__ cands, int cand_cap) {
+    const int lane = threadIdx.x & 63;
+    const int rows = WAVE * R;
+    const int nstripes = (alen_it + rows - 1) / rows;
+    const int nsteps = blen_it + WAVE - 1;
+    const char *subB = reinterpret_cast<const char *>(subT);
+    float rmax = 0.0f;
+    const float bJ = AFF ? G : 0.0f;  // J candidate entering row 1: max(-1e10 + E, 0 + G)
+
+    for (int s = 0; s < nstripes; ++s) {
+        int a4[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = s * rows + lane * R + r;
+            a4[r] = ((i < A.len) ? (int)res[A.off + i] : fill) * 4;
+        }
+        float Tp[R], Mp[R], Ip[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Tp[r] = 0.0f;
+            Mp[r] = 0.0f;
+            Ip[r] = GAP_INIT;
+        }
+        const bool above = s > 0;
+        const bool below = s + 1 < nstripes;
+        float inT = 0.0f, inJ = bJ, inM = 0.0f;  // staged previous-stripe row, lane l = column c0+l
+        float oT = 0.0f, oJ = 0.0f, oM = 0.0f;   // staged bottom row to hand down
+        if (above && lane < blen_it) {
+            inT = load_scr(&scr[3 * lane]);
+            if (AFF) inJ = load_scr(&scr[3 * lane + 1]);
+            if (DIRS && AFF) inM = load_scr(&scr[3 * lane + 2]);
+        }
+        // inputs of step 0: lane 0 reads column 0 of the row above; the other
+        // lanes start in the virtual columns left of the matrix
+        float upT = above ? rdlane(inT, 0) : 0.0f;
+        float upJ = above ? rdlane(inJ, 0) : bJ;
+        float upM = above ? rdlane(inM, 0) : 0.0f;
+        if (lane != 0) {
+            upT = 0.0f;
+            upJ = bJ;
+            upM = 0.0f;
+        }
+        float dgT = 0.0f, dgM = 0.0f;
+
+        for (int t = 0; t < nsteps; ++t) {
+            const int tw = t & (BWIN - 1);
+            if (tw == 0) {
+                __syncthreads();
+                for (int i = lane; i < BWIN + WAVE - 1; i += WAVE) {
+                    const int c = t - (WAVE - 1) + i;
+                    bwin[i] = (c >= 0 && c < B.len && c < blen_it) ? res[B.off + c] : (uint8_t)fill;
+                }
+                __syncthreads();
+            }
+            const int j = t - lane;  // this lane's column
+            const int boff = ((int)bwin[tw + (WAVE - 1) - lane]) << lkp;
+
+            float td = dgT, md = dgM, tup = upT, jin = upJ;
+            uint32_t bits[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float sc = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+                if (AFF) {
+                    const float M = __builtin_fmaxf(td + sc, 0.0f);
+                    const float I = __builtin_fmaxf(Ip[r] + E, Mp[r] + G);
+                    const float T = fmax3(M, I, jin);
+                    rmax = __builtin_fmaxf(rmax, T);
+                    if (DIRS) {
+                        // default_main_cpu.cl:283-345: source = M, I, J in that
+                        // priority among equal positive values, none when 0
+                        uint32_t code = T > 0.0f ? (T == M ? 1u : (T == I ? 2u : 3u)) : 0u;
+                        uint32_t b = code;
+                        if (code == 1u && md == 0.0f) b |= DB_STOP;
+                        if (code == 1u) {
+                            if (M >= t1 && M >= mp) b |= DB_CAND;
+                            if (-M >= t1 && -M >= mp) b |= DB_CANDM;
+                        }
+                        bits[r] = b;
+                    }
+                    const float jout = __builtin_fmaxf(jin + E, M + G);
+                    td = Tp[r];
+                    md = Mp[r];
+                    Tp[r] = T;
+                    Mp[r] = M;
+                    Ip[r] = I;
+                    jin = jout;
+                } else {
+                    const float hd = td;
+                    const float d = hd + sc;
+                    if (!DIRS) {
+                        const float h = fmax3(d, __builtin_fmaxf(tup, Tp[r]) + G, 0.0f);
+                        rmax = __builtin_fmaxf(rmax, h);
+                        td = Tp[r];
+                        Tp[r] = h;
+                        tup = h;
+                    } else {
+                        // default_main_cpu.cl:162-177: LEFT (x-1), UP (y-1), UL,
+                        // first strict improvement over 0 wins
+                        const float l = tup + G, u = Tp[r] + G;
+                        float cur = 0.0f;
+                        uint32_t code = 0u;
+                        if (cur < l) { cur = l; code = 3u; }
+                        if (cur < u) { cur = u; code = 2u; }
+                        if (cur < d) { cur = d; code = 1u; }
+                        uint32_t b = code;
+                        const float pv = code == 1u ? hd : (code == 3u ? tup : Tp[r]);
+                        if (code != 0u && pv == 0.0f) b |= DB_STOP;
+                        if (code == 1u) {
+                            if (cur >= t1 && cur >= mp) b |= DB_CAND;
+                            if (-cur >= t1 && -cur >= mp) b |= DB_CANDM;
+                        }
+                        bits[r] = b;
+                        rmax = __builtin_fmaxf(rmax, cur);
+                        td = Tp[r];
+                        Tp[r] = cur;
+                        tup = cur;
+                    }
+                }
+            }
+            if (DIRS) {
+                // one R-byte word per lane and step, tile-major
+                uint8_t *dst = dir + ((int64_t)(s * (int64_t)nsteps + t) * WAVE + lane) * R;
+                if (R == 1) {
+                    dst[0] = (uint8_t)bits[0];
+                } else if (R == 2) {
+                    *reinterpret_cast<uint16_t *>(dst) = (uint16_t)(bits[0] | (bits[1 % R] << 8));
+                } else {
+#pragma unroll
+                    for (int w = 0; w < R / 4; ++w) {
+                        const uint32_t word = bits[4 * w] | (bits[(4 * w + 1) % R] << 8) |
+                                              (bits[(4 * w + 2) % R] << 16) | (bits[(4 * w + 3) % R] << 24);
+                        reinterpret_cast<uint32_t *>(dst)[w] = word;
+                    }
+                }
+                bool any = false;
+#pragma unroll
+                for (int r = 0; r < R; ++r) any |= (bits[r] & (DB_CAND | DB_CANDM)) != 0;
+                if (any && j >= 0 && j < B.len) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int x = s * rows + lane * R + r;
+                        if ((bits[r] & (DB_CAND | DB_CANDM)) && x < A.len) {
+                            const int idx = atomicAdd(cand_count, 1);
+                            if (idx < cand_cap) {
+                                Cand c;
+                                c.x = x;
+                                c.y = j;
+                                c.v = Tp[r];
+                                cands[idx] = c;
+                            }
+                        }
+                    }
+                }
+            }
+
+            // bottom row of this lane for column j
+            const float botT = Tp[R - 1];
+            const float botJ = AFF ? jin : 0.0f;
+            const float botM = Mp[R - 1];
+            if (below) {
+                const int c63 = t - (WAVE - 1);
+                if (c63 >= 0 && c63 < blen_it) {
+                    const int sl = c63 & (WAVE - 1);
+                    oT = wrlane(rdlane(botT, WAVE - 1), sl, oT);
+                    if (AFF) oJ = wrlane(rdlane(botJ, WAVE - 1), sl, oJ);
+                    if (DIRS && AFF) oM = wrlane(rdlane(botM, WAVE - 1), sl, oM);
+                    if (sl == WAVE - 1 || c63 == blen_it - 1) {
+                        const int c = (c63 & ~(WAVE - 1)) + lane;
+                        if (c <= c63) {
+                            scr[3 * c] = oT;
+                            if (AFF) scr[3 * c + 1] = oJ;
+                            if (DIRS && AFF) scr[3 * c + 2] = oM;
+                        }
+                    }
+                }
+            }
+            // inputs of step t+1
+            const int tn = t + 1;
+            float injT = 0.0f, injJ = bJ, injM = 0.0f;
+            if (above) {
+                if ((tn & (WAVE - 1)) == 0) {
+                    const int c = tn + lane;
+                    inT = 0.0f;
+                    inJ = bJ;
+                    inM = 0.0f;
+                    if (c < blen_it) {
+                        inT = load_scr(&scr[3 * c]);
+                        if (AFF) inJ = load_scr(&scr[3 * c + 1]);
+                        if (DIRS && AFF) inM = load_scr(&scr[3 * c + 2]);
+                    }
+                }
+                injT = rdlane(inT, tn & (WAVE - 1));
+                if (AFF) injJ = rdlane(inJ, tn & (WAVE - 1));
+                if (DIRS && AFF) injM = rdlane(inM, tn & (WAVE - 1));
+            }
+            dgT = upT;
+            dgM = upM;
+            upT = shr1(injT, botT);
+            if (AFF) upJ = shr1(injJ, botJ);
+            if (DIRS && AFF) upM = shr1(injM, botM);
+        }
+        __threadfence();
+    }
+    return rmax;
+}
+
+// Loads the two transposed substitution tables (A = query, A = target) into LDS.
+__device__ __forceinline__ void load_tables(float *lds, const float *__restrict__ g, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+    __syncthreads();
+}
+
+template <bool AFF, int R>
+__device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *seqs, const ScoreItem &it,
+                                            const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float *scr,
+                                            float G, float E, int fill, int trans) {
+    const float *tab = lds_tab + (trans ? kp2 : 0);
+    return tile<AFF, R, false>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, scr, G, E,
+                               fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0);
+}
+
+// Scoring pass.  queue != nullptr: persistent workers pull items from a
+// device counter (longest first, as the host orders them); otherwise each
+// workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...
+template <bool AFF>
+__global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
+                                                   const ScoreItem *__restrict__ items, int nitems,
+                                                   const uint8_t *__restrict__ trans_flags,
+                                                   const float *__restrict__ tables, int kp, int lkp, float G,
+                                                   float E, float *__restrict__ scratch, int64_t scr_stride,
+                                                   unsigned *__restrict__ queue, float *__restrict__ maxima) {
+    extern __shared__ float lds[];
+    const int kp2 = kp * kp;
+    load_tables(lds, tables, 2 * kp2);
+    uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
+    float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
+    const int fill = kp - 1;
+    int k = 0;
+    for (;;) {
+        int item;
+        if (queue) {
+            unsigned v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
+            item = (int)__builtin_amdgcn_readfirstlane(v);
+        } else {
+            item = (int)blockIdx.x + k * (int)gridDim.x;
+            ++k;
+        }
+        if (item >= nitems) break;
+        const ScoreItem it = items[item];
+        const int tr = trans_flags ? trans_flags[item] : 0;
+        float m;
+        switch (it.R) {
+        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
+        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
+        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
+        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
+        }
+        m = wave_max(m);
+        if (threadIdx.x == 0) maxima[it.out] = m;
+    }
+}
+
+// Direction pass over pairs that passed the traceback gate.
+template <bool AFF>
+__global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
+                                                 const DirItem *__restrict__ items, int nitems,
+                                                 const float *__restrict__ tables, int kp, int lkp, float G, float E,
+                                                 float *__restrict__ scratch, int64_t scr_stride,
+                                                 uint8_t *__restrict__ dirs, int *__restrict__ cand_counts,
+                                                 Cand *__restrict__ cands) {
+    extern __shared__ float lds[];
+    const int kp2 = kp * kp;
+    load_tables(lds, tables, 2 * kp2);
+    uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
+    float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
+    const int fill = kp - 1;
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const DirItem it = items[item];
+        const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
+        uint8_t *d = dirs + it.dir_off;
+        int *cc = cand_counts + item;
+        Cand *cl = cands + it.cand_off;
+        switch (it.R) {
+        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
+        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
+        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
+        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Start selection, marking and the walk: one thread per pair.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int64_t cell_offset(const TbItem &it, int tlen, int x, int y) {
+    const int R = it.R, rows = WAVE * R;
+    const int s = x / rows, rem = x - s * rows, k = rem / R, r = rem - k * R;
+    const int64_t nsteps = (int64_t)tlen + WAVE - 1;
+    return it.dir_off + (((int64_t)s * nsteps + (y + k)) * WAVE + k) * R + r;
+}
+
+// Canonical emission order (larger key = emitted first): the serial reading
+// of the reverse sweep of SmithWaterman._traceback_host (SmithWaterman.py:
+// 626-653) and default_main_cpu.cl:414-420 — block anti-diagonal, block x,
+// work-item anti-diagonal, work-item x, then the j and k loops, all descending.
+__device__ __forceinline__ uint64_t order_key(uint32_t x, uint32_t y) {
+    const uint64_t bx = x >> 3, by = y >> 3, tx = (x >> 2) & 1, ty = (y >> 2) & 1, j = x & 3, k = y & 3;
+    return ((bx + by) << 45) | (bx << 26) | ((tx + ty) << 5) | (tx << 4) | (j << 2) | k;
+}
+
+template <class T, class Less>
+__device__ void heap_sort(T *a, int n, Less less) {
+    auto sift = [&](int root, int end) {
+        while (2 * root + 1 < end) {
+            int child = 2 * root + 1;
+            if (child + 1 < end && less(a[child], a[child + 1])) ++child;
+            if (less(a[root], a[child])) {
+                T tmp = a[root];
+                a[root] = a[child];
+                a[child] = tmp;
+                root = child;
+            } else {
+                return;
+            }
+        }
+    };
+    for (int i = n / 2 - 1; i >= 0; --i) sift(i, n);
+    for (int end = n - 1; end > 0; --end) {
+        T tmp = a[0];
+        a[0] = a[end];
+        a[end] = tmp;
+        sift(0, end);
+    }
+}
+
+// default_main_cpu.cl:389-604 restated per pair: candidates in canonical
+// reverse-sweep order; an unmarked candidate becomes a starting point and its
+// path is marked (linear: along the direction codes until a predecessor of
+// value 0; affine: along main-matrix cells only — I/J marks never arise
+// because they would need an existing I/J mark); a marked candidate becomes
+// a starting point only if its sign-marked value passes the tests.
+template <bool AFF>
+__global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs, const TbItem *__restrict__ items,
+                                                int nitems, uint8_t *__restrict__ dirs, Cand *__restrict__ cands,
+                                                SpRec *__restrict__ sps, int *__restrict__ sp_counts) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nitems) return;
+    const TbItem it = items[g];
+    const int tlen = seqs[it.t_seq].len;
+    Cand *c = cands + it.cand_off;
+    const int n = it.ncand;
+    heap_sort(c, n, [](const Cand &a, const Cand &b) { return order_key(a.x, a.y) > order_key(b.x, b.y); });
+    SpRec *out = sps + it.sp_off;
+    int nsp = 0;
+    for (int i = 0; i < n; ++i) {
+        int x = c[i].x, y = c[i].y;
+        const float v = c[i].v;
+        const int64_t off = cell_offset(it, tlen, x, y);
+        const uint8_t b = dirs[off];
+        float score;
+        bool emit = false;
+        if (!(b & DB_MARK)) {
+            if (b & DB_CAND) {
+                emit = true;
+                score = v;
+                // mark the path
+                int cx = x, cy = y;
+                for (;;) {
+                    const int64_t o = cell_offset(it, tlen, cx, cy);
+                    const uint8_t cb = dirs[o];
+                    dirs[o] = cb | DB_MARK;
+                    const uint32_t code = cb & 3u;
+                    if (cb & DB_STOP) break;
+                    if (code == 0u) break;
+                    if (AFF && code != 1u) break;
+                    const int px = cx - (code != 2u ? 1 : 0), py = cy - (code != 3u ? 1 : 0);
+                    if (px < 0 || py < 0) break;
+                    if (dirs[cell_offset(it, tlen, px, py)] & DB_MARK) break;
+                    cx = px;
+                    cy = py;
+                }
+            }
+        } else if (b & DB_CANDM) {
+            emit = true;
+            score = -v;
+        }
+        if (emit) {
+            SpRec r;
+            r.sequence = (uint32_t)it.qidx;
+            r.target = (uint32_t)it.tidx;
+            r.blockX = (uint32_t)x >> 3;
+            r.blockY = (uint32_t)y >> 3;
+            r.valueX = (uint32_t)x & 7;
+            r.valueY = (uint32_t)y & 7;
+            r.score = score;
+            r.maxScore = it.maxv;
+            r.posScore = it.mp;
+            out[nsp++] = r;
+        }
+    }
+    sp_counts[g] = nsp;
+}
+
+__device__ __forceinline__ uint32_t float_order(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Direction values of SmithWaterman.py:761-821 after the traceback rewrote
+// the matrix: STOP where a marked cell's predecessor is 0, otherwise the
+// code (linear none = 32; affine keeps direction & 3, so none = 0).
+template <bool AFF>
+__device__ __forceinline__ int eff_dir(uint8_t b) {
+    if (b & DB_WALK) return 96;
+    const int code = b & 3;
+    if ((b & DB_MARK) && (b & DB_STOP) && (AFF ? code == 1 : code != 0)) return 64;
+    if (code == 0) return AFF ? 0 : 32;
+    return code;
+}
+
+template <bool AFF>
+__global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
+                                                  const TbItem *__restrict__ items, int nitems,
+                                                  uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
+                                                  const int *__restrict__ sp_counts, uint64_t *__restrict__ keys,
+                                                  HitRec *__restrict__ hits, char *__restrict__ ops,
+                                                  int64_t *__restrict__ ops_used, double minimum) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nitems) return;
+    const TbItem it = items[g];
+    const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
+    const int nsp = sp_counts[g];
+    const SpRec *sp = sps + it.sp_off;
+    uint64_t *key = keys + it.key_off;
+    for (int i = 0; i < nsp; ++i) key[i] = ((uint64_t)float_order(sp[i].score) << 32) | (uint32_t)i;
+    // ascending score, stable (SmithWaterman.py:699)
+    heap_sort(key, nsp, [](const uint64_t &a, const uint64_t &b) { return a < b; });
+    char *op = ops + it.ops_off;
+    int64_t nops = 0;
+    HitRec *h = hits + it.hit_off;
+    for (int k = 0; k < nsp; ++k) {
+        const int i = (int)(key[k] & 0xffffffffu);
+        const SpRec r = sp[i];
+        int x = (int)(r.blockX * 8 + r.valueX), y = (int)(r.blockY * 8 + r.valueY);
+        const int s_end = x, t_end = y;
+        int s_start = s_end + 1, t_start = t_end + 1;
+        int matches = 0, mismatches = 0, len = 0;
+        int direction = eff_dir<AFF>(dirs[cell_offset(it, t.len, x, y)]);
+        bool show = true;
+        if (it.same_id) {
+            direction = 64;
+            show = false;
+        }
+        if ((double)r.score < minimum) show = false;
+        const int64_t start_ops = nops;
+        bool term = false;
+        while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
+            const int64_t o = cell_offset(it, t.len, x, y);
+            const uint8_t b = dirs[o];
+            direction = eff_dir<AFF>(b);
+            dirs[o] = b | DB_WALK;
+            if (direction == 96) {
+                show = false;
+            } else if (direction == 1 || direction == 64) {
+                if (res[q.off + x] == res[t.off + y]) ++matches; else ++mismatches;
+                op[nops++] = 'D';
+                ++len;
+                --s_start;
+                --t_start;
+                if (direction == 1) {
+                    --x;
+                    --y;
+                }
+            } else if (direction == 3) {
+                op[nops++] = 'L';
+                ++len;
+                --s_start;
+                --x;
+            } else if (direction == 2) {
+                op[nops++] = 'U';
+                ++len;
+                --t_start;
+                --y;
+            } else {
+                term = true;
+            }
+        }
+        if (!show) nops = start_ops;
+        HitRec hr;
+        hr.query = it.qidx;
+        hr.target = it.tidx;
+        hr.shown = show ? 1 : 0;
+        hr.s_start = s_start;
+        hr.s_end = s_end;
+        hr.t_start = t_start;
+        hr.t_end = t_end;
+        hr.matches = matches;
+        hr.mismatches = mismatches;
+        hr.length = show ? len : 0;
+        hr.score = r.score;
+        hr.sp_index = i;
+        hr.ops_offset = start_ops;
+        h[k] = hr;
+    }
+    ops_used[g] = nops;
+}
+
+}  // namespace pw

@@ -1,0 +1,124 @@
+"""Helpers shared by the parity tests: seeded inputs and the HIP-vs-oracle comparison."""
+from __future__ import annotations
+
+import numpy as np
+
+DNA = "ACGT"
+AMINO = "ARNDCQEGHILKMFPSTWYV"
+
+
+def rand_seq(rng, n, alphabet=DNA):
+    return "".join(rng.choice(list(alphabet), size=n))
+
+
+def mutate(rng, s, sub_rate, indel_rate=0.0, alphabet=DNA):
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < indel_rate / 2:
+            continue  # deletion
+        if r < indel_rate:
+            out.append(rng.choice(list(alphabet)))  # insertion before c
+        out.append(rng.choice(list(alphabet)) if rng.random() < sub_rate else c)
+    return "".join(out)
+
+
+def planted_set(rng, nq, nt, qlen, tlen, alphabet=DNA, plant=0.5, sub=0.05, indel=0.02):
+    """Queries and targets with shared regions so that hits exist."""
+    targets = [rand_seq(rng, int(rng.integers(tlen[0], tlen[1] + 1)), alphabet) for _ in range(nt)]
+    queries = []
+    for _ in range(nq):
+        q = rand_seq(rng, int(rng.integers(qlen[0], qlen[1] + 1)), alphabet)
+        if rng.random() < plant and targets:
+            t = targets[int(rng.integers(0, nt))]
+            a = int(rng.integers(0, max(1, len(t) // 2)))
+            b = int(rng.integers(a + 1, len(t) + 1))
+            piece = mutate(rng, t[a:b], sub, indel, alphabet)
+            pos = int(rng.integers(0, len(q) + 1))
+            q = q[:pos] + piece + q[pos:]
+        queries.append(q)
+    return queries, targets
+
+
+def hip_all_vs_all(dev, O, queries, targets, p, mode=1, same_id=None):
+    from pypaswas_amd._native import PackedSeqs
+    N, T = len(queries), len(targets)
+    pq = np.repeat(np.arange(N, dtype=np.int32), T)
+    pt = np.tile(np.arange(T, dtype=np.int32), N)
+    mss = np.array([O.min_start_score(p, len(queries[i]), len(targets[j])) for i, j in zip(pq, pt)],
+                   dtype=np.float32)
+    dev.set_scoring(p.subst_f32(), p.char_offset, p.gap, p.gap_ext if p.gap_ext is not None else 0.0,
+                    p.lower_limit, p.minimum)
+    return dev.align_pairs(PackedSeqs.pack(queries), PackedSeqs.pack(targets), pq, pt, mss, same_id, p.minimum,
+                           mode=mode)
+
+
+def sp_tuple(s):
+    return (int(s["sequence"]) if not hasattr(s, "sequence") else int(s.sequence),)
+
+
+def compare(O, queries, targets, p, res, qids=None, tids=None):
+    """Asserts bit-exact parity of maxima, starting points and walk results."""
+    N, T = len(queries), len(targets)
+    br = O.sched_batch(queries, targets, p)
+    # maxima, bit for bit
+    hip_max = res.maxima.reshape(N, T)
+    assert np.array_equal(hip_max.view(np.uint32), br.maxima.view(np.uint32)), \
+        "maxima differ at {}".format(np.argwhere(hip_max != br.maxima)[:5].tolist())
+    # starting points per pair in canonical order
+    o_sp = {}
+    for s in br.starting_points:
+        o_sp.setdefault((s.sequence, s.target), []).append(
+            (s.sequence, s.target, s.blockX, s.blockY, s.valueX, s.valueY, np.float32(s.score).view(np.uint32),
+             np.float32(s.maxScore).view(np.uint32), np.float32(s.posScore).view(np.uint32)))
+    h_sp = {}
+    for s in res.starting_points:
+        h_sp.setdefault((int(s["sequence"]), int(s["target"])), []).append(
+            (int(s["sequence"]), int(s["target"]), int(s["blockX"]), int(s["blockY"]), int(s["valueX"]),
+             int(s["valueY"]), np.float32(s["score"]).view(np.uint32), np.float32(s["maxScore"]).view(np.uint32),
+             np.float32(s["posScore"]).view(np.uint32)))
+    assert set(o_sp) == set(h_sp), "pairs with starting points differ: {} vs {}".format(
+        sorted(set(o_sp) - set(h_sp))[:5], sorted(set(h_sp) - set(o_sp))[:5])
+    for k in o_sp:
+        assert o_sp[k] == h_sp[k], "starting points differ for pair {}".format(k)
+    # walk
+    qids = qids or ["q%d" % i for i in range(N)]
+    tids = tids or ["t%d" % i for i in range(T)]
+    ws = O.walk_c(br.directions, br.starting_points, queries, targets, qids, tids, p.minimum)
+    o_w = {}
+    for w in ws:
+        o_w.setdefault((w.qi, w.ti), []).append(
+            (w.shown, w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches,
+             np.float32(w.score).view(np.uint32), w.sequence_match if w.shown else "",
+             w.alignment if w.shown else "", w.target_match if w.shown else ""))
+    h_w = {}
+    for h in res.hits:
+        qi, ti = int(h["query"]), int(h["target"])
+        shown = bool(h["shown"])
+        sm = al = tm = ""
+        if shown:
+            sm, al, tm = decode_ops(res.ops, int(h["ops_offset"]), int(h["length"]), queries[qi], targets[ti],
+                                    int(h["s_start"]), int(h["t_start"]))
+        h_w.setdefault((qi, ti), []).append(
+            (shown, int(h["s_start"]), int(h["s_end"]), int(h["t_start"]), int(h["t_end"]), int(h["matches"]),
+             int(h["mismatches"]), np.float32(h["score"]).view(np.uint32), sm, al, tm))
+    assert set(o_w) == set(h_w)
+    for k in o_w:
+        assert o_w[k] == h_w[k], "walk differs for pair {}:\n{}\n{}".format(k, o_w[k][:3], h_w[k][:3])
+    return br
+
+
+def decode_ops(ops: bytes, off: int, n: int, q: str, t: str, s_start: int, t_start: int):
+    x, y = s_start, t_start
+    sm, al, tm = [], [], []
+    for k in range(n):
+        op = ops[off + k]
+        if op == ord("D"):
+            a, b = q[x], t[y]
+            sm.append(a); tm.append(b); al.append("|" if a.lower() == b.lower() else ".")
+            x += 1; y += 1
+        elif op == ord("L"):
+            sm.append(q[x]); tm.append("-"); al.append(" "); x += 1
+        else:
+            sm.append("-"); tm.append(t[y]); al.append(" "); y += 1
+    return "".join(sm), "".join(al), "".join(tm)

@@ -1,0 +1,81 @@
+"""HIP path vs the oracle on seeded inputs: maxima, starting points and walk, bit for bit.
+
+Every case runs in both launch modes (0 = plain per-batch, 1 = batched persistent).
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from tests._parity import AMINO, DNA, compare, hip_all_vs_all, planted_set
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from pypaswas_amd._native import Device
+    d = Device(0)
+    yield d
+    d.close()
+
+
+def dna_params(O, affine, minimum=10.0, ll=1.0, ff=0.2):
+    return O.Params(matrix=O.dna_rna_matrix(1.0, -3.0, -1.0, 0.0), gap=-5.0, gap_ext=(-1.0 if affine else None),
+                    lower_limit=ll, minimum=minimum, filter_factor=ff, highest=1.0)
+
+
+def blosum_params(O, minimum=30.0, ll=1.0, ff=0.05):
+    return O.Params(matrix=O.BLOSUM62, gap=-11.0, gap_ext=-1.0, lower_limit=ll, minimum=minimum, filter_factor=ff,
+                    highest=11.0, dna=False)
+
+
+CASES = [
+    # name, alphabet, n queries, n targets, query lengths, target lengths, affine
+    ("dna-linear-small", DNA, 6, 5, (20, 150), (10, 60), False),
+    ("dna-affine-small", DNA, 6, 5, (20, 150), (10, 60), True),
+    ("dna-linear-multistripe", DNA, 3, 3, (600, 1300), (100, 700), False),
+    ("dna-affine-multistripe", DNA, 3, 3, (600, 1300), (100, 700), True),
+]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_dna_parity(dev, oracle, case, mode):
+    name, alpha, nq, nt, ql, tl, affine = case
+    print("case", name, "mode", mode, flush=True)
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    qs, ts = planted_set(rng, nq, nt, ql, tl, alpha, plant=0.7)
+    p = dna_params(oracle, affine)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=mode)
+    compare(oracle, qs, ts, p, res)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_protein_affine_parity(dev, oracle, mode):
+    rng = np.random.default_rng(7)
+    qs, ts = planted_set(rng, 5, 5, (30, 400), (30, 400), AMINO, plant=0.8, sub=0.5, indel=0.05)
+    p = blosum_params(oracle)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=mode)
+    compare(oracle, qs, ts, p, res)
+
+
+@pytest.mark.parametrize("ll", [1.0, 0.8, 0.5])
+@pytest.mark.parametrize("affine", [False, True])
+def test_lower_limit_many_starts(dev, oracle, ll, affine):
+    """--llimit < 1 produces many starting points, merging paths and suppressed walks."""
+    rng = np.random.default_rng(11)
+    qs, ts = planted_set(rng, 4, 4, (60, 200), (40, 120), DNA, plant=1.0, sub=0.03)
+    p = dna_params(oracle, affine, minimum=5.0, ll=ll, ff=0.0)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+
+
+def test_repeats_and_ties(dev, oracle):
+    """Homopolymers and tandem repeats: many equal maxima and merging paths."""
+    qs = ["A" * 40, "ACACACACACACACACACAC", "GGGGTTTTGGGGTTTT" * 3]
+    ts = ["A" * 25, "CACACACACA", "TTTTGGGG" * 4]
+    for affine in (False, True):
+        p = dna_params(oracle, affine, minimum=3.0, ll=1.0, ff=0.0)
+        res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+        compare(oracle, qs, ts, p, res)
