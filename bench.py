@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: GCUPS of the batched persistent path vs the plain per-batch launch.
+
+Workload (default): the affine-gap DNA trimmer configuration — 20,000 reads,
+lengths lognormal(median 250, sigma 0.5) clipped 50-1000, uniform ACGT with
+1% N, against 64 adapters of 20-80 nt plus their 64 reverse complements;
+match +1, mismatch -3, gap open -5, extension -1, FP32.  One step = score +
+traceback gate + direction recompute + start selection + walk over all
+2,560,000 pairs, results copied back; sequences and the work list are
+resident on the device before the timed region (pw_prepare).
+
+Prints one JSON line.  Also runs, untimed for the headline value:
+  * the plain per-batch launch (every pair padded to the batch's longest
+    read and adapter, static schedule) for the batched/plain ratio;
+  * the CPU baseline: the oracle's restatement of the reference OpenCL-CPU
+    schedule (C + OpenMP) on a bounded sample of the same workload, whose hits
+    are also compared with the GPU's hits on the same pairs.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS batched-persistent vs plain launch, bit-exact hits; p50/p99 latency"
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, 32-bit VALU ops
+OPS_PER_CELL = {"linear": 7, "affine": 17}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_job(cfg):
+    from pypaswas_amd import workloads as W
+    from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
+    from pypaswas_amd.Core.Scores import DnaRnaScore, Blosum62Score
+    import logging
+    gen = {"config2": W.config2, "config1": W.config1, "config4": W.config4}[cfg]
+    queries, targets, overrides = gen()
+    settings = W.settings_dict(overrides)
+    logger = logging.getLogger("bench")
+    score = (Blosum62Score if settings.matrix_name.upper() == "BLOSUM62" else DnaRnaScore)(logger, settings)
+    dna = score.score_type == "DNA_RNA"
+    q, t = W.reference_order(queries, targets, dna)
+    return q, t, settings, score
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-batch launch")
+    ap.add_argument("--cpu-reads", type=int, default=800, help="reads in the CPU-baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
+              "form (the sharding plan is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from pypaswas_amd._native import Device
+    from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
+
+    t0 = time.time()
+    q, t, settings, score = build_job(args.workload)
+    log("workload built: %d queries x %d targets in %.1fs" % (len(q), len(t), time.time() - t0))
+    qp, tp, pq, pt, mss, same = job_arrays(q, t, score.highest_score, float(settings.filter_factor))
+    dev = Device(0)
+    dev.set_scoring(score.to_float32(), ord(score.char_offset), score.gap_score,
+                    score.gap_extension if score.gap_extension is not None else 0.0, score.lower_limit_score,
+                    score.minimum_score)
+    minimum = float(settings.minimum_score)
+    kind = "affine" if score.gap_extension is not None else "linear"
+
+    t0 = time.time()
+    plan = dev.prepare(qp, tp, pq, pt, mss, same, mode=1)
+    log("batched plan prepared in %.1fs" % (time.time() - t0))
+    for i in range(args.warmup):
+        r = dev.execute(plan, minimum)
+        log("warmup %d: %.1f ms (score %.1f ms, dirs %.1f, tb %.1f, walk %.1f), %d traceback pairs, %d hits" %
+            (i, r.ms_total, r.ms_score, r.ms_directions, r.ms_traceback, r.ms_walk, r.n_traceback_pairs,
+             int(r.hits["shown"].sum()) if len(r.hits) else 0))
+    score_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = dev.execute(plan, minimum)
+        score_ms.append(r.ms_score)
+    elapsed = time.perf_counter() - t0
+    cells = r.cells
+    gcups = cells * args.steps / elapsed / 1e9
+    ms_step = elapsed * 1000 / args.steps
+    log("batched: %.1f ms/step, %.1f GCUPS, score kernel %.2f ms" % (ms_step, gcups, np.mean(score_ms)))
+    last = r
+    plan.release()
+
+    plain = None
+    if not args.no_plain:
+        plan0 = dev.prepare(qp, tp, pq, pt, mss, same, mode=0)
+        r0 = dev.execute(plan0, minimum)
+        t1 = time.perf_counter()
+        r0 = dev.execute(plan0, minimum)
+        el0 = time.perf_counter() - t1
+        plan0.release()
+        same_hits = (np.array_equal(r0.maxima, last.maxima) and np.array_equal(r0.hits, last.hits)
+                     and r0.ops == last.ops)
+        plain = {"gcups": cells / el0 / 1e9, "ms_per_step": el0 * 1000, "score_kernel_ms": r0.ms_score,
+                 "identical_results": bool(same_hits)}
+        log("plain: %.1f ms/step, %.2f GCUPS, identical=%s" % (el0 * 1000, plain["gcups"], same_hits))
+
+    avg_score_s = float(np.mean(score_ms)) / 1000.0
+    achieved = OPS_PER_CELL[kind] * cells / avg_score_s / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "kernel": "score_kernel<%s>" % ("true" if kind == "affine" else "false"),
+                "ops_per_cell": OPS_PER_CELL[kind], "cells_per_launch": cells,
+                "kernel_ms": round(float(np.mean(score_ms)), 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(q, t, settings, score, dev, args.cpu_reads)
+
+    out = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "affine-gap DNA trimmer: 20,000 reads lognormal(median 250, sigma 0.5) clipped "
+                               "50-1000 with 1% N x 64 adapters 20-80 nt + 64 reverse complements; +1/-3, "
+                               "gap -5/-1; batched persistent kernel" if args.workload == "config2" else
+                               args.workload,
+                   "pairs": int(len(pq)), "cells": cells, "mode": "batched-persistent"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_launch": plain,
+        "batched_over_plain": round(gcups / plain["gcups"], 2) if plain else None,
+        "traceback_pairs": last.n_traceback_pairs,
+        "hits": int(last.hits["shown"].sum()) if len(last.hits) else 0,
+        "stage_ms": {"score": round(last.ms_score, 3), "directions": round(last.ms_directions, 3),
+                     "traceback": round(last.ms_traceback, 3), "walk": round(last.ms_walk, 3),
+                     "call_wall": round(last.ms_total, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(q, t, settings, score, dev, n_reads):
+    """Oracle restatement of the reference CPU schedule on the first n_reads
+    reads (reader order) x all targets, in sub-batches of 8 reads (the
+    reference also batches by memory); hits compared with the GPU's."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
+    p = O.Params(matrix=score.matrix, char_offset=ord(score.char_offset), gap=score.gap_score,
+                 gap_ext=score.gap_extension, lower_limit=score.lower_limit_score, minimum=score.minimum_score,
+                 filter_factor=float(settings.filter_factor), highest=float(score.highest_score))
+    stride = max(1, len(q) // n_reads)
+    sample = q[::stride][:n_reads]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    cells = float(sum(len(s) for _, s in sample)) * float(sum(len(s) for _, s in t))
+    targets = [s for _, s in t]
+    tids = [i for i, _ in t]
+    o_hits = {}
+    t0 = time.perf_counter()
+    for b in range(0, len(sample), 8):
+        chunk = sample[b:b + 8]
+        qs = [s for _, s in chunk]
+        br = O.sched_batch(qs, targets, p, nthreads=threads)
+        ws = O.walk_c(br.directions, br.starting_points, qs, targets, [i for i, _ in chunk], tids, p.minimum)
+        for w in ws:
+            o_hits.setdefault((b + w.qi, w.ti), []).append(
+                (w.shown, w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches, w.score))
+    el = time.perf_counter() - t0
+    # the same sample on the GPU
+    qp, tp, pq, pt, mss, same = job_arrays(sample, t, score.highest_score, float(settings.filter_factor))
+    r = dev.align_pairs(qp, tp, pq, pt, mss, same, float(settings.minimum_score), mode=1)
+    g_hits = {}
+    for h in r.hits:
+        g_hits.setdefault((int(h["query"]), int(h["target"])), []).append(
+            (bool(h["shown"]), int(h["s_start"]), int(h["s_end"]), int(h["t_start"]), int(h["t_end"]),
+             int(h["matches"]), int(h["mismatches"]), float(h["score"])))
+    o_norm = {k: [(bool(a),) + tuple(v[1:]) for v in vs for a in [v[0]]] for k, vs in o_hits.items()}
+    parity = "bit-exact" if o_norm == g_hits else "MISMATCH"
+    log("cpu baseline: %.2e cells in %.1fs on %d threads = %.3f GCUPS; GPU parity on the sample: %s (%d walks)"
+        % (cells, el, threads, cells / el / 1e9, parity, sum(len(v) for v in o_hits.values())))
+    return {"value": round(cells / el / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": "%d reads (every %d-th in reader order) x all %d targets of the same workload, "
+                      "%.3g cells, oracle restatement of the reference OpenCL-CPU schedule (8x8 blocks, 2x2 "
+                      "work-items of 4x4 cells, full FP32 matrices) + host walk, OpenMP"
+                      % (len(sample), stride, len(t), cells),
+            "seconds": round(el, 2), "gpu_parity_on_sample": parity}
+
+
+if __name__ == "__main__":
+    main()

@@ -22,6 +22,7 @@ extern "C" {
 #define PW_EDEVICE (-3)  /* HIP runtime / kernel error -> HardwareException */
 
 typedef struct pw_ctx pw_ctx;
+typedef struct pw_plan pw_plan;
 
 /* Starting point record, 36 bytes, layout of the StartingPoint struct of
  * pyPaSWAS/Core/ocl/default_main_cpu.cl:66-76 (parsed by
@@ -103,6 +104,19 @@ int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const 
                    const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
                    const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
                    const uint8_t *same_id, double minimum, int32_t sp_cap, int32_t mode, pw_result **out);
+
+/* The same work as pw_align_pairs split in two, so that sequences, the
+ * substitution tables and the scoring work list stay resident in device
+ * memory across executions (the role of copy_sequences + _init_memory of
+ * SmithWatermanOcl.py:223-293, done once).  pw_prepare encodes and uploads;
+ * pw_execute runs scoring, gate, traceback and walk on the resident plan;
+ * pw_release_plan frees it. */
+int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+               const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+               const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
+               const uint8_t *same_id, int32_t mode, pw_plan **out);
+int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out);
+int pw_release_plan(pw_ctx *ctx, pw_plan *plan);
 
 /* Score pass only: pair maxima, the quantity the traceback gate of
  * default_main_cpu.cl:408-410 reads.  maxima_out is [n_pairs]. */

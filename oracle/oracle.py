@@ -371,7 +371,8 @@ def walk_c(directions: np.ndarray, sps: Sequence[SpRecord], queries: Sequence[st
         if r.shown:
             x = r.s_start
             y = r.t_start
-            for k in range(r.length):
+            # ops are recorded in walk order, from the alignment end back to its start
+            for k in range(r.length - 1, -1, -1):
                 op = chr(raw[r.ops_offset + k])
                 if op == "D":
                     a, b = q[x], t[y]

@@ -24,8 +24,8 @@ PW_ENOMEM = -2
 PW_EDEVICE = -3
 
 # exported entry points, in the order include/pypaswas_hip.h declares them
-EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_score_pairs",
-           "pw_free_result", "pw_last_error")
+EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
+           "pw_execute", "pw_release_plan", "pw_score_pairs", "pw_free_result", "pw_last_error")
 
 
 class StartingPointRec(ctypes.Structure):
@@ -82,6 +82,10 @@ def load() -> ctypes.CDLL:
         L.pw_align_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, vp, vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.POINTER(ctypes.POINTER(ResultRec))]
+        L.pw_prepare.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int64,
+                                 vp, vp, ctypes.c_int32, ctypes.POINTER(vp)]
+        L.pw_execute.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(ResultRec))]
+        L.pw_release_plan.argtypes = [vp, vp]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
@@ -206,6 +210,9 @@ class Device:
                                      _ptr(mss), _ptr(sid), float(minimum), int(sp_cap), int(mode),
                                      ctypes.byref(out))
         self._check(rc, "pw_align_pairs")
+        return self._take_result(out)
+
+    def _take_result(self, out) -> AlignResult:
         try:
             r = out.contents
             n = int(r.n_pairs)
@@ -225,6 +232,26 @@ class Device:
         finally:
             self.lib.pw_free_result(self.ctx, out)
 
+    def prepare(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
+                min_start_score: Optional[np.ndarray], same_id: Optional[np.ndarray], mode: int = 1) -> "Plan":
+        pq = np.ascontiguousarray(pq, dtype=np.int32)
+        pt = np.ascontiguousarray(pt, dtype=np.int32)
+        mss = None if min_start_score is None else np.ascontiguousarray(min_start_score, dtype=np.float32)
+        sid = None if same_id is None else np.ascontiguousarray(same_id, dtype=np.uint8)
+        handle = ctypes.c_void_p()
+        rc = self.lib.pw_prepare(self.ctx, _ptr(queries.data), _ptr(queries.off), _ptr(queries.length),
+                                 len(queries.length), _ptr(targets.data), _ptr(targets.off), _ptr(targets.length),
+                                 len(targets.length), _ptr(pq), _ptr(pt), len(pq), _ptr(mss), _ptr(sid), int(mode),
+                                 ctypes.byref(handle))
+        self._check(rc, "pw_prepare")
+        return Plan(self, handle)
+
+    def execute(self, plan: "Plan", minimum: float, sp_cap: int = 100) -> AlignResult:
+        out = ctypes.POINTER(ResultRec)()
+        self._check(self.lib.pw_execute(self.ctx, plan.handle, float(minimum), int(sp_cap), ctypes.byref(out)),
+                    "pw_execute")
+        return self._take_result(out)
+
     def score_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                     mode: int = 1):
         pq = np.ascontiguousarray(pq, dtype=np.int32)
@@ -237,3 +264,22 @@ class Device:
                                      int(mode), _ptr(out), ctypes.byref(ms))
         self._check(rc, "pw_score_pairs")
         return out, float(ms.value)
+
+
+class Plan:
+    """Device-resident sequences and work list (pw_prepare); release() frees them."""
+
+    def __init__(self, dev: Device, handle):
+        self.dev = dev
+        self.handle = handle
+
+    def release(self):
+        if self.handle:
+            self.dev.lib.pw_release_plan(self.dev.ctx, self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

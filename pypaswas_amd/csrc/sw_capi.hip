@@ -50,6 +50,33 @@ struct pw_ctx {
 
 static thread_local std::string g_init_err;
 
+namespace {
+
+struct Encoded {
+    int K = 0, KP = 8, lkp = 5;
+    std::vector<uint8_t> codes;     // all residues, queries then targets
+    std::vector<SeqRef> refs;       // nq + nt
+    std::vector<float> tables;      // 2 * KP * KP
+    int64_t max_len_q = 0, max_len_t = 0;
+};
+
+}  // namespace
+
+// A prepared job: sequences, substitution tables and the scoring work list
+// resident on the device, plus the host copies the later stages need.
+struct pw_plan {
+    Encoded E;
+    int32_t nq = 0, nt = 0;
+    int64_t n_pairs = 0;
+    int32_t mode = 1;
+    std::vector<int32_t> pq, pt, qlen, tlen;
+    std::vector<float> mss;
+    std::vector<uint8_t> same;
+    int64_t max_b = 0;
+    double cells = 0;
+    DevBuf res, seqs, tables, items, trans;
+};
+
 #define HIPCHK(ctx, call)                                                                   \
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \
@@ -193,14 +220,6 @@ extern "C" int pw_set_scoring(pw_ctx *ctx, const float *subst, int dim, int char
 
 namespace {
 
-struct Encoded {
-    int K = 0, KP = 8, lkp = 5;
-    std::vector<uint8_t> codes;     // all residues, queries then targets
-    std::vector<SeqRef> refs;       // nq + nt
-    std::vector<float> tables;      // 2 * KP * KP
-    int64_t max_len_q = 0, max_len_t = 0;
-};
-
 int encode(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
            const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, Encoded &E) {
     int map[256];
@@ -306,24 +325,62 @@ int dir_R(int qlen) {
 
 size_t lds_bytes(int KP) { return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE; }
 
-struct Prepared {
-    Encoded E;
-    int64_t max_b_it = 0;
-    int nitems = 0;
-};
+int check_pairs(pw_ctx *ctx, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
+    if (n_pairs < 0 || n_pairs > (int64_t)INT32_MAX) {
+        ctx->err = "n_pairs out of range";
+        return PW_EINVAL;
+    }
+    for (int64_t i = 0; i < n_pairs; ++i)
+        if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
+            ctx->err = "pair " + std::to_string(i) + " indexes outside the sequence lists";
+            return PW_EINVAL;
+        }
+    return PW_OK;
+}
 
-int upload_and_score(pw_ctx *ctx, Encoded &E, int32_t nq, const int32_t *qlen, const int32_t *tlen,
-                     const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, double *ms_score,
-                     int64_t *launches) {
-    const int KP = E.KP;
-    ENSURE(ctx, ctx->res, E.codes.size());
-    ENSURE(ctx, ctx->seqs, E.refs.size() * sizeof(SeqRef));
-    ENSURE(ctx, ctx->tables, E.tables.size() * sizeof(float));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->res.p, E.codes.data(), E.codes.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->seqs.p, E.refs.data(), E.refs.size() * sizeof(SeqRef), hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->tables.p, E.tables.data(), E.tables.size() * sizeof(float),
-                               hipMemcpyHostToDevice, ctx->stream));
+void free_buf(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+void free_plan(pw_plan *P) {
+    if (!P) return;
+    free_buf(P->res);
+    free_buf(P->seqs);
+    free_buf(P->tables);
+    free_buf(P->items);
+    free_buf(P->trans);
+    delete P;
+}
+
+// Encodes and uploads the sequences and builds the scoring work list.
+int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+            const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, const int32_t *pq,
+            const int32_t *pt, int64_t n_pairs, const float *mss, const uint8_t *same_id, int32_t mode) {
+    if (mode != 0 && mode != 1) {
+        ctx->err = "mode must be 0 (plain per-batch launch) or 1 (batched persistent)";
+        return PW_EINVAL;
+    }
+    int rc = check_pairs(ctx, nq, nt, pq, pt, n_pairs);
+    if (rc) return rc;
+    rc = encode(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, P->E);
+    if (rc) return rc;
+    Encoded &E = P->E;
+    P->nq = nq;
+    P->nt = nt;
+    P->n_pairs = n_pairs;
+    P->mode = mode;
+    P->pq.assign(pq, pq + n_pairs);
+    P->pt.assign(pt, pt + n_pairs);
+    P->qlen.assign(qlen, qlen + nq);
+    P->tlen.assign(tlen, tlen + nt);
+    if (mss) P->mss.assign(mss, mss + n_pairs);
+    else P->mss.assign((size_t)n_pairs, 0.0f);
+    if (same_id) P->same.assign(same_id, same_id + n_pairs);
+    else P->same.assign((size_t)n_pairs, 0);
+    P->cells = 0;
+    for (int64_t i = 0; i < n_pairs; ++i) P->cells += (double)qlen[pq[i]] * (double)tlen[pt[i]];
 
     std::vector<ScoreItem> items((size_t)n_pairs);
     std::vector<uint8_t> trans((size_t)n_pairs, 0);
@@ -332,8 +389,7 @@ int upload_and_score(pw_ctx *ctx, Encoded &E, int32_t nq, const int32_t *qlen, c
         // the reference's batch shape: every pair padded to the longest query
         // and target rounded up to 8 (SmithWaterman.py:459-461, :325)
         const int X = pad8((int)E.max_len_q), Y = pad8((int)E.max_len_t);
-        for (int64_t i = 0; i < n_pairs; ++i)
-            items[i] = ScoreItem{pq[i], nq + pt[i], (int32_t)i, 4, X, Y};
+        for (int64_t i = 0; i < n_pairs; ++i) items[i] = ScoreItem{pq[i], nq + pt[i], (int32_t)i, 4, X, Y};
         max_b = Y;
     } else {
         std::vector<std::pair<double, int64_t>> order((size_t)n_pairs);
@@ -363,34 +419,45 @@ int upload_and_score(pw_ctx *ctx, Encoded &E, int32_t nq, const int32_t *qlen, c
         items.swap(sorted);
         trans.swap(strans);
     }
-    ENSURE(ctx, ctx->items, items.size() * sizeof(ScoreItem));
-    ENSURE(ctx, ctx->trans, trans.size());
+    P->max_b = max_b;
+    ENSURE(ctx, P->res, E.codes.size());
+    ENSURE(ctx, P->seqs, E.refs.size() * sizeof(SeqRef));
+    ENSURE(ctx, P->tables, E.tables.size() * sizeof(float));
+    ENSURE(ctx, P->items, items.size() * sizeof(ScoreItem));
+    ENSURE(ctx, P->trans, trans.size());
+    HIPCHK(ctx, hipMemcpyAsync(P->res.p, E.codes.data(), E.codes.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(P->seqs.p, E.refs.data(), E.refs.size() * sizeof(SeqRef), hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(P->tables.p, E.tables.data(), E.tables.size() * sizeof(float),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (n_pairs > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(ScoreItem),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(P->trans.p, trans.data(), trans.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PW_OK;
+}
+
+// Scoring pass over all pairs of the plan; maxima land in ctx->maxima.
+int run_score(pw_ctx *ctx, pw_plan *P, double *ms_score, int64_t *launches) {
+    const int64_t n_pairs = P->n_pairs;
     ENSURE(ctx, ctx->maxima, (size_t)n_pairs * sizeof(float));
     ENSURE(ctx, ctx->queue, 64);
-    HIPCHK(ctx, hipMemcpyAsync(ctx->items.p, items.data(), items.size() * sizeof(ScoreItem), hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->trans.p, trans.data(), trans.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64, ctx->stream));
-
-    const size_t lds = lds_bytes(KP);
+    const size_t lds = lds_bytes(P->E.KP);
     const int64_t want = (int64_t)ctx->num_cus * 8;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_pairs, want));
-    const int64_t scr_stride = 3 * (max_b + 64);
+    const int64_t scr_stride = 3 * (P->max_b + 64);
     ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float));
-    unsigned *queue = mode == 0 ? nullptr : (unsigned *)ctx->queue.p;
+    unsigned *queue = P->mode == 0 ? nullptr : (unsigned *)ctx->queue.p;
+    HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (n_pairs > 0) {
-        if (ctx->affine)
-            hipLaunchKernelGGL(score_kernel<true>, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)ctx->res.p,
-                               (const SeqRef *)ctx->seqs.p, (const ScoreItem *)ctx->items.p, (int)n_pairs,
-                               (const uint8_t *)ctx->trans.p, (const float *)ctx->tables.p, KP, E.lkp, ctx->gap,
-                               ctx->ext, (float *)ctx->scratch.p, scr_stride, queue, (float *)ctx->maxima.p);
-        else
-            hipLaunchKernelGGL(score_kernel<false>, dim3(grid), dim3(64), lds, ctx->stream,
-                               (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
-                               (const ScoreItem *)ctx->items.p, (int)n_pairs, (const uint8_t *)ctx->trans.p,
-                               (const float *)ctx->tables.p, KP, E.lkp, ctx->gap, ctx->ext, (float *)ctx->scratch.p,
-                               scr_stride, queue, (float *)ctx->maxima.p);
+        auto kern = ctx->affine ? score_kernel<true> : score_kernel<false>;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
+                           (const SeqRef *)P->seqs.p, (const ScoreItem *)P->items.p, (int)n_pairs,
+                           (const uint8_t *)P->trans.p, (const float *)P->tables.p, P->E.KP, P->E.lkp, ctx->gap,
+                           ctx->ext, (float *)ctx->scratch.p, scr_stride, queue, (float *)ctx->maxima.p);
         HIPCHK(ctx, hipGetLastError());
         if (launches) *launches += 1;
     }
@@ -399,19 +466,6 @@ int upload_and_score(pw_ctx *ctx, Encoded &E, int32_t nq, const int32_t *qlen, c
     float ms = 0.0f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     if (ms_score) *ms_score += ms;
-    return PW_OK;
-}
-
-int check_pairs(pw_ctx *ctx, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
-    if (n_pairs < 0 || n_pairs > (int64_t)INT32_MAX) {
-        ctx->err = "n_pairs out of range";
-        return PW_EINVAL;
-    }
-    for (int64_t i = 0; i < n_pairs; ++i)
-        if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
-            ctx->err = "pair " + std::to_string(i) + " indexes outside the sequence lists";
-            return PW_EINVAL;
-        }
     return PW_OK;
 }
 
@@ -424,85 +478,23 @@ struct HostResult {
     std::vector<char> ops;
 };
 
-}  // namespace
-
-extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
-                              const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
-                              const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
-                              double *ms_out) {
-    if (!ctx) return PW_EINVAL;
-    if (!ctx->scoring_set) {
-        ctx->err = "pw_set_scoring has not been called";
-        return PW_EINVAL;
-    }
-    if ((n_pairs > 0 && (!pq || !pt || !maxima_out)) || nq < 0 || nt < 0) {
-        ctx->err = "pw_score_pairs: missing argument";
-        return PW_EINVAL;
-    }
-    HIPCHK(ctx, hipSetDevice(ctx->device));
-    int rc = check_pairs(ctx, nq, nt, pq, pt, n_pairs);
-    if (rc) return rc;
-    Encoded E;
-    rc = encode(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, E);
-    if (rc) return rc;
-    double ms = 0;
-    rc = upload_and_score(ctx, E, nq, qlen, tlen, pq, pt, n_pairs, mode, &ms, nullptr);
-    if (rc) return rc;
-    if (n_pairs > 0)
-        HIPCHK(ctx, hipMemcpy(maxima_out, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost));
-    if (ms_out) *ms_out = ms;
-    return PW_OK;
-}
-
-extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
-                              const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
-                              const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
-                              const uint8_t *same_id, double minimum, int32_t sp_cap, int32_t mode, pw_result **out) {
+int execute(pw_ctx *ctx, pw_plan *P, double minimum, int32_t sp_cap, HostResult *R) {
     const auto wall0 = std::chrono::steady_clock::now();
-    if (!ctx) return PW_EINVAL;
-    if (!out) {
-        ctx->err = "pw_align_pairs: out is NULL";
-        return PW_EINVAL;
-    }
-    *out = nullptr;
-    if (!ctx->scoring_set) {
-        ctx->err = "pw_set_scoring has not been called";
-        return PW_EINVAL;
-    }
-    if ((n_pairs > 0 && (!pq || !pt || !min_start_score)) || nq < 0 || nt < 0) {
-        ctx->err = "pw_align_pairs: missing argument";
-        return PW_EINVAL;
-    }
-    HIPCHK(ctx, hipSetDevice(ctx->device));
-    int rc = check_pairs(ctx, nq, nt, pq, pt, n_pairs);
-    if (rc) return rc;
-    Encoded E;
-    rc = encode(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, E);
-    if (rc) return rc;
-
-    HostResult *R = new HostResult();
-    std::memset(&R->pub, 0, sizeof(R->pub));
-    auto fail = [&](int code) {
-        delete R;
-        return code;
-    };
+    const int64_t n_pairs = P->n_pairs;
+    const int32_t nq = P->nq;
+    const int32_t *pq = P->pq.data(), *pt = P->pt.data(), *qlen = P->qlen.data(), *tlen = P->tlen.data();
+    const float *min_start_score = P->mss.data();
+    Encoded &E = P->E;
     double ms_score = 0;
     int64_t launches = 0;
-    rc = upload_and_score(ctx, E, nq, qlen, tlen, pq, pt, n_pairs, mode, &ms_score, &launches);
-    if (rc) return fail(rc);
+    int rc = run_score(ctx, P, &ms_score, &launches);
+    if (rc) return rc;
     R->maxima.resize((size_t)n_pairs);
-    R->pq.assign(pq, pq + n_pairs);
-    R->pt.assign(pt, pt + n_pairs);
-    if (n_pairs > 0) {
-        hipError_t e = hipMemcpy(R->maxima.data(), ctx->maxima.p, (size_t)n_pairs * sizeof(float),
-                                 hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            ctx->err = std::string("hipMemcpy maxima: ") + hipGetErrorString(e);
-            return fail(PW_EDEVICE);
-        }
-    }
-    double cells = 0;
-    for (int64_t i = 0; i < n_pairs; ++i) cells += (double)qlen[pq[i]] * (double)tlen[pt[i]];
+    R->pq = P->pq;
+    R->pt = P->pt;
+    if (n_pairs > 0)
+        HIPCHK(ctx, hipMemcpy(R->maxima.data(), ctx->maxima.p, (size_t)n_pairs * sizeof(float),
+                              hipMemcpyDeviceToHost));
 
     // traceback gate: maximum >= MINIMUM_SCORE (default_main_cpu.cl:410); a
     // start also needs a value >= the pair's minimum start score (:427)
@@ -513,10 +505,10 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     }
     const int ng = (int)gated.size();
     double ms_dir = 0, ms_tb = 0, ms_walk = 0;
-    std::vector<TbItem> tb(ng);
-    std::vector<int> spc(ng, 0);
-    std::vector<int64_t> opsused(ng, 0);
     if (ng > 0) {
+        std::vector<TbItem> tb(ng);
+        std::vector<int> spc(ng, 0);
+        std::vector<int64_t> opsused(ng, 0);
         // --- direction pass ---
         std::vector<DirItem> di(ng);
         int64_t dir_total = 0, cand_total = 0;
@@ -555,18 +547,11 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
                                        hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(ctx, hipMemsetAsync(ctx->ccount.p, 0, (size_t)n * sizeof(int), ctx->stream));
             HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-            if (ctx->affine)
-                hipLaunchKernelGGL(dir_kernel<true>, dim3(grid), dim3(64), lds, ctx->stream,
-                                   (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
-                                   (const DirItem *)ctx->ditems.p, n, (const float *)ctx->tables.p, E.KP, E.lkp,
-                                   ctx->gap, ctx->ext, (float *)ctx->scratch.p, scr_stride, (uint8_t *)ctx->dirs.p,
-                                   (int *)ctx->ccount.p, (Cand *)ctx->cands.p);
-            else
-                hipLaunchKernelGGL(dir_kernel<false>, dim3(grid), dim3(64), lds, ctx->stream,
-                                   (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
-                                   (const DirItem *)ctx->ditems.p, n, (const float *)ctx->tables.p, E.KP, E.lkp,
-                                   ctx->gap, ctx->ext, (float *)ctx->scratch.p, scr_stride, (uint8_t *)ctx->dirs.p,
-                                   (int *)ctx->ccount.p, (Cand *)ctx->cands.p);
+            auto kern = ctx->affine ? dir_kernel<true> : dir_kernel<false>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
+                               (const SeqRef *)P->seqs.p, (const DirItem *)ctx->ditems.p, n,
+                               (const float *)P->tables.p, E.KP, E.lkp, ctx->gap, ctx->ext, (float *)ctx->scratch.p,
+                               scr_stride, (uint8_t *)ctx->dirs.p, (int *)ctx->ccount.p, (Cand *)ctx->cands.p);
             HIPCHK(ctx, hipGetLastError());
             HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
             HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
@@ -579,21 +564,15 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         ENSURE(ctx, ctx->ccount, (size_t)ng * sizeof(int));
         ENSURE(ctx, ctx->cands, (size_t)cand_total * sizeof(Cand));
         rc = run_dir(di, ms_dir);
-        if (rc) return fail(rc);
+        if (rc) return rc;
         std::vector<int> cc(ng);
-        {
-            hipError_t e = hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) {
-                ctx->err = std::string("hipMemcpy candidate counts: ") + hipGetErrorString(e);
-                return fail(PW_EDEVICE);
-            }
-        }
+        HIPCHK(ctx, hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost));
         // pairs whose candidates overflowed the first guess run again with the
         // exact capacity; the recomputed directions are identical
-        std::vector<int> over;
+        bool over = false;
         for (int g = 0; g < ng; ++g)
-            if (cc[g] > di[g].cand_cap) over.push_back(g);
-        if (!over.empty()) {
+            if (cc[g] > di[g].cand_cap) over = true;
+        if (over) {
             int64_t ctot = 0;
             for (int g = 0; g < ng; ++g) {
                 const int cap = std::max(cc[g], 1);
@@ -603,12 +582,8 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
             }
             ENSURE(ctx, ctx->cands, (size_t)ctot * sizeof(Cand));
             rc = run_dir(di, ms_dir);
-            if (rc) return fail(rc);
-            hipError_t e = hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) {
-                ctx->err = std::string("hipMemcpy candidate counts: ") + hipGetErrorString(e);
-                return fail(PW_EDEVICE);
-            }
+            if (rc) return rc;
+            HIPCHK(ctx, hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost));
         }
         // --- start selection + marking ---
         int64_t sp_total = 0;
@@ -629,7 +604,7 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
             t.mp = min_start_score[p];
             t.qidx = pq[p];
             t.tidx = pt[p];
-            t.same_id = same_id ? same_id[p] : 0;
+            t.same_id = P->same[p];
             t.ncand = std::min(cc[g], di[g].cand_cap);
             tb[g] = t;
             sp_total += t.ncand;
@@ -641,14 +616,10 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
                                    ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
         const int tb_grid = (ng + 63) / 64;
-        if (ctx->affine)
-            hipLaunchKernelGGL(tb_kernel<true>, dim3(tb_grid), dim3(64), 0, ctx->stream, (const SeqRef *)ctx->seqs.p,
-                               (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (Cand *)ctx->cands.p,
-                               (SpRec *)ctx->sps.p, (int *)ctx->spcount.p);
-        else
-            hipLaunchKernelGGL(tb_kernel<false>, dim3(tb_grid), dim3(64), 0, ctx->stream, (const SeqRef *)ctx->seqs.p,
-                               (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (Cand *)ctx->cands.p,
-                               (SpRec *)ctx->sps.p, (int *)ctx->spcount.p);
+        auto tbk = ctx->affine ? tb_kernel<true> : tb_kernel<false>;
+        hipLaunchKernelGGL(tbk, dim3(tb_grid), dim3(64), 0, ctx->stream, (const SeqRef *)P->seqs.p,
+                           (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (Cand *)ctx->cands.p,
+                           (SpRec *)ctx->sps.p, (int *)ctx->spcount.p);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(spc.data(), ctx->spcount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost,
@@ -675,21 +646,13 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         HIPCHK(ctx, hipMemcpyAsync(ctx->tbitems.p, tb.data(), tb.size() * sizeof(TbItem), hipMemcpyHostToDevice,
                                    ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-        if (ctx->affine)
-            hipLaunchKernelGGL(walk_kernel<true>, dim3(tb_grid), dim3(64), 0, ctx->stream, (const uint8_t *)ctx->res.p,
-                               (const SeqRef *)ctx->seqs.p, (const TbItem *)ctx->tbitems.p, ng,
-                               (uint8_t *)ctx->dirs.p, (const SpRec *)ctx->sps.p, (const int *)ctx->spcount.p,
-                               (uint64_t *)ctx->keys.p, (HitRec *)ctx->hits.p, (char *)ctx->ops.p,
-                               (int64_t *)ctx->opsused.p, minimum);
-        else
-            hipLaunchKernelGGL(walk_kernel<false>, dim3(tb_grid), dim3(64), 0, ctx->stream,
-                               (const uint8_t *)ctx->res.p, (const SeqRef *)ctx->seqs.p,
-                               (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (const SpRec *)ctx->sps.p,
-                               (const int *)ctx->spcount.p, (uint64_t *)ctx->keys.p, (HitRec *)ctx->hits.p,
-                               (char *)ctx->ops.p, (int64_t *)ctx->opsused.p, minimum);
+        auto wk = ctx->affine ? walk_kernel<true> : walk_kernel<false>;
+        hipLaunchKernelGGL(wk, dim3(tb_grid), dim3(64), 0, ctx->stream, (const uint8_t *)P->res.p,
+                           (const SeqRef *)P->seqs.p, (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p,
+                           (const SpRec *)ctx->sps.p, (const int *)ctx->spcount.p, (uint64_t *)ctx->keys.p,
+                           (HitRec *)ctx->hits.p, (char *)ctx->ops.p, (int64_t *)ctx->opsused.p, minimum);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
-        // gather
         std::vector<SpRec> hsp((size_t)sp_total);
         std::vector<HitRec> hh((size_t)sp_total);
         std::vector<char> hops((size_t)ops_total);
@@ -710,16 +673,15 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
             HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
             ms_walk += t;
         }
-        // compact: per gated pair, its starting points (emission order), hits
-        // (walk order) and op characters
+        // per gated pair: starting points (emission order), hits (walk order), op characters
         static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
         for (int g = 0; g < ng; ++g) {
             const int64_t sp0 = (int64_t)R->sps.size();
             const int n = spc[g];
             for (int k = 0; k < n; ++k) {
-                pw_starting_point s;
-                std::memcpy(&s, &hsp[tb[g].sp_off + k], sizeof s);
-                R->sps.push_back(s);
+                pw_starting_point sp;
+                std::memcpy(&sp, &hsp[tb[g].sp_off + k], sizeof sp);
+                R->sps.push_back(sp);
             }
             const int64_t op0 = (int64_t)R->ops.size();
             R->ops.insert(R->ops.end(), hops.begin() + tb[g].ops_off, hops.begin() + tb[g].ops_off + opsused[g]);
@@ -743,35 +705,140 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
             }
         }
     }
-    // the reference keeps at most 100 starting points per pair of a batch in
-    // total (SmithWaterman.py:683-685); report what a cap would drop
+    // the reference keeps at most sp_cap starting points per pair of a batch in
+    // total (SmithWaterman.py:683-685); report how many a cap would drop
     int64_t truncated = 0;
     if (sp_cap > 0) {
         const int64_t cap_total = (int64_t)sp_cap * n_pairs;
         if ((int64_t)R->sps.size() >= cap_total) truncated = (int64_t)R->sps.size() - cap_total;
     }
-    pw_result &P = R->pub;
-    P.n_pairs = n_pairs;
-    P.maxima = R->maxima.data();
-    P.pair_query = R->pq.data();
-    P.pair_target = R->pt.data();
-    P.n_starting_points = (int64_t)R->sps.size();
-    P.starting_points = R->sps.data();
-    P.n_hits = (int64_t)R->hits.size();
-    P.hits = R->hits.data();
-    P.n_ops = (int64_t)R->ops.size();
-    P.ops = R->ops.data();
-    P.n_traceback_pairs = ng;
-    P.sp_truncated = truncated;
-    P.cells = cells;
-    P.ms_score = ms_score;
-    P.ms_directions = ms_dir;
-    P.ms_traceback = ms_tb;
-    P.ms_walk = ms_walk;
-    P.score_launches = launches;
-    P.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    pw_result &O = R->pub;
+    O.n_pairs = n_pairs;
+    O.maxima = R->maxima.data();
+    O.pair_query = R->pq.data();
+    O.pair_target = R->pt.data();
+    O.n_starting_points = (int64_t)R->sps.size();
+    O.starting_points = R->sps.data();
+    O.n_hits = (int64_t)R->hits.size();
+    O.hits = R->hits.data();
+    O.n_ops = (int64_t)R->ops.size();
+    O.ops = R->ops.data();
+    O.n_traceback_pairs = ng;
+    O.sp_truncated = truncated;
+    O.cells = P->cells;
+    O.ms_score = ms_score;
+    O.ms_directions = ms_dir;
+    O.ms_traceback = ms_tb;
+    O.ms_walk = ms_walk;
+    O.score_launches = launches;
+    O.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    return PW_OK;
+}
+
+int ready(pw_ctx *ctx) {
+    if (!ctx) return PW_EINVAL;
+    if (!ctx->scoring_set) {
+        ctx->err = "pw_set_scoring has not been called";
+        return PW_EINVAL;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                          const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                          const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
+                          const uint8_t *same_id, int32_t mode, pw_plan **out) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!out || nq < 0 || nt < 0 || (n_pairs > 0 && (!pq || !pt))) {
+        ctx->err = "pw_prepare: missing argument";
+        return PW_EINVAL;
+    }
+    *out = nullptr;
+    pw_plan *P = new pw_plan();
+    rc = prepare(ctx, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, min_start_score, same_id,
+                 mode);
+    if (rc) {
+        free_plan(P);
+        return rc;
+    }
+    *out = P;
+    return PW_OK;
+}
+
+extern "C" int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!plan || !out) {
+        ctx->err = "pw_execute: missing argument";
+        return PW_EINVAL;
+    }
+    *out = nullptr;
+    HostResult *R = new HostResult();
+    std::memset(&R->pub, 0, sizeof(R->pub));
+    rc = execute(ctx, plan, minimum, sp_cap, R);
+    if (rc) {
+        delete R;
+        return rc;
+    }
     *out = &R->pub;
     return PW_OK;
+}
+
+extern "C" int pw_release_plan(pw_ctx *ctx, pw_plan *plan) {
+    if (!ctx || !plan) return PW_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_plan(plan);
+    return PW_OK;
+}
+
+extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                              const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                              const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
+                              double *ms_out) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (nq < 0 || nt < 0 || (n_pairs > 0 && (!pq || !pt || !maxima_out))) {
+        ctx->err = "pw_score_pairs: missing argument";
+        return PW_EINVAL;
+    }
+    pw_plan *P = new pw_plan();
+    rc = prepare(ctx, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, nullptr, nullptr, mode);
+    double ms = 0;
+    if (!rc) rc = run_score(ctx, P, &ms, nullptr);
+    if (!rc && n_pairs > 0) {
+        hipError_t e = hipMemcpy(maxima_out, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipMemcpy maxima: ") + hipGetErrorString(e);
+            rc = PW_EDEVICE;
+        }
+    }
+    free_plan(P);
+    if (!rc && ms_out) *ms_out = ms;
+    return rc;
+}
+
+extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                              const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                              const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
+                              const uint8_t *same_id, double minimum, int32_t sp_cap, int32_t mode, pw_result **out) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!out || (n_pairs > 0 && !min_start_score)) {
+        ctx->err = "pw_align_pairs: missing argument";
+        return PW_EINVAL;
+    }
+    pw_plan *P = nullptr;
+    rc = pw_prepare(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, min_start_score, same_id, mode,
+                    &P);
+    if (rc) return rc;
+    rc = pw_execute(ctx, P, minimum, sp_cap, out);
+    free_plan(P);
+    return rc;
 }
 
 extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res) {

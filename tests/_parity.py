@@ -111,7 +111,7 @@ def compare(O, queries, targets, p, res, qids=None, tids=None):
 def decode_ops(ops: bytes, off: int, n: int, q: str, t: str, s_start: int, t_start: int):
     x, y = s_start, t_start
     sm, al, tm = [], [], []
-    for k in range(n):
+    for k in range(n - 1, -1, -1):  # recorded from the alignment end back to its start
         op = ops[off + k]
         if op == ord("D"):
             a, b = q[x], t[y]

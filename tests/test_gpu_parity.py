@@ -27,7 +27,7 @@ def dna_params(O, affine, minimum=10.0, ll=1.0, ff=0.2):
 
 def blosum_params(O, minimum=30.0, ll=1.0, ff=0.05):
     return O.Params(matrix=O.BLOSUM62, gap=-11.0, gap_ext=-1.0, lower_limit=ll, minimum=minimum, filter_factor=ff,
-                    highest=11.0, dna=False)
+                    highest=float(max(max(O.BLOSUM62))), dna=False)
 
 
 CASES = [
