@@ -44,8 +44,8 @@ struct pw_ctx {
     int dim = 0, char_offset = 0;
     float gap = 0, ext = 0, ll = 1, minimum = 0;
     bool affine = false;
-    DevBuf res, seqs, items, trans, tables, scratch, queue, maxima, ditems, dirs, ccount, cands, tbitems, sps,
-        spcount, keys, hits, ops, opsused;
+    DevBuf res, seqs, items, trans, tables, scratch, scratch2, queue, maxima, ditems, dirs, ccount, cands, tbitems,
+        sps, spcount, keys, hits, ops, opsused;
 };
 
 static thread_local std::string g_init_err;
@@ -62,6 +62,10 @@ struct Encoded {
 
 }  // namespace
 
+// chunk widths of the lane-per-pair kernel
+constexpr int NB_KINDS = 6;
+constexpr int NB_WIDTHS[NB_KINDS] = {16, 32, 48, 64, 80, 96};
+
 // A prepared job: sequences, substitution tables and the scoring work list
 // resident on the device, plus the host copies the later stages need.
 struct pw_plan {
@@ -74,7 +78,12 @@ struct pw_plan {
     std::vector<uint8_t> same;
     int64_t max_b = 0;
     double cells = 0;
-    DevBuf res, seqs, tables, items, trans;
+    int64_t n_tile = 0;        // pairs scored by the wave-per-pair tile kernel
+    int64_t n_groups = 0;      // lane-per-pair jobs
+    int64_t group_rows = 0;    // longest row sequence of a multi-chunk job (scratch sizing)
+    int32_t max_nb = 16;
+    int64_t nb_begin[NB_KINDS] = {}, nb_end[NB_KINDS] = {};  // job ranges per chunk width
+    DevBuf res, seqs, tables, items, trans, gitems, gents;
 };
 
 #define HIPCHK(ctx, call)                                                                   \
@@ -163,7 +172,8 @@ extern "C" int pw_destroy(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->res, &ctx->seqs, &ctx->items, &ctx->trans, &ctx->tables, &ctx->scratch, &ctx->queue,
+    DevBuf *bufs[] = {&ctx->res, &ctx->seqs, &ctx->items, &ctx->trans, &ctx->tables, &ctx->scratch, &ctx->scratch2,
+                      &ctx->queue,
                       &ctx->maxima, &ctx->ditems, &ctx->dirs, &ctx->ccount, &ctx->cands, &ctx->tbitems, &ctx->sps,
                       &ctx->spcount, &ctx->keys, &ctx->hits, &ctx->ops, &ctx->opsused};
     for (DevBuf *b : bufs)
@@ -317,6 +327,18 @@ void choose_shape(int qlen, int tlen, int &trans, int &R) {
     }
 }
 
+// pairs sharing a target needed before they are scored lane-per-pair, and
+// the longest row sequence such a job takes
+constexpr int64_t GROUP_MIN_PAIRS = 16;
+constexpr int GROUP_MAX_ROWS = 1 << 15;
+
+// columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
+// chunks of 64 beyond
+int group_nb(int clen) {
+    if (clen <= 96) return std::max(16, (clen + 15) / 16 * 16);
+    return 64;
+}
+
 int dir_R(int qlen) {
     int r = 1;
     while (r < 8 && 64 * r < qlen) r *= 2;
@@ -351,6 +373,8 @@ void free_plan(pw_plan *P) {
     free_buf(P->tables);
     free_buf(P->items);
     free_buf(P->trans);
+    free_buf(P->gitems);
+    free_buf(P->gents);
     delete P;
 }
 
@@ -392,34 +416,102 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
         for (int64_t i = 0; i < n_pairs; ++i) items[i] = ScoreItem{pq[i], nq + pt[i], (int32_t)i, 4, X, Y};
         max_b = Y;
     } else {
-        std::vector<std::pair<double, int64_t>> order((size_t)n_pairs);
-        for (int64_t i = 0; i < n_pairs; ++i) {
+        // lane-per-pair jobs for pairs whose target is shared by enough pairs;
+        // the rest (long or unshared pairs) go to the wave-per-pair tile kernel
+        std::vector<std::vector<int64_t>> by_target((size_t)nt);
+        for (int64_t i = 0; i < n_pairs; ++i) by_target[pt[i]].push_back(i);
+        std::vector<int64_t> tile_pairs;
+        std::vector<GroupItem> gitems;
+        std::vector<GroupEntry> gents;
+        std::vector<double> gcost;
+        for (int32_t t = 0; t < nt; ++t) {
+            std::vector<int64_t> &L = by_target[t];
+            if (L.empty()) continue;
+            std::vector<int64_t> grp;
+            for (int64_t i : L)
+                if ((int64_t)L.size() >= GROUP_MIN_PAIRS && qlen[pq[i]] <= GROUP_MAX_ROWS) grp.push_back(i);
+                else tile_pairs.push_back(i);
+            std::stable_sort(grp.begin(), grp.end(),
+                             [&](int64_t x, int64_t y) { return qlen[pq[x]] > qlen[pq[y]]; });
+            const int tl = tlen[t];
+            const int nb = group_nb(tl);
+            const int nch = tl > nb ? (tl + nb - 1) / nb : 1;
+            for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
+                const int cnt = (int)std::min<size_t>(64, grp.size() - g0);
+                GroupItem gi;
+                gi.c_seq = nq + t;
+                gi.entry_off = (int32_t)gents.size();
+                gi.count = cnt;
+                gi.rows = qlen[pq[grp[g0]]];
+                gi.nb = nb;
+                gi.trans = 0;
+                for (int u = 0; u < cnt; ++u) gents.push_back(GroupEntry{pq[grp[g0 + u]], (int32_t)grp[g0 + u]});
+                gitems.push_back(gi);
+                gcost.push_back((double)gi.rows * nch * nb);
+                if (nch > 1) P->group_rows = std::max<int64_t>(P->group_rows, gi.rows);
+                P->max_nb = std::max(P->max_nb, nb);
+            }
+        }
+        // longest jobs first for the persistent workers
+        std::vector<int64_t> gorder(gitems.size());
+        std::iota(gorder.begin(), gorder.end(), 0);
+        std::stable_sort(gorder.begin(), gorder.end(), [&](int64_t x, int64_t y) {
+            if (gitems[x].nb != gitems[y].nb) return gitems[x].nb < gitems[y].nb;
+            return gcost[x] > gcost[y];
+        });
+        std::vector<GroupItem> gsorted(gitems.size());
+        for (size_t i = 0; i < gorder.size(); ++i) gsorted[i] = gitems[gorder[i]];
+        for (int w = 0; w < NB_KINDS; ++w) {
+            P->nb_begin[w] = P->nb_end[w] = 0;
+            for (size_t i = 0; i < gsorted.size(); ++i)
+                if (gsorted[i].nb == NB_WIDTHS[w]) {
+                    if (P->nb_end[w] == 0) P->nb_begin[w] = (int64_t)i;
+                    P->nb_end[w] = (int64_t)i + 1;
+                }
+        }
+        P->n_groups = (int64_t)gsorted.size();
+        ENSURE(ctx, P->gitems, gsorted.size() * sizeof(GroupItem));
+        ENSURE(ctx, P->gents, gents.size() * sizeof(GroupEntry));
+        if (!gsorted.empty()) {
+            HIPCHK(ctx, hipMemcpyAsync(P->gitems.p, gsorted.data(), gsorted.size() * sizeof(GroupItem),
+                                       hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(P->gents.p, gents.data(), gents.size() * sizeof(GroupEntry),
+                                       hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        }
+
+        const int64_t ntile = (int64_t)tile_pairs.size();
+        items.resize((size_t)ntile);
+        trans.assign((size_t)ntile, 0);
+        std::vector<std::pair<double, int64_t>> order((size_t)ntile);
+        for (int64_t k = 0; k < ntile; ++k) {
+            const int64_t i = tile_pairs[k];
             const int ql = qlen[pq[i]], tl = tlen[pt[i]];
             int tr, R;
             choose_shape(ql, tl, tr, R);
             const int a = tr ? tl : ql, b = tr ? ql : tl;
             const int aseq = tr ? nq + pt[i] : pq[i], bseq = tr ? pq[i] : nq + pt[i];
-            items[i] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b};
-            trans[i] = (uint8_t)tr;
+            items[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b};
+            trans[k] = (uint8_t)tr;
             const int rows = 64 * R;
-            order[i] = {-(double)((a + rows - 1) / rows) * (b + 63) * (R + 1.5), i};
+            order[k] = {-(double)((a + rows - 1) / rows) * (b + 63) * (R + 1.5), k};
             max_b = std::max<int64_t>(max_b, b);
         }
-        // longest first for the persistent workers
         std::stable_sort(order.begin(), order.end(),
                          [](const std::pair<double, int64_t> &x, const std::pair<double, int64_t> &y) {
                              return x.first < y.first;
                          });
-        std::vector<ScoreItem> sorted((size_t)n_pairs);
-        std::vector<uint8_t> strans((size_t)n_pairs);
-        for (int64_t i = 0; i < n_pairs; ++i) {
-            sorted[i] = items[order[i].second];
-            strans[i] = trans[order[i].second];
+        std::vector<ScoreItem> sorted((size_t)ntile);
+        std::vector<uint8_t> strans((size_t)ntile);
+        for (int64_t k = 0; k < ntile; ++k) {
+            sorted[k] = items[order[k].second];
+            strans[k] = trans[order[k].second];
         }
         items.swap(sorted);
         trans.swap(strans);
     }
     P->max_b = max_b;
+    P->n_tile = (int64_t)items.size();
     ENSURE(ctx, P->res, E.codes.size());
     ENSURE(ctx, P->seqs, E.refs.size() * sizeof(SeqRef));
     ENSURE(ctx, P->tables, E.tables.size() * sizeof(float));
@@ -430,7 +522,7 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
                                ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(P->tables.p, E.tables.data(), E.tables.size() * sizeof(float),
                                hipMemcpyHostToDevice, ctx->stream));
-    if (n_pairs > 0) {
+    if (!items.empty()) {
         HIPCHK(ctx, hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(ScoreItem),
                                    hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(P->trans.p, trans.data(), trans.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -439,25 +531,69 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
     return PW_OK;
 }
 
+const void *group_kernel_for(bool affine, int nb) {
+    switch (nb) {
+    case 16: return affine ? (const void *)group_kernel<true, 16> : (const void *)group_kernel<false, 16>;
+    case 32: return affine ? (const void *)group_kernel<true, 32> : (const void *)group_kernel<false, 32>;
+    case 48: return affine ? (const void *)group_kernel<true, 48> : (const void *)group_kernel<false, 48>;
+    case 64: return affine ? (const void *)group_kernel<true, 64> : (const void *)group_kernel<false, 64>;
+    case 80: return affine ? (const void *)group_kernel<true, 80> : (const void *)group_kernel<false, 80>;
+    default: return affine ? (const void *)group_kernel<true, 96> : (const void *)group_kernel<false, 96>;
+    }
+}
+
+int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds) != hipSuccess || per_cu <= 0)
+        per_cu = 8;
+    const int64_t g = (int64_t)ctx->num_cus * per_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(work, g));
+}
+
 // Scoring pass over all pairs of the plan; maxima land in ctx->maxima.
 int run_score(pw_ctx *ctx, pw_plan *P, double *ms_score, int64_t *launches) {
     const int64_t n_pairs = P->n_pairs;
     ENSURE(ctx, ctx->maxima, (size_t)n_pairs * sizeof(float));
-    ENSURE(ctx, ctx->queue, 64);
-    const size_t lds = lds_bytes(P->E.KP);
-    const int64_t want = (int64_t)ctx->num_cus * 8;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(n_pairs, want));
-    const int64_t scr_stride = 3 * (P->max_b + 64);
-    ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float));
-    unsigned *queue = P->mode == 0 ? nullptr : (unsigned *)ctx->queue.p;
-    HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64, ctx->stream));
+    // work counters on separate 64-byte lines: [0] tile jobs, [16*(1+w)] jobs of chunk width w
+    ENSURE(ctx, ctx->queue, 64 * (1 + NB_KINDS));
+    HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64 * (1 + NB_KINDS), ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if (n_pairs > 0) {
+    if (P->n_groups > 0) {
+        const int64_t scr_stride = std::max<int64_t>(1, P->group_rows) * WAVE;
+        for (int w = 0; w < NB_KINDS; ++w) {
+            const int64_t n = P->nb_end[w] - P->nb_begin[w];
+            if (n <= 0) continue;
+            const void *kern = group_kernel_for(ctx->affine, NB_WIDTHS[w]);
+            const size_t lds = (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float);
+            const int grid = persistent_grid(ctx, kern, lds, n);
+            ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float2));
+            const GroupItem *items = (const GroupItem *)P->gitems.p + P->nb_begin[w];
+            const uint8_t *res = (const uint8_t *)P->res.p;
+            const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+            const GroupEntry *ents = (const GroupEntry *)P->gents.p;
+            const float *tabs = (const float *)P->tables.p;
+            int kp = P->E.KP, ni = (int)n;
+            float g = ctx->gap, e = ctx->ext;
+            float2 *scr = (float2 *)ctx->scratch.p;
+            int64_t ss = scr_stride;
+            unsigned *q = (unsigned *)ctx->queue.p + 16 * (1 + w);
+            float *mx = (float *)ctx->maxima.p;
+            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &g, &e, &scr, &ss, &q, &mx};
+            HIPCHK(ctx, hipLaunchKernel(kern, dim3(grid), dim3(64), args, lds, ctx->stream));
+            if (launches) *launches += 1;
+        }
+    }
+    if (P->n_tile > 0) {
         auto kern = ctx->affine ? score_kernel<true> : score_kernel<false>;
+        const size_t lds = lds_bytes(P->E.KP);
+        const int grid = persistent_grid(ctx, (const void *)kern, lds, P->n_tile);
+        const int64_t scr_stride = 3 * (P->max_b + 64);
+        ENSURE(ctx, ctx->scratch2, (size_t)grid * scr_stride * sizeof(float));
+        unsigned *queue = P->mode == 0 ? nullptr : (unsigned *)ctx->queue.p;
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
-                           (const SeqRef *)P->seqs.p, (const ScoreItem *)P->items.p, (int)n_pairs,
+                           (const SeqRef *)P->seqs.p, (const ScoreItem *)P->items.p, (int)P->n_tile,
                            (const uint8_t *)P->trans.p, (const float *)P->tables.p, P->E.KP, P->E.lkp, ctx->gap,
-                           ctx->ext, (float *)ctx->scratch.p, scr_stride, queue, (float *)ctx->maxima.p);
+                           ctx->ext, (float *)ctx->scratch2.p, scr_stride, queue, (float *)ctx->maxima.p);
         HIPCHK(ctx, hipGetLastError());
         if (launches) *launches += 1;
     }

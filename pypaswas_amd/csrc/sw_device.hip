@@ -330,6 +330,147 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lane-per-pair scoring.  One job = one column sequence C, shared by up to 64
+// pairs, and one row sequence per lane.  The wave builds a substitution
+// profile of C in LDS (one row of NB columns per residue code); each lane
+// sweeps its own row sequence top to bottom with the NB column states of the
+// previous row in registers, so there is no skew and no lane exchange.
+// C longer than NB is processed in chunks of NB columns; the right edge of a
+// chunk (T and the I candidate of every row) goes through a per-wave scratch
+// column, row-major with the 64 lanes contiguous.
+// Recurrences per cell (r, c), row = lane sequence, column = C:
+//   M = max(T[r-1][c-1] + s, 0)           T = max(M, I, J)
+//   I = max(I[r][c-1] + E, M[r][c-1] + G) J = max(J[r-1][c] + E, M[r-1][c] + G)
+// (linear: H = max(H[r-1][c-1] + s, max(H[r-1][c], H[r][c-1]) + G, 0)).
+// ---------------------------------------------------------------------------
+template <bool AFF, int NB>
+__device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs, const GroupItem &it,
+                            const GroupEntry *__restrict__ ents, const float *__restrict__ gtab, int kp,
+                            float *prof, float2 *__restrict__ scr, float G, float E, int fill) {
+    constexpr int NBP = NB + 4;
+    const int lane = threadIdx.x & 63;
+    const SeqRef C = seqs[it.c_seq];
+    const bool active = lane < it.count;
+    SeqRef Rw;
+    Rw.off = 0;
+    Rw.len = 0;
+    if (active) Rw = seqs[ents[it.entry_off + lane].row_seq];
+    const int nchunks = C.len > NB ? (C.len + NB - 1) / NB : 1;
+    const float *tab = gtab + (it.trans ? kp * kp : 0);
+    const float bJ = AFF ? G : 0.0f;
+    float rmax = 0.0f;
+    for (int k = 0; k < nchunks; ++k) {
+        const int c0 = k * NB;
+        __syncthreads();
+        for (int idx = lane; idx < kp * NB; idx += WAVE) {
+            const int a = idx / NB, c = idx - a * NB;
+            const int col = c0 + c;
+            const int b = col < C.len ? (int)res[C.off + col] : fill;
+            prof[a * NBP + c] = tab[b * kp + a];
+        }
+        __syncthreads();
+        const bool left_in = k > 0, right_out = k + 1 < nchunks;
+        float Tp[NB], Jc[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            Tp[c] = 0.0f;
+            Jc[c] = bJ;
+        }
+        float dl = 0.0f;  // T[r-1][c0-1]
+        int a_next = (active && 0 < Rw.len) ? (int)res[Rw.off] : fill;
+        float2 lb_next = make_float2(0.0f, bJ);
+        if (left_in && it.rows > 0) {
+            lb_next.x = load_scr(&scr[lane].x);
+            lb_next.y = load_scr(&scr[lane].y);
+        }
+#pragma unroll 1
+        for (int r = 0; r < it.rows; ++r) {
+            const int a = a_next;
+            const float2 lb = lb_next;
+            a_next = (active && r + 1 < Rw.len) ? (int)res[Rw.off + r + 1] : fill;
+            if (left_in && r + 1 < it.rows) {
+                lb_next.x = load_scr(&scr[(r + 1) * WAVE + lane].x);
+                lb_next.y = load_scr(&scr[(r + 1) * WAVE + lane].y);
+            }
+            const float *prow = prof + a * NBP;
+            float diag = dl;   // T[r-1][c-1] for the first column of the block
+            float ic = lb.y;   // affine: I candidate entering the column; linear: H[r][c-1]
+#pragma unroll
+            for (int b = 0; b < NB; b += 4) {
+                const float4 s4 = *reinterpret_cast<const float4 *>(prow + b);
+                const float dg0 = diag + s4.x;
+                const float dg1 = Tp[b] + s4.y;
+                const float dg2 = Tp[b + 1] + s4.z;
+                const float dg3 = Tp[b + 2] + s4.w;
+                diag = Tp[b + 3];
+                const float dg[4] = {dg0, dg1, dg2, dg3};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int c = b + u;
+                    if (AFF) {
+                        const float M = __builtin_fmaxf(dg[u], 0.0f);
+                        const float MG = M + G;
+                        const float T = fmax3(M, ic, Jc[c]);
+                        Jc[c] = __builtin_fmaxf(Jc[c] + E, MG);
+                        ic = __builtin_fmaxf(ic + E, MG);
+                        Tp[c] = T;
+                    } else {
+                        const float h = fmax3(dg[u], __builtin_fmaxf(Tp[c], ic) + G, 0.0f);
+                        Tp[c] = h;
+                        ic = h;
+                    }
+                }
+                rmax = fmax3(rmax, fmax3(Tp[b], Tp[b + 1], Tp[b + 2]), Tp[b + 3]);
+                // Materialise this block's column state here: without it the
+                // compiler sinks the T and J updates to the end of the row and
+                // keeps ~4 registers per column alive.  The empty asm emits no
+                // instruction; it only pins the values in registers.
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (AFF) asm volatile("" : "+v"(Tp[b + u]), "+v"(Jc[b + u]));
+                    else asm volatile("" : "+v"(Tp[b + u]));
+                }
+                asm volatile("" : "+v"(rmax), "+v"(ic), "+v"(diag));
+            }
+            if (right_out) scr[r * WAVE + lane] = make_float2(Tp[NB - 1], ic);
+            dl = lb.x;
+        }
+        __threadfence();
+    }
+    return rmax;
+}
+
+// One kernel per chunk width, so that each width gets its own register
+// budget; the host launches one per width present, over that width's jobs.
+template <bool AFF, int NB>
+__global__ void __launch_bounds__(64) group_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
+                                                   const GroupItem *__restrict__ items, int nitems,
+                                                   const GroupEntry *__restrict__ ents,
+                                                   const float *__restrict__ tables, int kp, float G, float E,
+                                                   float2 *__restrict__ scratch, int64_t scr_stride,
+                                                   unsigned *__restrict__ queue, float *__restrict__ maxima) {
+    extern __shared__ float lds[];
+    float2 *scr = scratch + (int64_t)blockIdx.x * scr_stride;
+    const int fill = kp - 1;
+    int k = 0;
+    for (;;) {
+        int item;
+        if (queue) {
+            unsigned v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(queue, 1u);
+            item = (int)__builtin_amdgcn_readfirstlane(v);
+        } else {
+            item = (int)blockIdx.x + k * (int)gridDim.x;
+            ++k;
+        }
+        if (item >= nitems) break;
+        const GroupItem it = items[item];
+        const float m = group_tile<AFF, NB>(res, seqs, it, ents, tables, kp, lds, scr, G, E, fill);
+        if ((int)threadIdx.x < it.count) maxima[ents[it.entry_off + threadIdx.x].pair] = m;
+    }
+}
+
 // Direction pass over pairs that passed the traceback gate.
 template <bool AFF>
 __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,

@@ -32,6 +32,22 @@ struct ScoreItem {
     int32_t blen;    // columns to iterate
 };
 
+// One job of the lane-per-pair scoring kernel: a column sequence shared by up
+// to 64 pairs (one per lane), each lane sweeping its own row sequence.
+struct GroupItem {
+    int32_t c_seq;      // column sequence (held as a substitution profile in LDS)
+    int32_t entry_off;  // first GroupEntry of this job
+    int32_t count;      // lanes in use (<= 64)
+    int32_t rows;       // longest row sequence of the job
+    int32_t nb;         // columns per chunk: 16, 32, 48, 64, 80 or 96
+    int32_t trans;      // 1 when the column sequence is the query
+};
+
+struct GroupEntry {
+    int32_t row_seq;
+    int32_t pair;
+};
+
 // One pair of the direction pass (query along lanes, target swept).
 struct DirItem {
     int32_t q_seq, t_seq;

@@ -79,3 +79,34 @@ def test_repeats_and_ties(dev, oracle):
         p = dna_params(oracle, affine, minimum=3.0, ll=1.0, ff=0.0)
         res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
         compare(oracle, qs, ts, p, res)
+
+
+GROUP_CASES = [
+    # enough queries per target to be scored lane-per-pair; targets above 96
+    # residues are processed in chunks of 64 columns
+    ("group-dna-linear", DNA, 70, 4, (30, 400), (12, 300), False),
+    ("group-dna-affine", DNA, 70, 4, (30, 400), (12, 300), True),
+    ("group-dna-affine-short", DNA, 130, 3, (50, 260), (20, 80), True),
+]
+
+
+@pytest.mark.parametrize("case", GROUP_CASES, ids=[c[0] for c in GROUP_CASES])
+def test_group_kernel_parity(dev, oracle, case):
+    name, alpha, nq, nt, ql, tl, affine = case
+    print("case", name, flush=True)
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    qs, ts = planted_set(rng, nq, nt, ql, tl, alpha, plant=0.6)
+    p = dna_params(oracle, affine, minimum=8.0)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs > 0
+
+
+def test_group_kernel_protein(dev, oracle):
+    rng = np.random.default_rng(99)
+    qs, ts = planted_set(rng, 40, 3, (30, 500), (50, 700), AMINO, plant=0.8, sub=0.5, indel=0.05)
+    p = blosum_params(oracle, minimum=25.0)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    res0 = hip_all_vs_all(dev, oracle, qs, ts, p, mode=0)
+    assert np.array_equal(res0.maxima, res.maxima)
