@@ -60,7 +60,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-batch launch")
     ap.add_argument("--cpu-reads", type=int, default=800, help="reads in the CPU-baseline sample")
+    ap.add_argument("--no-protein", action="store_true", help="skip the uneven-length protein comparison")
+    ap.add_argument("--protein", default="100x400", help="queries x targets of the protein sample")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device three times (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
               "form (the sharding plan is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -127,6 +134,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(q, t, settings, score, dev, args.cpu_reads)
 
+    protein = None
+    if not args.no_protein:  # last: it changes the device's scoring constants
+        nq, nt = (int(v) for v in args.protein.split("x"))
+        protein = protein_comparison(dev, nq, nt)
+
     out = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -140,6 +152,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_launch": plain,
         "batched_over_plain": round(gcups / plain["gcups"], 2) if plain else None,
+        "protein_uneven": protein,
         "traceback_pairs": last.n_traceback_pairs,
         "hits": int(last.hits["shown"].sum()) if len(last.hits) else 0,
         "stage_ms": {"score": round(last.ms_score, 3), "directions": round(last.ms_directions, 3),
@@ -147,6 +160,58 @@ def main():
                      "call_wall": round(last.ms_total, 3)},
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copied %d MiB x 3" % mib)
+
+
+def protein_comparison(dev, nq, nt):
+    """Uneven-length protein all-vs-all sample (BLOSUM62, -11/-1; lengths lognormal
+    median 350, sigma 0.6, clipped 30-5000): batched persistent vs plain launch,
+    same pairs, results compared."""
+    import logging
+    from pypaswas_amd import workloads as W
+    from pypaswas_amd.Core.Scores import Blosum62Score
+    from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
+    queries, targets, overrides = W.config3(n_queries=nq, n_targets=nt)
+    settings = W.settings_dict(overrides)
+    score = Blosum62Score(logging.getLogger("bench"), settings)
+    q, t = W.reference_order(queries, targets, False)
+    qp, tp, pq, pt, mss, same = job_arrays(q, t, score.highest_score, float(settings.filter_factor))
+    dev.set_scoring(score.to_float32(), ord(score.char_offset), score.gap_score, score.gap_extension,
+                    score.lower_limit_score, score.minimum_score)
+    minimum = float(settings.minimum_score)
+    out = {"pairs": int(len(pq)), "workload": "%d x %d proteins, lognormal(350, 0.6) clipped 30-5000, BLOSUM62 "
+                                              "-11/-1, 100 families at ~40%% pairwise identity" % (nq, nt)}
+    res = {}
+    for mode, name in ((1, "batched"), (0, "plain")):
+        plan = dev.prepare(qp, tp, pq, pt, mss, same, mode=mode)
+        dev.execute(plan, minimum)
+        t0 = time.perf_counter()
+        r = dev.execute(plan, minimum)
+        el = time.perf_counter() - t0
+        plan.release()
+        res[name] = r
+        out[name + "_gcups"] = round(r.cells / el / 1e9, 2)
+        out[name + "_score_kernel_ms"] = round(r.ms_score, 3)
+        log("protein %s: %.1f ms, %.2f GCUPS (score kernels %.2f ms)" % (name, el * 1000, r.cells / el / 1e9,
+                                                                        r.ms_score))
+    out["cells"] = res["batched"].cells
+    out["batched_over_plain"] = round(out["batched_gcups"] / out["plain_gcups"], 2)
+    out["identical_results"] = bool(np.array_equal(res["batched"].maxima, res["plain"].maxima)
+                                    and np.array_equal(res["batched"].hits, res["plain"].hits)
+                                    and res["batched"].ops == res["plain"].ops)
+    out["hits"] = int(res["batched"].hits["shown"].sum()) if len(res["batched"].hits) else 0
+    return out
 
 
 def cpu_baseline(q, t, settings, score, dev, n_reads):

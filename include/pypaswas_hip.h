@@ -49,9 +49,7 @@ typedef struct {
 
 typedef struct {
     int64_t n_pairs;
-    const float *maxima;                 /* [n_pairs] pair maximum (globalMaxima of the last block) */
-    const int32_t *pair_query;           /* [n_pairs] */
-    const int32_t *pair_target;          /* [n_pairs] */
+    const float *maxima;                 /* [n_pairs] pair maximum (globalMaxima of the last block), in pair order */
     int64_t n_starting_points;
     const pw_starting_point *starting_points; /* per pair, emission order */
     int64_t n_hits;

@@ -45,7 +45,6 @@ class HitRec(ctypes.Structure):
 
 class ResultRec(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_int64), ("maxima", ctypes.POINTER(ctypes.c_float)),
-                ("pair_query", ctypes.POINTER(ctypes.c_int32)), ("pair_target", ctypes.POINTER(ctypes.c_int32)),
                 ("n_starting_points", ctypes.c_int64), ("starting_points", ctypes.POINTER(StartingPointRec)),
                 ("n_hits", ctypes.c_int64), ("hits", ctypes.POINTER(HitRec)), ("n_ops", ctypes.c_int64),
                 ("ops", ctypes.POINTER(ctypes.c_char)), ("n_traceback_pairs", ctypes.c_int64),
@@ -118,8 +117,6 @@ class PackedSeqs:
 @dataclass
 class AlignResult:
     maxima: np.ndarray          # [n_pairs] f32
-    pair_query: np.ndarray
-    pair_target: np.ndarray
     starting_points: np.ndarray  # structured, 36-byte records
     hits: np.ndarray             # structured
     ops: bytes
@@ -217,8 +214,6 @@ class Device:
             r = out.contents
             n = int(r.n_pairs)
             maxima = np.ctypeslib.as_array(r.maxima, shape=(n,)).copy() if n else np.zeros(0, np.float32)
-            pqa = np.ctypeslib.as_array(r.pair_query, shape=(n,)).copy() if n else np.zeros(0, np.int32)
-            pta = np.ctypeslib.as_array(r.pair_target, shape=(n,)).copy() if n else np.zeros(0, np.int32)
             nsp = int(r.n_starting_points)
             sps = (np.frombuffer(ctypes.string_at(r.starting_points, nsp * SP_DTYPE.itemsize), dtype=SP_DTYPE).copy()
                    if nsp else np.zeros(0, SP_DTYPE))
@@ -226,7 +221,7 @@ class Device:
             hits = (np.frombuffer(ctypes.string_at(r.hits, nh * HIT_DTYPE.itemsize), dtype=HIT_DTYPE).copy()
                     if nh else np.zeros(0, HIT_DTYPE))
             ops = ctypes.string_at(r.ops, int(r.n_ops)) if r.n_ops else b""
-            return AlignResult(maxima, pqa, pta, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
+            return AlignResult(maxima, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
                                float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
                                float(r.ms_walk), float(r.ms_total), int(r.score_launches))
         finally:

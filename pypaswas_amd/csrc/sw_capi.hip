@@ -37,6 +37,7 @@ struct pw_ctx {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // tile-kernel work that overlaps the lane-per-pair kernels
     hipEvent_t ev[10] = {};
     std::string err;
     bool scoring_set = false;
@@ -45,7 +46,9 @@ struct pw_ctx {
     float gap = 0, ext = 0, ll = 1, minimum = 0;
     bool affine = false;
     DevBuf res, seqs, items, trans, tables, scratch, scratch2, queue, maxima, ditems, dirs, ccount, cands, tbitems,
-        sps, spcount, keys, hits, ops, opsused;
+        sps, spcount, keys, hits, ops, opsused, gated;
+    // pinned host buffers for result maxima, recycled by pw_free_result
+    std::vector<std::pair<float *, size_t>> pinned_free;
 };
 
 static thread_local std::string g_init_err;
@@ -83,7 +86,8 @@ struct pw_plan {
     int64_t group_rows = 0;    // longest row sequence of a multi-chunk job (scratch sizing)
     int32_t max_nb = 16;
     int64_t nb_begin[NB_KINDS] = {}, nb_end[NB_KINDS] = {};  // job ranges per chunk width
-    DevBuf res, seqs, tables, items, trans, gitems, gents;
+    std::vector<std::pair<int64_t, int64_t>> batches;         // plain launch: item range of each batch
+    DevBuf res, seqs, tables, items, trans, gitems, gents, d_mss;
 };
 
 #define HIPCHK(ctx, call)                                                                   \
@@ -151,6 +155,7 @@ extern "C" int pw_init(int device, pw_ctx **out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
     if (ctx->num_cus <= 0) ctx->num_cus = 256;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (e != hipSuccess) {
         g_init_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
         delete ctx;
@@ -175,12 +180,14 @@ extern "C" int pw_destroy(pw_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->res, &ctx->seqs, &ctx->items, &ctx->trans, &ctx->tables, &ctx->scratch, &ctx->scratch2,
                       &ctx->queue,
                       &ctx->maxima, &ctx->ditems, &ctx->dirs, &ctx->ccount, &ctx->cands, &ctx->tbitems, &ctx->sps,
-                      &ctx->spcount, &ctx->keys, &ctx->hits, &ctx->ops, &ctx->opsused};
+                      &ctx->spcount, &ctx->keys, &ctx->hits, &ctx->ops, &ctx->opsused, &ctx->gated};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
+    for (auto &pb : ctx->pinned_free) (void)hipHostFree(pb.first);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return PW_OK;
 }
@@ -327,9 +334,9 @@ void choose_shape(int qlen, int tlen, int &trans, int &R) {
     }
 }
 
-// pairs sharing a target needed before they are scored lane-per-pair, and
-// the longest row sequence such a job takes
-constexpr int64_t GROUP_MIN_PAIRS = 16;
+// a 64-lane job is used when its lanes are busy for at least this fraction
+// of its rows; the longest row sequence such a job takes
+constexpr double GROUP_MIN_FILL = 0.6;
 constexpr int GROUP_MAX_ROWS = 1 << 15;
 
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
@@ -375,6 +382,7 @@ void free_plan(pw_plan *P) {
     free_buf(P->trans);
     free_buf(P->gitems);
     free_buf(P->gents);
+    free_buf(P->d_mss);
     delete P;
 }
 
@@ -410,11 +418,35 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
     std::vector<uint8_t> trans((size_t)n_pairs, 0);
     int64_t max_b = 0;
     if (mode == 0) {
-        // the reference's batch shape: every pair padded to the longest query
-        // and target rounded up to 8 (SmithWaterman.py:459-461, :325)
-        const int X = pad8((int)E.max_len_q), Y = pad8((int)E.max_len_t);
-        for (int64_t i = 0; i < n_pairs; ++i) items[i] = ScoreItem{pq[i], nq + pt[i], (int32_t)i, 4, X, Y};
-        max_b = Y;
+        // The reference's batching: consecutive pairs form a batch while the
+        // batch's full matrices fit in 80% of device memory (SmithWaterman.py:
+        // 202-221, 420-432: 1 direction byte + 4 (linear) or 12 (affine) bytes
+        // of FP32 matrix per cell), every pair of a batch padded to the
+        // batch's longest query and target rounded up to 8 (:459-461, :325).
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(ctx, hipMemGetInfo(&free_b, &total_b));
+        const double budget = 0.8 * (double)total_b;
+        const double bpc = ctx->affine ? 13.0 : 5.0;
+        int64_t begin = 0;
+        int bx = 0, by = 0;
+        auto close = [&](int64_t end) {
+            for (int64_t k = begin; k < end; ++k) items[k] = ScoreItem{pq[k], nq + pt[k], (int32_t)k, 4, bx, by};
+            P->batches.push_back({begin, end});
+            max_b = std::max<int64_t>(max_b, by);
+        };
+        for (int64_t i = 0; i < n_pairs; ++i) {
+            const int x = std::max(bx, pad8(qlen[pq[i]])), y = std::max(by, pad8(tlen[pt[i]]));
+            if (i > begin && (double)(i - begin + 1) * x * y * bpc > budget) {
+                close(i);
+                begin = i;
+                bx = pad8(qlen[pq[i]]);
+                by = pad8(tlen[pt[i]]);
+            } else {
+                bx = x;
+                by = y;
+            }
+        }
+        if (n_pairs > begin) close(n_pairs);
     } else {
         // lane-per-pair jobs for pairs whose target is shared by enough pairs;
         // the rest (long or unshared pairs) go to the wave-per-pair tile kernel
@@ -427,12 +459,23 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
         for (int32_t t = 0; t < nt; ++t) {
             std::vector<int64_t> &L = by_target[t];
             if (L.empty()) continue;
-            std::vector<int64_t> grp;
+            std::vector<int64_t> cand;
             for (int64_t i : L)
-                if ((int64_t)L.size() >= GROUP_MIN_PAIRS && qlen[pq[i]] <= GROUP_MAX_ROWS) grp.push_back(i);
+                if (qlen[pq[i]] <= GROUP_MAX_ROWS) cand.push_back(i);
                 else tile_pairs.push_back(i);
-            std::stable_sort(grp.begin(), grp.end(),
+            std::stable_sort(cand.begin(), cand.end(),
                              [&](int64_t x, int64_t y) { return qlen[pq[x]] > qlen[pq[y]]; });
+            // keep a 64-lane job only if its lanes are busy for most of its
+            // rows (sum of row lengths vs 64 x the longest); otherwise its
+            // pairs go to the tile kernel
+            std::vector<int64_t> grp;
+            for (size_t g0 = 0; g0 < cand.size(); g0 += 64) {
+                const size_t g1 = std::min(cand.size(), g0 + 64);
+                double sum = 0;
+                for (size_t u = g0; u < g1; ++u) sum += qlen[pq[cand[u]]];
+                const double fill = sum / (64.0 * std::max(1, qlen[pq[cand[g0]]]));
+                for (size_t u = g0; u < g1; ++u) (fill >= GROUP_MIN_FILL ? grp : tile_pairs).push_back(cand[u]);
+            }
             const int tl = tlen[t];
             const int nb = group_nb(tl);
             const int nch = tl > nb ? (tl + nb - 1) / nb : 1;
@@ -522,6 +565,10 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
                                ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(P->tables.p, E.tables.data(), E.tables.size() * sizeof(float),
                                hipMemcpyHostToDevice, ctx->stream));
+    ENSURE(ctx, P->d_mss, (size_t)n_pairs * sizeof(float));
+    if (n_pairs > 0)
+        HIPCHK(ctx, hipMemcpyAsync(P->d_mss.p, P->mss.data(), (size_t)n_pairs * sizeof(float),
+                                   hipMemcpyHostToDevice, ctx->stream));
     if (!items.empty()) {
         HIPCHK(ctx, hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(ScoreItem),
                                    hipMemcpyHostToDevice, ctx->stream));
@@ -554,19 +601,34 @@ int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work) {
 int run_score(pw_ctx *ctx, pw_plan *P, double *ms_score, int64_t *launches) {
     const int64_t n_pairs = P->n_pairs;
     ENSURE(ctx, ctx->maxima, (size_t)n_pairs * sizeof(float));
+    if (P->n_tile > 0) {  // sized before any launch, for the same reason as the group scratch
+        const size_t lds = lds_bytes(P->E.KP);
+        const int g = persistent_grid(ctx, (const void *)(ctx->affine ? score_kernel<true> : score_kernel<false>), lds,
+                                      P->n_tile);
+        ENSURE(ctx, ctx->scratch2, (size_t)g * 3 * (P->max_b + 64) * sizeof(float));
+    }
     // work counters on separate 64-byte lines: [0] tile jobs, [16*(1+w)] jobs of chunk width w
     ENSURE(ctx, ctx->queue, 64 * (1 + NB_KINDS));
     HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64 * (1 + NB_KINDS), ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (P->n_groups > 0) {
         const int64_t scr_stride = std::max<int64_t>(1, P->group_rows) * WAVE;
+        // size the scratch once for the largest grid: resizing frees memory a
+        // kernel launched just before may still be using
+        int grid_all = 1;
+        for (int w = 0; w < NB_KINDS; ++w) {
+            const int64_t n = P->nb_end[w] - P->nb_begin[w];
+            if (n <= 0) continue;
+            const size_t lds = (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float);
+            grid_all = std::max(grid_all, persistent_grid(ctx, group_kernel_for(ctx->affine, NB_WIDTHS[w]), lds, n));
+        }
+        ENSURE(ctx, ctx->scratch, (size_t)grid_all * scr_stride * sizeof(float2));
         for (int w = 0; w < NB_KINDS; ++w) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
             if (n <= 0) continue;
             const void *kern = group_kernel_for(ctx->affine, NB_WIDTHS[w]);
             const size_t lds = (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float);
             const int grid = persistent_grid(ctx, kern, lds, n);
-            ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float2));
             const GroupItem *items = (const GroupItem *)P->gitems.p + P->nb_begin[w];
             const uint8_t *res = (const uint8_t *)P->res.p;
             const SeqRef *seqs = (const SeqRef *)P->seqs.p;
@@ -586,19 +648,45 @@ int run_score(pw_ctx *ctx, pw_plan *P, double *ms_score, int64_t *launches) {
     if (P->n_tile > 0) {
         auto kern = ctx->affine ? score_kernel<true> : score_kernel<false>;
         const size_t lds = lds_bytes(P->E.KP);
-        const int grid = persistent_grid(ctx, (const void *)kern, lds, P->n_tile);
         const int64_t scr_stride = 3 * (P->max_b + 64);
-        ENSURE(ctx, ctx->scratch2, (size_t)grid * scr_stride * sizeof(float));
-        unsigned *queue = P->mode == 0 ? nullptr : (unsigned *)ctx->queue.p;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
-                           (const SeqRef *)P->seqs.p, (const ScoreItem *)P->items.p, (int)P->n_tile,
-                           (const uint8_t *)P->trans.p, (const float *)P->tables.p, P->E.KP, P->E.lkp, ctx->gap,
-                           ctx->ext, (float *)ctx->scratch2.p, scr_stride, queue, (float *)ctx->maxima.p);
-        HIPCHK(ctx, hipGetLastError());
-        if (launches) *launches += 1;
+        const int grid_max = persistent_grid(ctx, (const void *)kern, lds, P->n_tile);
+        const ScoreItem *items = (const ScoreItem *)P->items.p;
+        const uint8_t *trans = (const uint8_t *)P->trans.p;
+        if (P->mode == 0) {
+            // plain per-batch launches: one launch per batch, static schedule
+            for (const auto &b : P->batches) {
+                const int64_t n = b.second - b.first;
+                const int grid = (int)std::min<int64_t>(n, grid_max);
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
+                                   (const SeqRef *)P->seqs.p, items + b.first, (int)n, trans + b.first,
+                                   (const float *)P->tables.p, P->E.KP, P->E.lkp, ctx->gap, ctx->ext,
+                                   (float *)ctx->scratch2.p, scr_stride, (unsigned *)nullptr,
+                                   (float *)ctx->maxima.p);
+                HIPCHK(ctx, hipGetLastError());
+                if (launches) *launches += 1;
+            }
+        } else {
+            // persistent workers on the second stream, overlapping the lane-per-pair kernels
+            hipStream_t st = P->n_groups > 0 ? ctx->stream2 : ctx->stream;
+            if (st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(st, ctx->ev[0], 0));
+            hipLaunchKernelGGL(kern, dim3(grid_max), dim3(64), lds, st, (const uint8_t *)P->res.p,
+                               (const SeqRef *)P->seqs.p, items, (int)P->n_tile, trans, (const float *)P->tables.p,
+                               P->E.KP, P->E.lkp, ctx->gap, ctx->ext, (float *)ctx->scratch2.p, scr_stride,
+                               (unsigned *)ctx->queue.p, (float *)ctx->maxima.p);
+            HIPCHK(ctx, hipGetLastError());
+            if (launches) *launches += 1;
+            if (st != ctx->stream) {
+                HIPCHK(ctx, hipEventRecord(ctx->ev[8], st));
+                HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev[8], 0));
+            }
+        }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    (void)ms_score;  // read from events ev[0]..ev[1] once the stream has been synchronised
+    return PW_OK;
+}
+
+int score_ms(pw_ctx *ctx, double *ms_score) {
     float ms = 0.0f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     if (ms_score) *ms_score += ms;
@@ -607,8 +695,8 @@ int run_score(pw_ctx *ctx, pw_plan *P, double *ms_score, int64_t *launches) {
 
 struct HostResult {
     pw_result pub;
-    std::vector<float> maxima;
-    std::vector<int32_t> pq, pt;
+    float *maxima = nullptr;  // pinned host memory of maxima_cap floats
+    size_t maxima_cap = 0;
     std::vector<pw_starting_point> sps;
     std::vector<pw_hit> hits;
     std::vector<char> ops;
@@ -625,19 +713,45 @@ int execute(pw_ctx *ctx, pw_plan *P, double minimum, int32_t sp_cap, HostResult 
     int64_t launches = 0;
     int rc = run_score(ctx, P, &ms_score, &launches);
     if (rc) return rc;
-    R->maxima.resize((size_t)n_pairs);
-    R->pq = P->pq;
-    R->pt = P->pt;
-    if (n_pairs > 0)
-        HIPCHK(ctx, hipMemcpy(R->maxima.data(), ctx->maxima.p, (size_t)n_pairs * sizeof(float),
+    // traceback gate on the device: maximum >= MINIMUM_SCORE (default_main_cpu.cl:410)
+    // and >= the pair's minimum start score, without which no cell can start (:427)
+    ENSURE(ctx, ctx->gated, (size_t)(n_pairs + 16) * sizeof(int));
+    HIPCHK(ctx, hipMemsetAsync(ctx->gated.p, 0, sizeof(int), ctx->stream));
+    if (n_pairs > 0) {
+        const int grid = (int)std::min<int64_t>((n_pairs + 255) / 256, (int64_t)ctx->num_cus * 16);
+        hipLaunchKernelGGL(gate_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const float *)ctx->maxima.p,
+                           (const float *)P->d_mss.p, n_pairs, ctx->minimum, (int *)ctx->gated.p,
+                           (int *)ctx->gated.p + 16);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    if (n_pairs > 0) {
+        // reuse a pinned buffer from an earlier, freed result when one is large enough
+        for (size_t i = 0; i < ctx->pinned_free.size(); ++i)
+            if (ctx->pinned_free[i].second >= (size_t)n_pairs) {
+                R->maxima = ctx->pinned_free[i].first;
+                R->maxima_cap = ctx->pinned_free[i].second;
+                ctx->pinned_free.erase(ctx->pinned_free.begin() + i);
+                break;
+            }
+        if (!R->maxima) {
+            HIPCHK(ctx, hipHostMalloc((void **)&R->maxima, (size_t)n_pairs * sizeof(float), hipHostMallocDefault));
+            R->maxima_cap = (size_t)n_pairs;
+        }
+        HIPCHK(ctx, hipMemcpyAsync(R->maxima, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    }
+    int ngated = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&ngated, ctx->gated.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    rc = score_ms(ctx, &ms_score);
+    if (rc) return rc;
+    std::vector<int64_t> gated((size_t)ngated);
+    if (ngated > 0) {
+        std::vector<int> g32((size_t)ngated);
+        HIPCHK(ctx, hipMemcpy(g32.data(), (int *)ctx->gated.p + 16, (size_t)ngated * sizeof(int),
                               hipMemcpyDeviceToHost));
-
-    // traceback gate: maximum >= MINIMUM_SCORE (default_main_cpu.cl:410); a
-    // start also needs a value >= the pair's minimum start score (:427)
-    std::vector<int64_t> gated;
-    for (int64_t i = 0; i < n_pairs; ++i) {
-        const float m = R->maxima[i];
-        if (m >= ctx->minimum && m >= min_start_score[i]) gated.push_back(i);
+        std::sort(g32.begin(), g32.end());
+        for (int i = 0; i < ngated; ++i) gated[i] = g32[i];
     }
     const int ng = (int)gated.size();
     double ms_dir = 0, ms_tb = 0, ms_walk = 0;
@@ -850,9 +964,7 @@ int execute(pw_ctx *ctx, pw_plan *P, double minimum, int32_t sp_cap, HostResult 
     }
     pw_result &O = R->pub;
     O.n_pairs = n_pairs;
-    O.maxima = R->maxima.data();
-    O.pair_query = R->pq.data();
-    O.pair_target = R->pt.data();
+    O.maxima = R->maxima;
     O.n_starting_points = (int64_t)R->sps.size();
     O.starting_points = R->sps.data();
     O.n_hits = (int64_t)R->hits.size();
@@ -917,7 +1029,7 @@ extern "C" int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp
     std::memset(&R->pub, 0, sizeof(R->pub));
     rc = execute(ctx, plan, minimum, sp_cap, R);
     if (rc) {
-        delete R;
+        pw_free_result(ctx, &R->pub);
         return rc;
     }
     *out = &R->pub;
@@ -946,6 +1058,14 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     rc = prepare(ctx, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, nullptr, nullptr, mode);
     double ms = 0;
     if (!rc) rc = run_score(ctx, P, &ms, nullptr);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+            rc = PW_EDEVICE;
+        }
+    }
+    if (!rc) rc = score_ms(ctx, &ms);
     if (!rc && n_pairs > 0) {
         hipError_t e = hipMemcpy(maxima_out, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
@@ -978,9 +1098,14 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
 }
 
 extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res) {
-    (void)ctx;
     if (!res) return PW_EINVAL;
     // pub is the first member of HostResult
-    delete reinterpret_cast<HostResult *>(res);
+    HostResult *R = reinterpret_cast<HostResult *>(res);
+    if (R->maxima) {
+        if (ctx && ctx->pinned_free.size() < 4) ctx->pinned_free.push_back({R->maxima, R->maxima_cap});
+        else (void)hipHostFree(R->maxima);
+        R->maxima = nullptr;
+    }
+    delete R;
     return PW_OK;
 }

@@ -500,6 +500,28 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
     }
 }
 
+// Traceback gate: pairs whose maximum reaches MINIMUM_SCORE and their minimum
+// start score get the direction pass.  One atomic per wave.
+__global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
+                                                   int64_t n, float minimum, int *__restrict__ count,
+                                                   int *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        bool pass = false;
+        if (i < n) {
+            const float m = maxima[i];
+            pass = m >= minimum && m >= mss[i];
+        }
+        const uint64_t mask = __ballot(pass);
+        if (mask == 0) continue;
+        int first = 0;
+        if (lane == 0) first = atomicAdd(count, __popcll(mask));
+        first = __shfl(first, 0, 64);
+        if (pass) out[first + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Start selection, marking and the walk: one thread per pair.
 // ---------------------------------------------------------------------------

@@ -82,12 +82,28 @@ def test_repeats_and_ties(dev, oracle):
 
 
 GROUP_CASES = [
-    # enough queries per target to be scored lane-per-pair; targets above 96
-    # residues are processed in chunks of 64 columns
-    ("group-dna-linear", DNA, 70, 4, (30, 400), (12, 300), False),
-    ("group-dna-affine", DNA, 70, 4, (30, 400), (12, 300), True),
-    ("group-dna-affine-short", DNA, 130, 3, (50, 260), (20, 80), True),
+    # queries of similar length and enough of them per target to be scored
+    # lane-per-pair; targets above 96 residues are processed in chunks of 64
+    ("group-dna-linear", DNA, 70, 4, (240, 300), (12, 300), False),
+    ("group-dna-affine", DNA, 70, 4, (240, 300), (12, 300), True),
+    ("group-dna-affine-short", DNA, 130, 3, (200, 260), (20, 80), True),
 ]
+
+
+def overwrite_set(rng, nq, nt, ql, tl, alphabet, plant=0.6, sub=0.05):
+    """Queries of length in ql; some carry a mutated target piece written over a window."""
+    from tests._parity import mutate, rand_seq
+    targets = [rand_seq(rng, int(rng.integers(tl[0], tl[1] + 1)), alphabet) for _ in range(nt)]
+    queries = []
+    for _ in range(nq):
+        q = list(rand_seq(rng, int(rng.integers(ql[0], ql[1] + 1)), alphabet))
+        if rng.random() < plant:
+            t = targets[int(rng.integers(0, nt))]
+            piece = mutate(rng, t, sub, 0.0, alphabet)[: len(q)]
+            off = int(rng.integers(0, len(q) - len(piece) + 1))
+            q[off:off + len(piece)] = list(piece)
+        queries.append("".join(q))
+    return queries, targets
 
 
 @pytest.mark.parametrize("case", GROUP_CASES, ids=[c[0] for c in GROUP_CASES])
@@ -95,7 +111,7 @@ def test_group_kernel_parity(dev, oracle, case):
     name, alpha, nq, nt, ql, tl, affine = case
     print("case", name, flush=True)
     rng = np.random.default_rng(zlib.crc32(name.encode()))
-    qs, ts = planted_set(rng, nq, nt, ql, tl, alpha, plant=0.6)
+    qs, ts = overwrite_set(rng, nq, nt, ql, tl, alpha)
     p = dna_params(oracle, affine, minimum=8.0)
     res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
     compare(oracle, qs, ts, p, res)
@@ -104,9 +120,10 @@ def test_group_kernel_parity(dev, oracle, case):
 
 def test_group_kernel_protein(dev, oracle):
     rng = np.random.default_rng(99)
-    qs, ts = planted_set(rng, 40, 3, (30, 500), (50, 700), AMINO, plant=0.8, sub=0.5, indel=0.05)
+    qs, ts = overwrite_set(rng, 70, 3, (200, 260), (50, 700), AMINO, plant=0.8, sub=0.4)
     p = blosum_params(oracle, minimum=25.0)
     res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
     compare(oracle, qs, ts, p, res)
     res0 = hip_all_vs_all(dev, oracle, qs, ts, p, mode=0)
     assert np.array_equal(res0.maxima, res.maxima)
+    assert np.array_equal(res0.hits, res.hits)

@@ -304,7 +304,7 @@ def _fake_result(O, br, queries, targets, qids, tids, minimum):
             hits.append((w.qi, w.ti, int(w.shown), w.s_start, w.s_end, w.t_start, w.t_end, w.matches,
                          w.mismatches, len(codes), w.score, 0, off))
     N, T = br.maxima.shape
-    return AlignResult(br.maxima.reshape(-1), np.repeat(np.arange(N), T), np.tile(np.arange(T), N),
+    return AlignResult(br.maxima.reshape(-1),
                        np.zeros(0, SP_DTYPE), np.array(hits, dtype=HIT_DTYPE), bytes(ops), 0, 0, 0.0, 0, 0, 0, 0, 0,
                        0)
 
