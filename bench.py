@@ -28,7 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS batched-persistent vs plain launch, bit-exact hits; p50/p99 latency"
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, 32-bit VALU ops
+# Published FP32 vector peak of MI355X, 157.3 TFLOP/s with an FMA counted as 2: 78.6 T lane-ops/s
+# (256 CUs x 128 FP32 lanes/clk x 2.4 GHz).  The score pass issues adds and maxima, one op each.
+VALU_PEAK_TOPS = 157.3 / 2
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "score_config2_traffic.txt")
 OPS_PER_CELL = {"linear": 7, "affine": 17}
 
 
@@ -124,9 +127,14 @@ def main():
 
     avg_score_s = float(np.mean(score_ms)) / 1000.0
     achieved = OPS_PER_CELL[kind] * cells / avg_score_s / 1e12
+    traffic = None
+    if args.workload == "config2" and os.path.exists(TRAFFIC_FILE):
+        # HBM bytes per score pass from the committed FETCH_SIZE / WRITE_SIZE runs (see the file for the command)
+        traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_pass")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                "kernel": "score_kernel<%s>" % ("true" if kind == "affine" else "false"),
+                "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic,
+                "kernel": "score pass (group_kernel<%s,NB> + score_kernel<%s>)" % (
+                    ("true",) * 2 if kind == "affine" else ("false",) * 2),
                 "ops_per_cell": OPS_PER_CELL[kind], "cells_per_launch": cells,
                 "kernel_ms": round(float(np.mean(score_ms)), 3)}
 
