@@ -116,6 +116,33 @@ int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int3
 int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out);
 int pw_release_plan(pw_ctx *ctx, pw_plan *plan);
 
+/* pw_execute with the device work of the execution (scoring, gate, direction
+ * pass, start selection, walk and the control-block read-back) captured once
+ * in a HIP graph and replayed on later calls; the graph is captured again
+ * when a work area was resized or `minimum` changed.  Results are identical
+ * to pw_execute.  Replaces the per-block launch loops of
+ * SmithWaterman._calculate_score / _traceback_host
+ * (pyPaSWAS/Core/SmithWaterman.py:592-653) for a job run repeatedly.  In the
+ * result, ms_score holds the device time of the whole graph and the other
+ * stage times only cover tail rounds. */
+int pw_execute_graph(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out);
+
+/* Sizing of the device work areas of the traceback stages, the role of the
+ * memory model of SmithWaterman._get_max_number_sequences and
+ * maximum_memory_usage (pyPaSWAS/Core/SmithWaterman.py:202-221).  Values <= 0
+ * keep the current setting.
+ *   max_direction_bytes  upper bound of the direction area (default: a quarter
+ *                        of device memory; the first area is at most 256 MiB)
+ *   candidates_per_pair  candidate-start slots a gated pair gets (default 32)
+ *   ops_bytes            first size of the op-character area (default 16 MiB)
+ *   grow                 1: after an execution whose pairs did not fit, the
+ *                        areas of the next one are sized to what it needed
+ * Pairs that do not fit finish in host-driven tail rounds with exact sizes,
+ * so the limits change speed and memory, never results.  Applies to plans
+ * prepared afterwards and to the one-shot calls. */
+int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t candidates_per_pair, int64_t ops_bytes,
+                  int32_t grow);
+
 /* Score pass only: pair maxima, the quantity the traceback gate of
  * default_main_cpu.cl:408-410 reads.  maxima_out is [n_pairs]. */
 int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,

@@ -25,7 +25,8 @@ PW_EDEVICE = -3
 
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
-           "pw_execute", "pw_release_plan", "pw_score_pairs", "pw_free_result", "pw_last_error")
+           "pw_execute", "pw_release_plan", "pw_execute_graph", "pw_set_limits", "pw_score_pairs", "pw_free_result",
+           "pw_last_error")
 
 
 class StartingPointRec(ctypes.Structure):
@@ -85,6 +86,9 @@ def load() -> ctypes.CDLL:
                                  vp, vp, ctypes.c_int32, ctypes.POINTER(vp)]
         L.pw_execute.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(ResultRec))]
         L.pw_release_plan.argtypes = [vp, vp]
+        L.pw_execute_graph.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.POINTER(ResultRec))]
+        L.pw_set_limits.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
@@ -241,11 +245,19 @@ class Device:
         self._check(rc, "pw_prepare")
         return Plan(self, handle)
 
-    def execute(self, plan: "Plan", minimum: float, sp_cap: int = 100) -> AlignResult:
+    def execute(self, plan: "Plan", minimum: float, sp_cap: int = 100, graph: bool = False) -> AlignResult:
+        """Runs a prepared plan; graph=True replays a captured HIP graph of its device work."""
         out = ctypes.POINTER(ResultRec)()
-        self._check(self.lib.pw_execute(self.ctx, plan.handle, float(minimum), int(sp_cap), ctypes.byref(out)),
-                    "pw_execute")
+        fn = self.lib.pw_execute_graph if graph else self.lib.pw_execute
+        self._check(fn(self.ctx, plan.handle, float(minimum), int(sp_cap), ctypes.byref(out)),
+                    "pw_execute_graph" if graph else "pw_execute")
         return self._take_result(out)
+
+    def set_limits(self, max_direction_bytes: int = 0, candidates_per_pair: int = 0, ops_bytes: int = 0,
+                   grow: bool = True):
+        """Work-area sizing of the traceback stages (pw_set_limits); 0 keeps a setting."""
+        self._check(self.lib.pw_set_limits(self.ctx, int(max_direction_bytes), int(candidates_per_pair),
+                                           int(ops_bytes), 1 if grow else 0), "pw_set_limits")
 
     def score_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                     mode: int = 1):

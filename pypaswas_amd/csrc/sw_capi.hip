@@ -1,13 +1,21 @@
 // C ABI of the gfx950 Smith-Waterman library (declared in
-// include/pypaswas_hip.h).  Host orchestration of one call:
-//   1. encode residues into a compact alphabet, upload sequences and the two
-//      transposed substitution tables;
-//   2. scoring pass over all pairs (plain: padded shapes, static schedule;
-//      batched: exact shapes, per-pair tile shape, persistent workers pulling
-//      from a device counter, longest pairs first);
-//   3. the traceback gate of default_main_cpu.cl:408-410 on the pair maxima;
-//   4. direction pass over gated pairs, start selection + marking, walk;
-//   5. gather results into one host record.
+// include/pypaswas_hip.h).  One execution of a prepared plan is a single
+// sequence of device work on one stream pair, with no host round trip in the
+// middle:
+//   1. clear the control block (work counters, bump allocators);
+//   2. scoring pass over all pairs (plain: padded shapes, static schedule,
+//      one launch per batch; batched: exact shapes, lane-per-pair jobs and
+//      wave-per-pair tiles pulled by persistent workers from device counters);
+//   3. the traceback gate of default_main_cpu.cl:408-410, which writes one
+//      task per passing pair and takes its direction and candidate areas from
+//      device bump allocators;
+//   4. direction pass, start selection + path marking, walk, each over the
+//      device-side task count;
+//   5. read back the control block and the pair maxima, then the tasks and
+//      the compacted starting points, hits and op characters.
+// Tasks that did not fit the work areas are finished by host-driven tail
+// rounds with exact sizes.  Because nothing in 1-4 depends on the host, the
+// sequence can be captured once in a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,22 +39,58 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Scoring {
+    std::vector<float> subst;
+    int dim = 0, char_offset = 0;
+    float gap = 0, ext = 0, ll = 1, minimum = 0;
+    bool affine = false;
+};
+
+// Two streams and the events of one execution.  ev: 0 score start, 1 score
+// end, 2 gate end, 3 directions end, 4 traceback end, 5 walk end; fork/join
+// order the tile-kernel stream against the main one.
+struct Lane {
+    hipStream_t s = nullptr, s2 = nullptr;
+    hipEvent_t ev[6] = {};
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+// Work-area sizing rules (pw_set_limits).
+struct Limits {
+    int64_t dir_init = (int64_t)256 << 20;  // first direction area, bytes
+    int64_t dir_max = 0;                    // largest direction area the pools grow to (0: a quarter of HBM)
+    int32_t cand0 = 32;                     // candidate slots per gated pair
+    int64_t ops_init = (int64_t)16 << 20;   // first ops area, bytes
+    bool grow = true;                       // resize the areas after an execution that overflowed them
+};
+
+// Device work areas of one execution, plus the pinned host staging of the
+// read-back.  want_* are the sizes the next execution allocates.
+struct Workspace {
+    DevBuf ctl, maxima, tasks, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch, dscratch;
+    int64_t want_dir = 0, want_cand = 0, want_ops = 0;
+    int32_t cand0 = 0;
+    int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
+    PinBuf h_ctl, h_tasks, h_sps, h_hits, h_ops;
+};
+
 }  // namespace
 
 struct pw_ctx {
     int device = 0;
     int num_cus = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;  // tile-kernel work that overlaps the lane-per-pair kernels
-    hipEvent_t ev[10] = {};
+    size_t total_mem = 0;
+    Lane lane;
     std::string err;
     bool scoring_set = false;
-    std::vector<float> subst;
-    int dim = 0, char_offset = 0;
-    float gap = 0, ext = 0, ll = 1, minimum = 0;
-    bool affine = false;
-    DevBuf res, seqs, items, trans, tables, scratch, scratch2, queue, maxima, ditems, dirs, ccount, cands, tbitems,
-        sps, spcount, keys, hits, ops, opsused, gated;
+    Scoring sc;
+    Limits lim;
+    Workspace ws;  // shared by the one-shot calls (pw_align_pairs, pw_score_pairs)
     // pinned host buffers for result maxima, recycled by pw_free_result
     std::vector<std::pair<float *, size_t>> pinned_free;
 };
@@ -63,16 +107,30 @@ struct Encoded {
     int64_t max_len_q = 0, max_len_t = 0;
 };
 
-}  // namespace
-
 // chunk widths of the lane-per-pair kernel
-constexpr int NB_KINDS = 6;
 constexpr int NB_WIDTHS[NB_KINDS] = {16, 32, 48, 64, 80, 96};
 
-// A prepared job: sequences, substitution tables and the scoring work list
-// resident on the device, plus the host copies the later stages need.
+// Launch geometry of one execution, computed before any launch (and before
+// a graph capture, during which no occupancy query is made).
+struct Geometry {
+    int group_grid[NB_KINDS] = {};
+    int64_t gscr_stride = 0;  // float2 per workgroup
+    int tile_grid = 0;
+    int64_t tscr_stride = 0;  // floats per workgroup
+    int gate_grid = 0;
+    int dir_grid = 0;
+    int64_t dscr_stride = 0;
+    int task_grid = 0;  // workgroups of 64 threads of the start-selection and walk kernels
+};
+
+}  // namespace
+
+// A prepared job: sequences, substitution tables, the scoring work list and
+// the pair list resident on the device, plus the host copies the read-back
+// needs and (once executed through a graph) the instantiated graph.
 struct pw_plan {
     Encoded E;
+    Scoring sc;
     int32_t nq = 0, nt = 0;
     int64_t n_pairs = 0;
     int32_t mode = 1;
@@ -80,6 +138,9 @@ struct pw_plan {
     std::vector<float> mss;
     std::vector<uint8_t> same;
     int64_t max_b = 0;
+    int64_t max_t = 0;          // longest target of any pair
+    int64_t dir_worst = 0;      // direction bytes if every pair passed the gate
+    int64_t dir_largest = 0;    // largest single pair's direction area
     double cells = 0;
     int64_t n_tile = 0;        // pairs scored by the wave-per-pair tile kernel
     int64_t n_groups = 0;      // lane-per-pair jobs
@@ -87,7 +148,15 @@ struct pw_plan {
     int32_t max_nb = 16;
     int64_t nb_begin[NB_KINDS] = {}, nb_end[NB_KINDS] = {};  // job ranges per chunk width
     std::vector<std::pair<int64_t, int64_t>> batches;         // plain launch: item range of each batch
-    DevBuf res, seqs, tables, items, trans, gitems, gents, d_mss;
+    std::vector<ScoreItem> items;
+    std::vector<uint8_t> trans;
+    std::vector<GroupItem> gitems;
+    std::vector<GroupEntry> gents;
+    DevBuf res, seqs, tables, d_items, d_trans, d_gitems, d_gents, d_mss, d_pairs, d_same;
+    Workspace own;
+    // graph of the execution and the launch parameters it was captured with
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_sig;
 };
 
 #define HIPCHK(ctx, call)                                                                   \
@@ -122,11 +191,86 @@ static int ensure(pw_ctx *ctx, DevBuf &b, size_t bytes) {
     return PW_OK;
 }
 
+static int ensure_pinned(pw_ctx *ctx, PinBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 64;
+    if (b.cap >= bytes) return PW_OK;
+    if (b.p) {
+        (void)hipHostFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipHostMalloc(&b.p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        ctx->err = "hipHostMalloc of " + std::to_string(want) + " bytes: " + hipGetErrorString(e);
+        return PW_ENOMEM;
+    }
+    b.cap = want;
+    return PW_OK;
+}
+
 #define ENSURE(ctx, buf, bytes)                      \
     do {                                             \
         int rc_ = ensure((ctx), (buf), (bytes));     \
         if (rc_ != PW_OK) return rc_;                \
     } while (0)
+
+#define ENSURE_PINNED(ctx, buf, bytes)                  \
+    do {                                                \
+        int rc_ = ensure_pinned((ctx), (buf), (bytes)); \
+        if (rc_ != PW_OK) return rc_;                   \
+    } while (0)
+
+namespace {
+
+void free_buf(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+void free_pinned(PinBuf &b) {
+    if (b.p) (void)hipHostFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+void free_workspace(Workspace &W) {
+    DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
+                      &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch};
+    for (DevBuf *b : bufs) free_buf(*b);
+    PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_sps, &W.h_hits, &W.h_ops};
+    for (PinBuf *b : pins) free_pinned(*b);
+}
+
+int create_lane(Lane &L, std::string &err) {
+    hipError_t e = hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.s2, hipStreamNonBlocking);
+    for (auto &ev : L.ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        err = std::string("stream/event creation: ") + hipGetErrorString(e);
+        return PW_EDEVICE;
+    }
+    return PW_OK;
+}
+
+void destroy_lane(Lane &L) {
+    if (L.s) (void)hipStreamSynchronize(L.s);
+    if (L.s2) (void)hipStreamSynchronize(L.s2);
+    for (auto &ev : L.ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (L.fork) (void)hipEventDestroy(L.fork);
+    if (L.join) (void)hipEventDestroy(L.join);
+    if (L.s) (void)hipStreamDestroy(L.s);
+    if (L.s2) (void)hipStreamDestroy(L.s2);
+    L = Lane();
+}
+
+}  // namespace
 
 extern "C" int pw_init(int device, pw_ctx **out) {
     if (!out) {
@@ -152,22 +296,19 @@ extern "C" int pw_init(int device, pw_ctx **out) {
     pw_ctx *ctx = new pw_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->num_cus = prop.multiProcessorCount;
+        ctx->total_mem = prop.totalGlobalMem;
+    }
     if (ctx->num_cus <= 0) ctx->num_cus = 256;
-    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        g_init_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    if (ctx->total_mem == 0) {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) == hipSuccess) ctx->total_mem = t;
+    }
+    if (create_lane(ctx->lane, g_init_err) != PW_OK) {
+        destroy_lane(ctx->lane);
         delete ctx;
         return PW_EDEVICE;
-    }
-    for (auto &ev : ctx->ev) {
-        e = hipEventCreate(&ev);
-        if (e != hipSuccess) {
-            g_init_err = std::string("hipEventCreate: ") + hipGetErrorString(e);
-            delete ctx;
-            return PW_EDEVICE;
-        }
     }
     *out = ctx;
     return PW_OK;
@@ -176,18 +317,9 @@ extern "C" int pw_init(int device, pw_ctx **out) {
 extern "C" int pw_destroy(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->res, &ctx->seqs, &ctx->items, &ctx->trans, &ctx->tables, &ctx->scratch, &ctx->scratch2,
-                      &ctx->queue,
-                      &ctx->maxima, &ctx->ditems, &ctx->dirs, &ctx->ccount, &ctx->cands, &ctx->tbitems, &ctx->sps,
-                      &ctx->spcount, &ctx->keys, &ctx->hits, &ctx->ops, &ctx->opsused, &ctx->gated};
-    for (DevBuf *b : bufs)
-        if (b->p) (void)hipFree(b->p);
+    destroy_lane(ctx->lane);
+    free_workspace(ctx->ws);
     for (auto &pb : ctx->pinned_free) (void)hipHostFree(pb.first);
-    for (auto &ev : ctx->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return PW_OK;
 }
@@ -203,62 +335,95 @@ extern "C" int pw_device_mem_bytes(pw_ctx *ctx, uint64_t *bytes) {
     return PW_OK;
 }
 
-extern "C" int pw_set_scoring(pw_ctx *ctx, const float *subst, int dim, int char_offset, float gap, float gap_ext,
-                              float lower_limit, float minimum_score) {
-    if (!ctx) return PW_EINVAL;
+namespace {
+
+int make_scoring(std::string &err, const float *subst, int dim, int char_offset, float gap, float gap_ext,
+                 float lower_limit, float minimum_score, Scoring &sc) {
     if (!subst || dim <= 0 || dim > 255) {
-        ctx->err = "pw_set_scoring: substitution matrix must be dim x dim with 0 < dim <= 255";
+        err = "scoring: substitution matrix must be dim x dim with 0 < dim <= 255";
         return PW_EINVAL;
     }
     if (!(gap <= 0.0f) || !(gap_ext <= 0.0f)) {
-        ctx->err = "pw_set_scoring: gap and gap extension scores must be <= 0";
+        err = "scoring: gap and gap extension scores must be <= 0";
         return PW_EINVAL;
     }
     if (!(gap > GAP_INIT) || !std::isfinite(lower_limit) || !std::isfinite(minimum_score)) {
-        ctx->err = "pw_set_scoring: scoring constants out of range";
+        err = "scoring: scoring constants out of range";
         return PW_EINVAL;
     }
     for (int i = 0; i < dim * dim; ++i)
         if (!std::isfinite(subst[i]) || std::fabs(subst[i]) >= 1e9f) {
-            ctx->err = "pw_set_scoring: substitution scores must be finite and below 1e9 in magnitude";
+            err = "scoring: substitution scores must be finite and below 1e9 in magnitude";
             return PW_EINVAL;
         }
-    ctx->subst.assign(subst, subst + (size_t)dim * dim);
-    ctx->dim = dim;
-    ctx->char_offset = char_offset;
-    ctx->gap = gap;
-    ctx->ext = gap_ext;
-    ctx->affine = gap_ext != 0.0f;
-    ctx->ll = lower_limit;
-    ctx->minimum = minimum_score;
+    sc.subst.assign(subst, subst + (size_t)dim * dim);
+    sc.dim = dim;
+    sc.char_offset = char_offset;
+    sc.gap = gap;
+    sc.ext = gap_ext;
+    sc.affine = gap_ext != 0.0f;
+    sc.ll = lower_limit;
+    sc.minimum = minimum_score;
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" int pw_set_scoring(pw_ctx *ctx, const float *subst, int dim, int char_offset, float gap, float gap_ext,
+                              float lower_limit, float minimum_score) {
+    if (!ctx) return PW_EINVAL;
+    Scoring sc;
+    int rc = make_scoring(ctx->err, subst, dim, char_offset, gap, gap_ext, lower_limit, minimum_score, sc);
+    if (rc) return rc;
+    ctx->sc = std::move(sc);
     ctx->scoring_set = true;
+    return PW_OK;
+}
+
+extern "C" int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t candidates_per_pair,
+                             int64_t ops_bytes, int32_t grow) {
+    if (!ctx) return PW_EINVAL;
+    if (max_direction_bytes < 0 || candidates_per_pair < 0 || ops_bytes < 0) {
+        ctx->err = "pw_set_limits: negative size";
+        return PW_EINVAL;
+    }
+    if (max_direction_bytes > 0) {
+        ctx->lim.dir_max = max_direction_bytes;
+        ctx->lim.dir_init = std::min<int64_t>(ctx->lim.dir_init, max_direction_bytes);
+    }
+    if (candidates_per_pair > 0) ctx->lim.cand0 = std::min<int32_t>(candidates_per_pair, 1 << 16);
+    if (ops_bytes > 0) ctx->lim.ops_init = ops_bytes;
+    ctx->lim.grow = grow != 0;
+    // the shared work areas follow the new rules from the next call on
+    ctx->ws.want_dir = ctx->ws.want_cand = ctx->ws.want_ops = 0;
+    ctx->ws.cand0 = 0;
     return PW_OK;
 }
 
 namespace {
 
-int encode(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
-           const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, Encoded &E) {
+int encode(std::string &err, const Scoring &sc, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen,
+           int32_t nq, const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, Encoded &E) {
     int map[256];
     for (int &m : map) m = -1;
     std::vector<int> used;
     auto scan = [&](const uint8_t *res, const int64_t *off, const int32_t *len, int32_t n) -> int {
         for (int32_t i = 0; i < n; ++i) {
             if (len[i] < 0 || off[i] < 0) {
-                ctx->err = "negative sequence length or offset";
+                err = "negative sequence length or offset";
                 return PW_EINVAL;
             }
             const uint8_t *p = res + off[i];
             for (int32_t k = 0; k < len[i]; ++k) {
                 const int c = p[k];
                 if (map[c] < 0) {
-                    const int idx = c - ctx->char_offset;
-                    if (idx < 0 || idx >= ctx->dim) {
+                    const int idx = c - sc.char_offset;
+                    if (idx < 0 || idx >= sc.dim) {
                         char buf[160];
                         snprintf(buf, sizeof buf,
                                  "character %d ('%c') is outside the scoring alphabet (offset %d, dimension %d)", c,
-                                 (c >= 32 && c < 127) ? c : '?', ctx->char_offset, ctx->dim);
-                        ctx->err = buf;
+                                 (c >= 32 && c < 127) ? c : '?', sc.char_offset, sc.dim);
+                        err = buf;
                         return PW_EINVAL;
                     }
                     map[c] = (int)used.size();
@@ -274,24 +439,23 @@ int encode(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t 
     if (rc) return rc;
     E.K = (int)used.size();
     if (E.K > KPMAX - 1) {
-        ctx->err = "more than 63 distinct residues in one call";
+        err = "more than 63 distinct residues in one call";
         return PW_EINVAL;
     }
     E.KP = 8;
     while (E.KP < E.K + 1) E.KP *= 2;
     E.lkp = 2;
     for (int v = E.KP; v > 1; v >>= 1) ++E.lkp;  // log2(KP) + 2: byte shift of a table row
-    const int KP = E.KP, FILL = KP - 1;
+    const int KP = E.KP;
     E.tables.assign(2 * KP * KP, FILL_SCORE);
     for (int a = 0; a < E.K; ++a)
         for (int b = 0; b < E.K; ++b) {
-            const int ia = used[a] - ctx->char_offset, ib = used[b] - ctx->char_offset;
+            const int ia = used[a] - sc.char_offset, ib = used[b] - sc.char_offset;
             // table 0: A = query  -> entry [b][a] = S(query a, target b)
-            E.tables[b * KP + a] = ctx->subst[(size_t)ia * ctx->dim + ib];
+            E.tables[b * KP + a] = sc.subst[(size_t)ia * sc.dim + ib];
             // table 1: A = target -> entry [b][a] = S(query b, target a)
-            E.tables[KP * KP + b * KP + a] = ctx->subst[(size_t)ib * ctx->dim + ia];
+            E.tables[KP * KP + b * KP + a] = sc.subst[(size_t)ib * sc.dim + ia];
         }
-    (void)FILL;
     int64_t total = 0;
     for (int32_t i = 0; i < nq; ++i) total += qlen[i];
     for (int32_t i = 0; i < nt; ++i) total += tlen[i];
@@ -346,59 +510,53 @@ int group_nb(int clen) {
     return 64;
 }
 
-int dir_R(int qlen) {
-    int r = 1;
-    while (r < 8 && 64 * r < qlen) r *= 2;
-    return r;
-}
-
 size_t lds_bytes(int KP) { return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE; }
 
-int check_pairs(pw_ctx *ctx, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
+int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
     if (n_pairs < 0 || n_pairs > (int64_t)INT32_MAX) {
-        ctx->err = "n_pairs out of range";
+        err = "n_pairs out of range";
         return PW_EINVAL;
     }
     for (int64_t i = 0; i < n_pairs; ++i)
         if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
-            ctx->err = "pair " + std::to_string(i) + " indexes outside the sequence lists";
+            err = "pair " + std::to_string(i) + " indexes outside the sequence lists";
             return PW_EINVAL;
         }
     return PW_OK;
 }
 
-void free_buf(DevBuf &b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
+void free_plan_buffers(pw_plan *P) {
+    DevBuf *bufs[] = {&P->res, &P->seqs, &P->tables, &P->d_items, &P->d_trans, &P->d_gitems,
+                      &P->d_gents, &P->d_mss, &P->d_pairs, &P->d_same};
+    for (DevBuf *b : bufs) free_buf(*b);
+    free_workspace(P->own);
+    if (P->graph) (void)hipGraphExecDestroy(P->graph);
+    P->graph = nullptr;
+    P->graph_sig.clear();
 }
 
 void free_plan(pw_plan *P) {
     if (!P) return;
-    free_buf(P->res);
-    free_buf(P->seqs);
-    free_buf(P->tables);
-    free_buf(P->items);
-    free_buf(P->trans);
-    free_buf(P->gitems);
-    free_buf(P->gents);
-    free_buf(P->d_mss);
+    free_plan_buffers(P);
     delete P;
 }
 
-// Encodes and uploads the sequences and builds the scoring work list.
-int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
-            const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, const int32_t *pq,
-            const int32_t *pt, int64_t n_pairs, const float *mss, const uint8_t *same_id, int32_t mode) {
+// Host half of pw_prepare: encodes the residues and builds the scoring work
+// list.  No device call.
+int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P, const uint8_t *qres,
+              const int64_t *qoff, const int32_t *qlen, int32_t nq, const uint8_t *tres, const int64_t *toff,
+              const int32_t *tlen, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs,
+              const float *mss, const uint8_t *same_id, int32_t mode) {
     if (mode != 0 && mode != 1) {
-        ctx->err = "mode must be 0 (plain per-batch launch) or 1 (batched persistent)";
+        err = "mode must be 0 (plain per-batch launch) or 1 (batched persistent)";
         return PW_EINVAL;
     }
-    int rc = check_pairs(ctx, nq, nt, pq, pt, n_pairs);
+    int rc = check_pairs(err, nq, nt, pq, pt, n_pairs);
     if (rc) return rc;
-    rc = encode(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, P->E);
+    P->E = Encoded();
+    rc = encode(err, sc, qres, qoff, qlen, nq, tres, toff, tlen, nt, P->E);
     if (rc) return rc;
-    Encoded &E = P->E;
+    P->sc = sc;
     P->nq = nq;
     P->nt = nt;
     P->n_pairs = n_pairs;
@@ -412,10 +570,29 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
     if (same_id) P->same.assign(same_id, same_id + n_pairs);
     else P->same.assign((size_t)n_pairs, 0);
     P->cells = 0;
-    for (int64_t i = 0; i < n_pairs; ++i) P->cells += (double)qlen[pq[i]] * (double)tlen[pt[i]];
+    P->max_t = 0;
+    P->dir_worst = 0;
+    P->dir_largest = 0;
+    for (int64_t i = 0; i < n_pairs; ++i) {
+        const int ql = qlen[pq[i]], tl = tlen[pt[i]];
+        P->cells += (double)ql * (double)tl;
+        P->max_t = std::max<int64_t>(P->max_t, tl);
+        const int64_t a = dir_area(ql, tl, dir_residues(ql));
+        P->dir_worst += a;
+        P->dir_largest = std::max(P->dir_largest, a);
+    }
+    P->batches.clear();
+    P->gitems.clear();
+    P->gents.clear();
+    P->n_groups = 0;
+    P->group_rows = 0;
+    P->max_nb = 16;
+    for (int w = 0; w < NB_KINDS; ++w) P->nb_begin[w] = P->nb_end[w] = 0;
 
-    std::vector<ScoreItem> items((size_t)n_pairs);
-    std::vector<uint8_t> trans((size_t)n_pairs, 0);
+    std::vector<ScoreItem> &items = P->items;
+    std::vector<uint8_t> &trans = P->trans;
+    items.assign((size_t)n_pairs, ScoreItem{});
+    trans.assign((size_t)n_pairs, 0);
     int64_t max_b = 0;
     if (mode == 0) {
         // The reference's batching: consecutive pairs form a batch while the
@@ -423,10 +600,8 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
         // 202-221, 420-432: 1 direction byte + 4 (linear) or 12 (affine) bytes
         // of FP32 matrix per cell), every pair of a batch padded to the
         // batch's longest query and target rounded up to 8 (:459-461, :325).
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(ctx, hipMemGetInfo(&free_b, &total_b));
-        const double budget = 0.8 * (double)total_b;
-        const double bpc = ctx->affine ? 13.0 : 5.0;
+        const double budget = 0.8 * (double)total_mem;
+        const double bpc = sc.affine ? 13.0 : 5.0;
         int64_t begin = 0;
         int bx = 0, by = 0;
         auto close = [&](int64_t end) {
@@ -454,7 +629,7 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
         for (int64_t i = 0; i < n_pairs; ++i) by_target[pt[i]].push_back(i);
         std::vector<int64_t> tile_pairs;
         std::vector<GroupItem> gitems;
-        std::vector<GroupEntry> gents;
+        std::vector<GroupEntry> &gents = P->gents;
         std::vector<double> gcost;
         for (int32_t t = 0; t < nt; ++t) {
             std::vector<int64_t> &L = by_target[t];
@@ -502,30 +677,20 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
             if (gitems[x].nb != gitems[y].nb) return gitems[x].nb < gitems[y].nb;
             return gcost[x] > gcost[y];
         });
-        std::vector<GroupItem> gsorted(gitems.size());
-        for (size_t i = 0; i < gorder.size(); ++i) gsorted[i] = gitems[gorder[i]];
+        P->gitems.resize(gitems.size());
+        for (size_t i = 0; i < gorder.size(); ++i) P->gitems[i] = gitems[gorder[i]];
         for (int w = 0; w < NB_KINDS; ++w) {
-            P->nb_begin[w] = P->nb_end[w] = 0;
-            for (size_t i = 0; i < gsorted.size(); ++i)
-                if (gsorted[i].nb == NB_WIDTHS[w]) {
+            for (size_t i = 0; i < P->gitems.size(); ++i)
+                if (P->gitems[i].nb == NB_WIDTHS[w]) {
                     if (P->nb_end[w] == 0) P->nb_begin[w] = (int64_t)i;
                     P->nb_end[w] = (int64_t)i + 1;
                 }
         }
-        P->n_groups = (int64_t)gsorted.size();
-        ENSURE(ctx, P->gitems, gsorted.size() * sizeof(GroupItem));
-        ENSURE(ctx, P->gents, gents.size() * sizeof(GroupEntry));
-        if (!gsorted.empty()) {
-            HIPCHK(ctx, hipMemcpyAsync(P->gitems.p, gsorted.data(), gsorted.size() * sizeof(GroupItem),
-                                       hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(P->gents.p, gents.data(), gents.size() * sizeof(GroupEntry),
-                                       hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        }
+        P->n_groups = (int64_t)P->gitems.size();
 
         const int64_t ntile = (int64_t)tile_pairs.size();
-        items.resize((size_t)ntile);
-        trans.assign((size_t)ntile, 0);
+        std::vector<ScoreItem> titems((size_t)ntile);
+        std::vector<uint8_t> ttrans((size_t)ntile, 0);
         std::vector<std::pair<double, int64_t>> order((size_t)ntile);
         for (int64_t k = 0; k < ntile; ++k) {
             const int64_t i = tile_pairs[k];
@@ -534,8 +699,8 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
             choose_shape(ql, tl, tr, R);
             const int a = tr ? tl : ql, b = tr ? ql : tl;
             const int aseq = tr ? nq + pt[i] : pq[i], bseq = tr ? pq[i] : nq + pt[i];
-            items[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b};
-            trans[k] = (uint8_t)tr;
+            titems[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b};
+            ttrans[k] = (uint8_t)tr;
             const int rows = 64 * R;
             order[k] = {-(double)((a + rows - 1) / rows) * (b + 63) * (R + 1.5), k};
             max_b = std::max<int64_t>(max_b, b);
@@ -544,37 +709,51 @@ int prepare(pw_ctx *ctx, pw_plan *P, const uint8_t *qres, const int64_t *qoff, c
                          [](const std::pair<double, int64_t> &x, const std::pair<double, int64_t> &y) {
                              return x.first < y.first;
                          });
-        std::vector<ScoreItem> sorted((size_t)ntile);
-        std::vector<uint8_t> strans((size_t)ntile);
+        items.resize((size_t)ntile);
+        trans.resize((size_t)ntile);
         for (int64_t k = 0; k < ntile; ++k) {
-            sorted[k] = items[order[k].second];
-            strans[k] = trans[order[k].second];
+            items[k] = titems[order[k].second];
+            trans[k] = ttrans[order[k].second];
         }
-        items.swap(sorted);
-        trans.swap(strans);
     }
     P->max_b = max_b;
     P->n_tile = (int64_t)items.size();
+    return PW_OK;
+}
+
+// Device half of pw_prepare: uploads everything plan_host built, on stream s.
+int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s) {
+    const Encoded &E = P->E;
+    const int64_t n = P->n_pairs;
     ENSURE(ctx, P->res, E.codes.size());
     ENSURE(ctx, P->seqs, E.refs.size() * sizeof(SeqRef));
     ENSURE(ctx, P->tables, E.tables.size() * sizeof(float));
-    ENSURE(ctx, P->items, items.size() * sizeof(ScoreItem));
-    ENSURE(ctx, P->trans, trans.size());
-    HIPCHK(ctx, hipMemcpyAsync(P->res.p, E.codes.data(), E.codes.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(P->seqs.p, E.refs.data(), E.refs.size() * sizeof(SeqRef), hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(P->tables.p, E.tables.data(), E.tables.size() * sizeof(float),
-                               hipMemcpyHostToDevice, ctx->stream));
-    ENSURE(ctx, P->d_mss, (size_t)n_pairs * sizeof(float));
-    if (n_pairs > 0)
-        HIPCHK(ctx, hipMemcpyAsync(P->d_mss.p, P->mss.data(), (size_t)n_pairs * sizeof(float),
-                                   hipMemcpyHostToDevice, ctx->stream));
-    if (!items.empty()) {
-        HIPCHK(ctx, hipMemcpyAsync(P->items.p, items.data(), items.size() * sizeof(ScoreItem),
-                                   hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(P->trans.p, trans.data(), trans.size(), hipMemcpyHostToDevice, ctx->stream));
-    }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ENSURE(ctx, P->d_items, P->items.size() * sizeof(ScoreItem));
+    ENSURE(ctx, P->d_trans, P->trans.size());
+    ENSURE(ctx, P->d_gitems, P->gitems.size() * sizeof(GroupItem));
+    ENSURE(ctx, P->d_gents, P->gents.size() * sizeof(GroupEntry));
+    ENSURE(ctx, P->d_mss, (size_t)n * sizeof(float));
+    ENSURE(ctx, P->d_pairs, (size_t)n * sizeof(int2));
+    ENSURE(ctx, P->d_same, (size_t)n);
+    std::vector<int2> pairs((size_t)n);
+    for (int64_t i = 0; i < n; ++i) pairs[i] = make_int2(P->pq[i], P->pt[i]);
+    auto up = [&](DevBuf &d, const void *h, size_t bytes) -> int {
+        if (bytes) HIPCHK(ctx, hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+        return PW_OK;
+    };
+    int rc = 0;
+    if ((rc = up(P->res, E.codes.data(), E.codes.size()))) return rc;
+    if ((rc = up(P->seqs, E.refs.data(), E.refs.size() * sizeof(SeqRef)))) return rc;
+    if ((rc = up(P->tables, E.tables.data(), E.tables.size() * sizeof(float)))) return rc;
+    if ((rc = up(P->d_items, P->items.data(), P->items.size() * sizeof(ScoreItem)))) return rc;
+    if ((rc = up(P->d_trans, P->trans.data(), P->trans.size()))) return rc;
+    if ((rc = up(P->d_gitems, P->gitems.data(), P->gitems.size() * sizeof(GroupItem)))) return rc;
+    if ((rc = up(P->d_gents, P->gents.data(), P->gents.size() * sizeof(GroupEntry)))) return rc;
+    if ((rc = up(P->d_mss, P->mss.data(), (size_t)n * sizeof(float)))) return rc;
+    if ((rc = up(P->d_pairs, pairs.data(), (size_t)n * sizeof(int2)))) return rc;
+    if ((rc = up(P->d_same, P->same.data(), (size_t)n))) return rc;
+    // the pageable sources above must stay alive until the copies finish
+    HIPCHK(ctx, hipStreamSynchronize(s));
     return PW_OK;
 }
 
@@ -597,99 +776,183 @@ int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(work, g));
 }
 
-// Scoring pass over all pairs of the plan; maxima land in ctx->maxima.
-int run_score(pw_ctx *ctx, pw_plan *P, double *ms_score, int64_t *launches) {
-    const int64_t n_pairs = P->n_pairs;
-    ENSURE(ctx, ctx->maxima, (size_t)n_pairs * sizeof(float));
-    if (P->n_tile > 0) {  // sized before any launch, for the same reason as the group scratch
-        const size_t lds = lds_bytes(P->E.KP);
-        const int g = persistent_grid(ctx, (const void *)(ctx->affine ? score_kernel<true> : score_kernel<false>), lds,
-                                      P->n_tile);
-        ENSURE(ctx, ctx->scratch2, (size_t)g * 3 * (P->max_b + 64) * sizeof(float));
-    }
-    // work counters on separate 64-byte lines: [0] tile jobs, [16*(1+w)] jobs of chunk width w
-    ENSURE(ctx, ctx->queue, 64 * (1 + NB_KINDS));
-    HIPCHK(ctx, hipMemsetAsync(ctx->queue.p, 0, 64 * (1 + NB_KINDS), ctx->stream));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+size_t group_lds(const pw_plan *P, int w) { return (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float); }
+
+Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
+    Geometry g;
+    const bool aff = P->sc.affine;
     if (P->n_groups > 0) {
-        const int64_t scr_stride = std::max<int64_t>(1, P->group_rows) * WAVE;
-        // size the scratch once for the largest grid: resizing frees memory a
-        // kernel launched just before may still be using
-        int grid_all = 1;
+        g.gscr_stride = std::max<int64_t>(1, P->group_rows) * WAVE;
         for (int w = 0; w < NB_KINDS; ++w) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
-            if (n <= 0) continue;
-            const size_t lds = (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float);
-            grid_all = std::max(grid_all, persistent_grid(ctx, group_kernel_for(ctx->affine, NB_WIDTHS[w]), lds, n));
+            if (n > 0) g.group_grid[w] = persistent_grid(ctx, group_kernel_for(aff, NB_WIDTHS[w]), group_lds(P, w), n);
         }
-        ENSURE(ctx, ctx->scratch, (size_t)grid_all * scr_stride * sizeof(float2));
+    }
+    if (P->n_tile > 0) {
+        g.tile_grid = persistent_grid(ctx, (const void *)(aff ? score_kernel<true> : score_kernel<false>),
+                                      lds_bytes(P->E.KP), P->n_tile);
+        g.tscr_stride = 3 * (P->max_b + 64);
+    }
+    const int64_t n = P->n_pairs;
+    g.gate_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
+    g.dir_grid = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)ctx->num_cus * 8));
+    g.dscr_stride = 3 * (P->max_t + 64);
+    g.task_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * 4));
+    return g;
+}
+
+int64_t dir_limit(pw_ctx *ctx) {
+    return ctx->lim.dir_max > 0 ? ctx->lim.dir_max : (int64_t)(ctx->total_mem / 4);
+}
+
+// Sizes the work areas of W for an execution of P; every allocation happens
+// here, before the first launch.
+int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
+    const int64_t n = P->n_pairs;
+    if (W.cand0 <= 0) W.cand0 = ctx->lim.cand0;
+    if (W.want_dir <= 0) W.want_dir = ctx->lim.dir_init;
+    if (W.want_ops <= 0) W.want_ops = ctx->lim.ops_init;
+    if (W.want_cand <= 0) W.want_cand = (int64_t)W.cand0 * 16384;
+    ENSURE(ctx, W.ctl, sizeof(Ctl));
+    ENSURE(ctx, W.maxima, (size_t)n * sizeof(float));
+    ENSURE(ctx, W.tasks, (size_t)n * sizeof(PairTask));
+    const int64_t dcap = std::max<int64_t>(16, std::min(P->dir_worst, std::min(W.want_dir, dir_limit(ctx))));
+    ENSURE(ctx, W.dirs, (size_t)dcap);
+    W.dir_cap = (int64_t)W.dirs.cap;
+    const int64_t ccap = std::max<int64_t>(16, std::min<int64_t>(n * (int64_t)W.cand0, W.want_cand));
+    ENSURE(ctx, W.cands, (size_t)ccap * sizeof(Cand));
+    ENSURE(ctx, W.spslot, (size_t)ccap * sizeof(SpRec));
+    ENSURE(ctx, W.keys, (size_t)ccap * sizeof(uint64_t));
+    ENSURE(ctx, W.out_sps, (size_t)ccap * sizeof(SpRec));
+    ENSURE(ctx, W.out_hits, (size_t)ccap * sizeof(HitRec));
+    W.cand_cap = std::min<int64_t>({(int64_t)(W.cands.cap / sizeof(Cand)), (int64_t)(W.spslot.cap / sizeof(SpRec)),
+                                    (int64_t)(W.keys.cap / sizeof(uint64_t)),
+                                    (int64_t)(W.out_sps.cap / sizeof(SpRec)),
+                                    (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
+    ENSURE(ctx, W.ops, (size_t)std::max<int64_t>(16, W.want_ops));
+    W.ops_cap = (int64_t)W.ops.cap;
+    int gmax = 0;
+    for (int w = 0; w < NB_KINDS; ++w) gmax = std::max(gmax, g.group_grid[w]);
+    if (gmax > 0) ENSURE(ctx, W.gscratch, (size_t)gmax * g.gscr_stride * sizeof(float2));
+    if (g.tile_grid > 0) ENSURE(ctx, W.tscratch, (size_t)g.tile_grid * g.tscr_stride * sizeof(float));
+    ENSURE(ctx, W.dscratch, (size_t)g.dir_grid * g.dscr_stride * sizeof(float));
+    ENSURE_PINNED(ctx, W.h_ctl, sizeof(Ctl));
+    return PW_OK;
+}
+
+// Scoring pass: queue counters live in the (already cleared) control block.
+int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, bool events,
+                 int64_t *launches) {
+    const bool aff = P->sc.affine;
+    Ctl *ctl = (Ctl *)W.ctl.p;
+    if (events) HIPCHK(ctx, hipEventRecord(L.ev[0], L.s));
+    const uint8_t *res = (const uint8_t *)P->res.p;
+    const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+    const float *tabs = (const float *)P->tables.p;
+    float *mx = (float *)W.maxima.p;
+    if (P->n_groups > 0) {
         for (int w = 0; w < NB_KINDS; ++w) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
             if (n <= 0) continue;
-            const void *kern = group_kernel_for(ctx->affine, NB_WIDTHS[w]);
-            const size_t lds = (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float);
-            const int grid = persistent_grid(ctx, kern, lds, n);
-            const GroupItem *items = (const GroupItem *)P->gitems.p + P->nb_begin[w];
-            const uint8_t *res = (const uint8_t *)P->res.p;
-            const SeqRef *seqs = (const SeqRef *)P->seqs.p;
-            const GroupEntry *ents = (const GroupEntry *)P->gents.p;
-            const float *tabs = (const float *)P->tables.p;
+            const void *kern = group_kernel_for(aff, NB_WIDTHS[w]);
+            const GroupItem *items = (const GroupItem *)P->d_gitems.p + P->nb_begin[w];
+            const GroupEntry *ents = (const GroupEntry *)P->d_gents.p;
             int kp = P->E.KP, ni = (int)n;
-            float g = ctx->gap, e = ctx->ext;
-            float2 *scr = (float2 *)ctx->scratch.p;
-            int64_t ss = scr_stride;
-            unsigned *q = (unsigned *)ctx->queue.p + 16 * (1 + w);
-            float *mx = (float *)ctx->maxima.p;
-            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &g, &e, &scr, &ss, &q, &mx};
-            HIPCHK(ctx, hipLaunchKernel(kern, dim3(grid), dim3(64), args, lds, ctx->stream));
+            float gp = P->sc.gap, e = P->sc.ext;
+            float2 *scr = (float2 *)W.gscratch.p;
+            int64_t ss = g.gscr_stride;
+            unsigned *q = ctl->queue + 16 * (1 + w);
+            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &gp, &e, &scr, &ss, &q, &mx};
+            HIPCHK(ctx, hipLaunchKernel(kern, dim3(g.group_grid[w]), dim3(64), args, group_lds(P, w), L.s));
             if (launches) *launches += 1;
         }
     }
     if (P->n_tile > 0) {
-        auto kern = ctx->affine ? score_kernel<true> : score_kernel<false>;
+        auto kern = aff ? score_kernel<true> : score_kernel<false>;
         const size_t lds = lds_bytes(P->E.KP);
-        const int64_t scr_stride = 3 * (P->max_b + 64);
-        const int grid_max = persistent_grid(ctx, (const void *)kern, lds, P->n_tile);
-        const ScoreItem *items = (const ScoreItem *)P->items.p;
-        const uint8_t *trans = (const uint8_t *)P->trans.p;
+        const ScoreItem *items = (const ScoreItem *)P->d_items.p;
+        const uint8_t *trans = (const uint8_t *)P->d_trans.p;
         if (P->mode == 0) {
             // plain per-batch launches: one launch per batch, static schedule
             for (const auto &b : P->batches) {
                 const int64_t n = b.second - b.first;
-                const int grid = (int)std::min<int64_t>(n, grid_max);
-                hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
-                                   (const SeqRef *)P->seqs.p, items + b.first, (int)n, trans + b.first,
-                                   (const float *)P->tables.p, P->E.KP, P->E.lkp, ctx->gap, ctx->ext,
-                                   (float *)ctx->scratch2.p, scr_stride, (unsigned *)nullptr,
-                                   (float *)ctx->maxima.p);
+                const int grid = (int)std::min<int64_t>(n, g.tile_grid);
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, L.s, res, seqs, items + b.first, (int)n,
+                                   trans + b.first, tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
+                                   (float *)W.tscratch.p, g.tscr_stride, (unsigned *)nullptr, mx);
                 HIPCHK(ctx, hipGetLastError());
                 if (launches) *launches += 1;
             }
         } else {
             // persistent workers on the second stream, overlapping the lane-per-pair kernels
-            hipStream_t st = P->n_groups > 0 ? ctx->stream2 : ctx->stream;
-            if (st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(st, ctx->ev[0], 0));
-            hipLaunchKernelGGL(kern, dim3(grid_max), dim3(64), lds, st, (const uint8_t *)P->res.p,
-                               (const SeqRef *)P->seqs.p, items, (int)P->n_tile, trans, (const float *)P->tables.p,
-                               P->E.KP, P->E.lkp, ctx->gap, ctx->ext, (float *)ctx->scratch2.p, scr_stride,
-                               (unsigned *)ctx->queue.p, (float *)ctx->maxima.p);
+            hipStream_t st = P->n_groups > 0 ? L.s2 : L.s;
+            if (st != L.s) {
+                HIPCHK(ctx, hipEventRecord(L.fork, L.s));
+                HIPCHK(ctx, hipStreamWaitEvent(st, L.fork, 0));
+            }
+            hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
+                               tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
+                               ctl->queue, mx);
             HIPCHK(ctx, hipGetLastError());
             if (launches) *launches += 1;
-            if (st != ctx->stream) {
-                HIPCHK(ctx, hipEventRecord(ctx->ev[8], st));
-                HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev[8], 0));
+            if (st != L.s) {
+                HIPCHK(ctx, hipEventRecord(L.join, st));
+                HIPCHK(ctx, hipStreamWaitEvent(L.s, L.join, 0));
             }
         }
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    (void)ms_score;  // read from events ev[0]..ev[1] once the stream has been synchronised
+    if (events) HIPCHK(ctx, hipEventRecord(L.ev[1], L.s));
     return PW_OK;
 }
 
-int score_ms(pw_ctx *ctx, double *ms_score) {
-    float ms = 0.0f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    if (ms_score) *ms_score += ms;
+// Direction pass, start selection and walk over the device-side task list.
+int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum,
+                     bool events) {
+    const bool aff = P->sc.affine;
+    Ctl *ctl = (Ctl *)W.ctl.p;
+    PairTask *tasks = (PairTask *)W.tasks.p;
+    const uint8_t *res = (const uint8_t *)P->res.p;
+    const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+    auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
+    hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks, (const Ctl *)ctl,
+                       (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.dscratch.p,
+                       g.dscr_stride, (uint8_t *)W.dirs.p, (Cand *)W.cands.p);
+    HIPCHK(ctx, hipGetLastError());
+    if (events) HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
+    auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
+    hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)ctl, (uint8_t *)W.dirs.p,
+                       (Cand *)W.cands.p, (SpRec *)W.spslot.p);
+    HIPCHK(ctx, hipGetLastError());
+    if (events) HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
+    auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
+    hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, (uint8_t *)W.dirs.p,
+                       (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
+                       (char *)W.ops.p, W.ops_cap, minimum);
+    HIPCHK(ctx, hipGetLastError());
+    if (events) HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
+    return PW_OK;
+}
+
+// The whole device sequence of one execution (capturable: kernels, event
+// fork/join and one device-to-host copy into pinned memory).
+int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, bool events,
+                    int64_t *launches) {
+    Ctl *ctl = (Ctl *)W.ctl.p;
+    hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(256), 0, L.s, (uint32_t *)ctl, (int)(sizeof(Ctl) / 4));
+    HIPCHK(ctx, hipGetLastError());
+    int rc = launch_score(ctx, P, W, L, g, events, launches);
+    if (rc) return rc;
+    if (P->n_pairs > 0) {
+        hipLaunchKernelGGL(gate_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
+                           (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
+                           (const SeqRef *)P->seqs.p, P->nq, P->n_pairs, P->sc.minimum, P->sc.ll, ctl,
+                           (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
+    rc = launch_traceback(ctx, P, W, L, g, minimum, events);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
     return PW_OK;
 }
 
@@ -702,266 +965,382 @@ struct HostResult {
     std::vector<char> ops;
 };
 
-int execute(pw_ctx *ctx, pw_plan *P, double minimum, int32_t sp_cap, HostResult *R) {
-    const auto wall0 = std::chrono::steady_clock::now();
-    const int64_t n_pairs = P->n_pairs;
-    const int32_t nq = P->nq;
-    const int32_t *pq = P->pq.data(), *pt = P->pt.data(), *qlen = P->qlen.data(), *tlen = P->tlen.data();
-    const float *min_start_score = P->mss.data();
-    Encoded &E = P->E;
-    double ms_score = 0;
-    int64_t launches = 0;
-    int rc = run_score(ctx, P, &ms_score, &launches);
-    if (rc) return rc;
-    // traceback gate on the device: maximum >= MINIMUM_SCORE (default_main_cpu.cl:410)
-    // and >= the pair's minimum start score, without which no cell can start (:427)
-    ENSURE(ctx, ctx->gated, (size_t)(n_pairs + 16) * sizeof(int));
-    HIPCHK(ctx, hipMemsetAsync(ctx->gated.p, 0, sizeof(int), ctx->stream));
-    if (n_pairs > 0) {
-        const int grid = (int)std::min<int64_t>((n_pairs + 255) / 256, (int64_t)ctx->num_cus * 16);
-        hipLaunchKernelGGL(gate_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const float *)ctx->maxima.p,
-                           (const float *)P->d_mss.p, n_pairs, ctx->minimum, (int *)ctx->gated.p,
-                           (int *)ctx->gated.p + 16);
-        HIPCHK(ctx, hipGetLastError());
-    }
-    if (n_pairs > 0) {
-        // reuse a pinned buffer from an earlier, freed result when one is large enough
-        for (size_t i = 0; i < ctx->pinned_free.size(); ++i)
-            if (ctx->pinned_free[i].second >= (size_t)n_pairs) {
-                R->maxima = ctx->pinned_free[i].first;
-                R->maxima_cap = ctx->pinned_free[i].second;
-                ctx->pinned_free.erase(ctx->pinned_free.begin() + i);
-                break;
-            }
-        if (!R->maxima) {
-            HIPCHK(ctx, hipHostMalloc((void **)&R->maxima, (size_t)n_pairs * sizeof(float), hipHostMallocDefault));
-            R->maxima_cap = (size_t)n_pairs;
-        }
-        HIPCHK(ctx, hipMemcpyAsync(R->maxima, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-    }
-    int ngated = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&ngated, ctx->gated.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    rc = score_ms(ctx, &ms_score);
-    if (rc) return rc;
-    std::vector<int64_t> gated((size_t)ngated);
-    if (ngated > 0) {
-        std::vector<int> g32((size_t)ngated);
-        HIPCHK(ctx, hipMemcpy(g32.data(), (int *)ctx->gated.p + 16, (size_t)ngated * sizeof(int),
-                              hipMemcpyDeviceToHost));
-        std::sort(g32.begin(), g32.end());
-        for (int i = 0; i < ngated; ++i) gated[i] = g32[i];
-    }
-    const int ng = (int)gated.size();
-    double ms_dir = 0, ms_tb = 0, ms_walk = 0;
-    if (ng > 0) {
-        std::vector<TbItem> tb(ng);
-        std::vector<int> spc(ng, 0);
-        std::vector<int64_t> opsused(ng, 0);
-        // --- direction pass ---
-        std::vector<DirItem> di(ng);
-        int64_t dir_total = 0, cand_total = 0;
-        const int CAND0 = 16;
-        int64_t max_t = 0;
-        for (int g = 0; g < ng; ++g) {
-            const int64_t p = gated[g];
-            const int ql = qlen[pq[p]], tl = tlen[pt[p]];
-            const int Rr = dir_R(ql);
-            const int rows = 64 * Rr, stripes = std::max(1, (ql + rows - 1) / rows);
-            DirItem d;
-            d.q_seq = pq[p];
-            d.t_seq = nq + pt[p];
-            d.pair = (int32_t)p;
-            d.R = Rr;
-            d.dir_off = dir_total;
-            d.t1 = ctx->ll * R->maxima[p];  // LOWER_LIMIT_SCORE * maximum, f32
-            d.mp = min_start_score[p];
-            d.cand_off = cand_total;
-            d.cand_cap = CAND0;
-            d.pad = 0;
-            di[g] = d;
-            dir_total += (int64_t)stripes * (tl + 63) * 64 * Rr;
-            dir_total = (dir_total + 15) & ~(int64_t)15;
-            cand_total += CAND0;
-            max_t = std::max<int64_t>(max_t, tl);
-        }
-        const size_t lds = lds_bytes(E.KP);
-        const int64_t scr_stride = 3 * (max_t + 64);
-        auto run_dir = [&](const std::vector<DirItem> &list, double &ms) -> int {
-            const int n = (int)list.size();
-            const int grid = std::max(1, std::min(n, ctx->num_cus * 8));
-            ENSURE(ctx, ctx->ditems, list.size() * sizeof(DirItem));
-            ENSURE(ctx, ctx->scratch, (size_t)grid * scr_stride * sizeof(float));
-            HIPCHK(ctx, hipMemcpyAsync(ctx->ditems.p, list.data(), list.size() * sizeof(DirItem),
-                                       hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(ctx, hipMemsetAsync(ctx->ccount.p, 0, (size_t)n * sizeof(int), ctx->stream));
-            HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-            auto kern = ctx->affine ? dir_kernel<true> : dir_kernel<false>;
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, (const uint8_t *)P->res.p,
-                               (const SeqRef *)P->seqs.p, (const DirItem *)ctx->ditems.p, n,
-                               (const float *)P->tables.p, E.KP, E.lkp, ctx->gap, ctx->ext, (float *)ctx->scratch.p,
-                               scr_stride, (uint8_t *)ctx->dirs.p, (int *)ctx->ccount.p, (Cand *)ctx->cands.p);
-            HIPCHK(ctx, hipGetLastError());
-            HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-            HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
-            float t = 0;
-            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
-            ms += t;
+// Finished traceback work of one gated pair: where its records sit in the
+// read-back arrays.
+struct PairOut {
+    int32_t pair;
+    int32_t nsp;
+    const SpRec *sps;
+    const HitRec *hits;
+    const char *ops;
+    int64_t nops;
+};
+
+// Host copies of the records of the tail rounds (kept alive until assembly).
+struct TailStore {
+    std::vector<std::vector<SpRec>> sps;
+    std::vector<std::vector<HitRec>> hits;
+    std::vector<std::vector<char>> ops;
+};
+
+int take_maxima_buffer(pw_ctx *ctx, int64_t n, HostResult *R) {
+    if (n <= 0) return PW_OK;
+    for (size_t i = 0; i < ctx->pinned_free.size(); ++i)
+        if (ctx->pinned_free[i].second >= (size_t)n) {
+            R->maxima = ctx->pinned_free[i].first;
+            R->maxima_cap = ctx->pinned_free[i].second;
+            ctx->pinned_free.erase(ctx->pinned_free.begin() + i);
             return PW_OK;
-        };
-        ENSURE(ctx, ctx->dirs, (size_t)dir_total);
-        ENSURE(ctx, ctx->ccount, (size_t)ng * sizeof(int));
-        ENSURE(ctx, ctx->cands, (size_t)cand_total * sizeof(Cand));
-        rc = run_dir(di, ms_dir);
+        }
+    HIPCHK(ctx, hipHostMalloc((void **)&R->maxima, (size_t)n * sizeof(float), hipHostMallocDefault));
+    R->maxima_cap = (size_t)n;
+    return PW_OK;
+}
+
+// Reads the tasks and compacted records of a finished device sequence whose
+// control block is in W.h_ctl; finished pairs go to outs, the others to pending.
+int read_tasks(pw_ctx *ctx, Workspace &W, Lane &L, int ntasks, std::vector<PairOut> &outs,
+               std::vector<PairTask> &pending) {
+    const Ctl &c = *(const Ctl *)W.h_ctl.p;
+    if (ntasks <= 0) return PW_OK;
+    const int64_t nout = std::min<int64_t>((int64_t)c.out_used, W.cand_cap);
+    const int64_t nops = std::min<int64_t>((int64_t)c.ops_used, W.ops_cap);
+    ENSURE_PINNED(ctx, W.h_tasks, (size_t)ntasks * sizeof(PairTask));
+    ENSURE_PINNED(ctx, W.h_sps, (size_t)nout * sizeof(SpRec));
+    ENSURE_PINNED(ctx, W.h_hits, (size_t)nout * sizeof(HitRec));
+    ENSURE_PINNED(ctx, W.h_ops, (size_t)nops);
+    HIPCHK(ctx, hipMemcpyAsync(W.h_tasks.p, W.tasks.p, (size_t)ntasks * sizeof(PairTask), hipMemcpyDeviceToHost,
+                               L.s));
+    if (nout > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(W.h_sps.p, W.out_sps.p, (size_t)nout * sizeof(SpRec), hipMemcpyDeviceToHost, L.s));
+        HIPCHK(ctx, hipMemcpyAsync(W.h_hits.p, W.out_hits.p, (size_t)nout * sizeof(HitRec), hipMemcpyDeviceToHost,
+                                   L.s));
+    }
+    if (nops > 0) HIPCHK(ctx, hipMemcpyAsync(W.h_ops.p, W.ops.p, (size_t)nops, hipMemcpyDeviceToHost, L.s));
+    HIPCHK(ctx, hipStreamSynchronize(L.s));
+    const PairTask *T = (const PairTask *)W.h_tasks.p;
+    for (int i = 0; i < ntasks; ++i) {
+        const PairTask &t = T[i];
+        if (t.status != TASK_OK) {
+            pending.push_back(t);
+            continue;
+        }
+        if ((int64_t)t.out_off + t.nsp > nout || t.ops_off + t.nops > nops) {
+            ctx->err = "traceback records outside the read-back range";
+            return PW_EDEVICE;
+        }
+        outs.push_back(PairOut{t.pair, t.nsp, (const SpRec *)W.h_sps.p + t.out_off,
+                               (const HitRec *)W.h_hits.p + t.out_off, (const char *)W.h_ops.p + t.ops_off, t.nops});
+    }
+    return PW_OK;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+// Tail rounds: pairs that did not fit the work areas (or overflowed their
+// candidate or ops ranges) run again, host-driven, with exact sizes.
+int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum,
+                std::vector<PairTask> pending, std::vector<PairOut> &outs, TailStore &store, double *ms_dir,
+                double *ms_tb, double *ms_walk) {
+    int rounds = 0;
+    while (!pending.empty()) {
+        if (++rounds > 64) {
+            ctx->err = "traceback tail rounds did not converge";
+            return PW_EDEVICE;
+        }
+        // exact candidate capacity where the count is known
+        for (PairTask &t : pending) {
+            if (t.status == TASK_CAND_OVERFLOW || t.status == TASK_OPS_OVERFLOW) t.cand_cap = std::max(t.ncand, 1);
+            else t.cand_cap = std::max(t.cand_cap, 1);
+        }
+        // areas large enough for the first pair; then take pairs while they fit
+        const PairTask &f = pending.front();
+        const int64_t farea = dir_area(P->qlen[P->pq[f.pair]], P->tlen[P->pt[f.pair]], f.R);
+        ENSURE(ctx, W.dirs, (size_t)std::max<int64_t>(farea, W.dir_cap));
+        W.dir_cap = (int64_t)W.dirs.cap;
+        if (f.cand_cap > W.cand_cap) {
+            const size_t c = (size_t)f.cand_cap;
+            ENSURE(ctx, W.cands, c * sizeof(Cand));
+            ENSURE(ctx, W.spslot, c * sizeof(SpRec));
+            ENSURE(ctx, W.keys, c * sizeof(uint64_t));
+            ENSURE(ctx, W.out_sps, c * sizeof(SpRec));
+            ENSURE(ctx, W.out_hits, c * sizeof(HitRec));
+            W.cand_cap = std::min<int64_t>({(int64_t)(W.cands.cap / sizeof(Cand)),
+                                            (int64_t)(W.spslot.cap / sizeof(SpRec)),
+                                            (int64_t)(W.keys.cap / sizeof(uint64_t)),
+                                            (int64_t)(W.out_sps.cap / sizeof(SpRec)),
+                                            (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
+        }
+        std::vector<PairTask> chunk, rest;
+        int64_t dused = 0, cused = 0;
+        for (PairTask t : pending) {
+            const int64_t a = dir_area(P->qlen[P->pq[t.pair]], P->tlen[P->pt[t.pair]], t.R);
+            if (rest.empty() && dused + a <= W.dir_cap && cused + t.cand_cap <= W.cand_cap &&
+                (int64_t)chunk.size() < P->n_pairs) {
+                t.dir_off = dused;
+                t.cand_off = cused;
+                t.status = TASK_OK;
+                t.ncand = 0;
+                t.nsp = 0;
+                t.ops_off = 0;
+                t.nops = 0;
+                t.out_off = 0;
+                dused += a;
+                cused += t.cand_cap;
+                chunk.push_back(t);
+            } else {
+                rest.push_back(t);
+            }
+        }
+        const int nc = (int)chunk.size();
+        Ctl c;
+        std::memset(&c, 0, sizeof c);
+        c.ngated = nc;
+        HIPCHK(ctx, hipMemcpyAsync(W.tasks.p, chunk.data(), (size_t)nc * sizeof(PairTask), hipMemcpyHostToDevice,
+                                   L.s));
+        HIPCHK(ctx, hipMemcpyAsync(W.ctl.p, &c, sizeof c, hipMemcpyHostToDevice, L.s));
+        const bool aff = P->sc.affine;
+        Ctl *dctl = (Ctl *)W.ctl.p;
+        PairTask *tasks = (PairTask *)W.tasks.p;
+        const uint8_t *res = (const uint8_t *)P->res.p;
+        const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+        HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
+        auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
+        hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks,
+                           (const Ctl *)dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
+                           (float *)W.dscratch.p, g.dscr_stride, (uint8_t *)W.dirs.p, (Cand *)W.cands.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
+        auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
+        hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)dctl,
+                           (uint8_t *)W.dirs.p, (Cand *)W.cands.p, (SpRec *)W.spslot.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
+        // exact ops capacity from the starting-point counts
+        std::vector<PairTask> after((size_t)nc);
+        HIPCHK(ctx, hipMemcpyAsync(after.data(), W.tasks.p, (size_t)nc * sizeof(PairTask), hipMemcpyDeviceToHost,
+                                   L.s));
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        *ms_dir += elapsed(L.ev[2], L.ev[3]);
+        *ms_tb += elapsed(L.ev[3], L.ev[4]);
+        int64_t ops_need = 0;
+        for (const PairTask &t : after)
+            if (t.status == TASK_OK && t.nsp > 0) {
+                const int64_t ql = P->qlen[P->pq[t.pair]], tl = P->tlen[P->pt[t.pair]];
+                ops_need += std::min<int64_t>((int64_t)t.nsp * (ql + tl), ql * tl + t.nsp);
+            }
+        if (ops_need > W.ops_cap) {
+            ENSURE(ctx, W.ops, (size_t)ops_need);
+            W.ops_cap = (int64_t)W.ops.cap;
+        }
+        HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
+        auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
+        hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, dctl, (uint8_t *)W.dirs.p,
+                           (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
+                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
+        HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        *ms_walk += elapsed(L.ev[4], L.ev[5]);
+        std::vector<PairOut> got;
+        std::vector<PairTask> again;
+        int rc = read_tasks(ctx, W, L, nc, got, again);
         if (rc) return rc;
-        std::vector<int> cc(ng);
-        HIPCHK(ctx, hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost));
-        // pairs whose candidates overflowed the first guess run again with the
-        // exact capacity; the recomputed directions are identical
-        bool over = false;
-        for (int g = 0; g < ng; ++g)
-            if (cc[g] > di[g].cand_cap) over = true;
-        if (over) {
-            int64_t ctot = 0;
-            for (int g = 0; g < ng; ++g) {
-                const int cap = std::max(cc[g], 1);
-                di[g].cand_off = ctot;
-                di[g].cand_cap = cap;
-                ctot += cap;
-            }
-            ENSURE(ctx, ctx->cands, (size_t)ctot * sizeof(Cand));
-            rc = run_dir(di, ms_dir);
-            if (rc) return rc;
-            HIPCHK(ctx, hipMemcpy(cc.data(), ctx->ccount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost));
+        // copy this round's records out of the staging buffers the next round reuses
+        for (PairOut &o : got) {
+            store.sps.emplace_back(o.sps, o.sps + o.nsp);
+            store.hits.emplace_back(o.hits, o.hits + o.nsp);
+            store.ops.emplace_back(o.ops, o.ops + o.nops);
+            o.sps = store.sps.back().data();
+            o.hits = store.hits.back().data();
+            o.ops = store.ops.back().data();
+            outs.push_back(o);
         }
-        // --- start selection + marking ---
-        int64_t sp_total = 0;
-        for (int g = 0; g < ng; ++g) {
-            const int64_t p = gated[g];
-            TbItem t;
-            t.pair = (int32_t)p;
-            t.q_seq = pq[p];
-            t.t_seq = nq + pt[p];
-            t.R = di[g].R;
-            t.dir_off = di[g].dir_off;
-            t.cand_off = di[g].cand_off;
-            t.sp_off = sp_total;
-            t.ops_off = 0;
-            t.hit_off = sp_total;
-            t.key_off = sp_total;
-            t.maxv = R->maxima[p];
-            t.mp = min_start_score[p];
-            t.qidx = pq[p];
-            t.tidx = pt[p];
-            t.same_id = P->same[p];
-            t.ncand = std::min(cc[g], di[g].cand_cap);
-            tb[g] = t;
-            sp_total += t.ncand;
+        for (PairTask &t : again) rest.push_back(t);
+        pending.swap(rest);
+    }
+    return PW_OK;
+}
+
+// Sizes for the next execution after one that overflowed its work areas.
+void adapt_workspace(pw_ctx *ctx, Workspace &W, const Ctl &c, const std::vector<PairTask> &pending) {
+    if (!ctx->lim.grow || pending.empty()) return;
+    W.want_dir = std::min<int64_t>(std::max<int64_t>(W.want_dir, (int64_t)c.dir_used), dir_limit(ctx));
+    int32_t maxc = W.cand0;
+    for (const PairTask &t : pending)
+        if (t.status == TASK_CAND_OVERFLOW) maxc = std::max(maxc, t.ncand);
+    W.cand0 = std::min<int32_t>(maxc, 1024);
+    W.want_cand = std::max<int64_t>({W.want_cand, (int64_t)c.cand_used, (int64_t)c.ngated * W.cand0});
+    W.want_ops = std::max<int64_t>(W.want_ops, (int64_t)c.ops_used);
+}
+
+// Per-pair records in pair order into the host result.
+void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostResult *R) {
+    std::sort(outs.begin(), outs.end(), [](const PairOut &a, const PairOut &b) { return a.pair < b.pair; });
+    int64_t nsp = 0, nops = 0;
+    for (const PairOut &o : outs) {
+        nsp += o.nsp;
+        nops += o.nops;
+    }
+    R->sps.resize((size_t)nsp);
+    R->hits.resize((size_t)nsp);
+    R->ops.resize((size_t)nops);
+    static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
+    int64_t sp0 = 0, op0 = 0;
+    for (const PairOut &o : outs) {
+        if (o.nsp > 0) std::memcpy(&R->sps[sp0], o.sps, (size_t)o.nsp * sizeof(SpRec));
+        if (o.nops > 0) std::memcpy(&R->ops[op0], o.ops, (size_t)o.nops);
+        for (int k = 0; k < o.nsp; ++k) {
+            const HitRec &hr = o.hits[k];
+            pw_hit &h = R->hits[sp0 + k];
+            h.query = hr.query;
+            h.target = hr.target;
+            h.shown = hr.shown;
+            h.s_start = hr.s_start;
+            h.s_end = hr.s_end;
+            h.t_start = hr.t_start;
+            h.t_end = hr.t_end;
+            h.matches = hr.matches;
+            h.mismatches = hr.mismatches;
+            h.length = hr.length;
+            h.score = hr.score;
+            h.sp_index = (int32_t)(sp0 + hr.sp_index);
+            h.ops_offset = op0 + hr.ops_offset;
         }
-        ENSURE(ctx, ctx->tbitems, (size_t)ng * sizeof(TbItem));
-        ENSURE(ctx, ctx->sps, (size_t)sp_total * sizeof(SpRec));
-        ENSURE(ctx, ctx->spcount, (size_t)ng * sizeof(int));
-        HIPCHK(ctx, hipMemcpyAsync(ctx->tbitems.p, tb.data(), tb.size() * sizeof(TbItem), hipMemcpyHostToDevice,
-                                   ctx->stream));
-        HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        const int tb_grid = (ng + 63) / 64;
-        auto tbk = ctx->affine ? tb_kernel<true> : tb_kernel<false>;
-        hipLaunchKernelGGL(tbk, dim3(tb_grid), dim3(64), 0, ctx->stream, (const SeqRef *)P->seqs.p,
-                           (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p, (Cand *)ctx->cands.p,
-                           (SpRec *)ctx->sps.p, (int *)ctx->spcount.p);
-        HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(spc.data(), ctx->spcount.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        {
-            float t = 0;
-            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[4], ctx->ev[5]));
-            ms_tb += t;
-        }
-        // --- walk ---
-        int64_t ops_total = 0;
-        for (int g = 0; g < ng; ++g) {
-            const int64_t p = gated[g];
-            const int64_t ql = qlen[pq[p]], tl = tlen[pt[p]];
-            const int64_t cap = std::min<int64_t>((int64_t)spc[g] * (ql + tl), ql * tl + spc[g]);
-            tb[g].ops_off = ops_total;
-            ops_total += cap;
-        }
-        ENSURE(ctx, ctx->keys, (size_t)sp_total * sizeof(uint64_t));
-        ENSURE(ctx, ctx->hits, (size_t)sp_total * sizeof(HitRec));
-        ENSURE(ctx, ctx->ops, (size_t)ops_total + 1);
-        ENSURE(ctx, ctx->opsused, (size_t)ng * sizeof(int64_t));
-        HIPCHK(ctx, hipMemcpyAsync(ctx->tbitems.p, tb.data(), tb.size() * sizeof(TbItem), hipMemcpyHostToDevice,
-                                   ctx->stream));
-        HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-        auto wk = ctx->affine ? walk_kernel<true> : walk_kernel<false>;
-        hipLaunchKernelGGL(wk, dim3(tb_grid), dim3(64), 0, ctx->stream, (const uint8_t *)P->res.p,
-                           (const SeqRef *)P->seqs.p, (const TbItem *)ctx->tbitems.p, ng, (uint8_t *)ctx->dirs.p,
-                           (const SpRec *)ctx->sps.p, (const int *)ctx->spcount.p, (uint64_t *)ctx->keys.p,
-                           (HitRec *)ctx->hits.p, (char *)ctx->ops.p, (int64_t *)ctx->opsused.p, minimum);
-        HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
-        std::vector<SpRec> hsp((size_t)sp_total);
-        std::vector<HitRec> hh((size_t)sp_total);
-        std::vector<char> hops((size_t)ops_total);
-        if (sp_total > 0) {
-            HIPCHK(ctx, hipMemcpyAsync(hsp.data(), ctx->sps.p, (size_t)sp_total * sizeof(SpRec),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(hh.data(), ctx->hits.p, (size_t)sp_total * sizeof(HitRec),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-        }
-        if (ops_total > 0)
-            HIPCHK(ctx, hipMemcpyAsync(hops.data(), ctx->ops.p, (size_t)ops_total, hipMemcpyDeviceToHost,
-                                       ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(opsused.data(), ctx->opsused.p, (size_t)ng * sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        {
-            float t = 0;
-            HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
-            ms_walk += t;
-        }
-        // per gated pair: starting points (emission order), hits (walk order), op characters
-        static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
-        for (int g = 0; g < ng; ++g) {
-            const int64_t sp0 = (int64_t)R->sps.size();
-            const int n = spc[g];
-            for (int k = 0; k < n; ++k) {
-                pw_starting_point sp;
-                std::memcpy(&sp, &hsp[tb[g].sp_off + k], sizeof sp);
-                R->sps.push_back(sp);
-            }
-            const int64_t op0 = (int64_t)R->ops.size();
-            R->ops.insert(R->ops.end(), hops.begin() + tb[g].ops_off, hops.begin() + tb[g].ops_off + opsused[g]);
-            for (int k = 0; k < n; ++k) {
-                const HitRec &hr = hh[tb[g].hit_off + k];
-                pw_hit h;
-                h.query = hr.query;
-                h.target = hr.target;
-                h.shown = hr.shown;
-                h.s_start = hr.s_start;
-                h.s_end = hr.s_end;
-                h.t_start = hr.t_start;
-                h.t_end = hr.t_end;
-                h.matches = hr.matches;
-                h.mismatches = hr.mismatches;
-                h.length = hr.length;
-                h.score = hr.score;
-                h.sp_index = (int32_t)(sp0 + hr.sp_index);
-                h.ops_offset = op0 + hr.ops_offset;
-                R->hits.push_back(h);
-            }
-        }
+        sp0 += o.nsp;
+        op0 += o.nops;
     }
     // the reference keeps at most sp_cap starting points per pair of a batch in
     // total (SmithWaterman.py:683-685); report how many a cap would drop
     int64_t truncated = 0;
     if (sp_cap > 0) {
         const int64_t cap_total = (int64_t)sp_cap * n_pairs;
-        if ((int64_t)R->sps.size() >= cap_total) truncated = (int64_t)R->sps.size() - cap_total;
+        if (nsp >= cap_total) truncated = nsp - cap_total;
     }
+    R->pub.sp_truncated = truncated;
+}
+
+// Launch parameters a captured graph depends on.
+std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, const Lane &L, const Geometry &g,
+                                      double minimum) {
+    std::vector<uint64_t> s;
+    auto add = [&](uint64_t v) { s.push_back(v); };
+    const DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
+                            &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch};
+    for (const DevBuf *b : bufs) {
+        add((uint64_t)(uintptr_t)b->p);
+        add(b->cap);
+    }
+    add((uint64_t)(uintptr_t)W.h_ctl.p);
+    add((uint64_t)W.dir_cap);
+    add((uint64_t)W.cand_cap);
+    add((uint64_t)W.ops_cap);
+    add((uint64_t)W.cand0);
+    add((uint64_t)(uintptr_t)L.s);
+    add((uint64_t)(uintptr_t)L.s2);
+    uint64_t mb;
+    std::memcpy(&mb, &minimum, sizeof mb);
+    add(mb);
+    for (int w = 0; w < NB_KINDS; ++w) add((uint64_t)g.group_grid[w]);
+    add((uint64_t)g.tile_grid);
+    add((uint64_t)g.dir_grid);
+    add((uint64_t)g.task_grid);
+    (void)P;
+    return s;
+}
+
+int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int32_t sp_cap, bool use_graph,
+            HostResult *R) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    const int64_t n_pairs = P->n_pairs;
+    const Geometry g = geometry(ctx, P);
+    int rc = size_workspace(ctx, P, W, g);
+    if (rc) return rc;
+    int64_t launches = 0;
+    double ms_score = 0, ms_dir = 0, ms_tb = 0, ms_walk = 0, ms_device = 0;
+    if (use_graph) {
+        std::vector<uint64_t> sig = graph_signature(P, W, L, g, minimum);
+        if (!P->graph || sig != P->graph_sig) {
+            if (P->graph) {
+                (void)hipGraphExecDestroy(P->graph);
+                P->graph = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            HIPCHK(ctx, hipStreamBeginCapture(L.s, hipStreamCaptureModeThreadLocal));
+            rc = launch_pipeline(ctx, P, W, L, g, minimum, false, &launches);
+            hipError_t e = hipStreamEndCapture(L.s, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+                return PW_EDEVICE;
+            }
+            e = hipGraphInstantiate(&P->graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                P->graph = nullptr;
+                ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+                return PW_EDEVICE;
+            }
+            P->graph_sig = sig;
+        } else {
+            for (int w = 0; w < NB_KINDS; ++w) launches += P->nb_end[w] > P->nb_begin[w] ? 1 : 0;
+            launches += P->n_tile > 0 ? (P->mode == 0 ? (int64_t)P->batches.size() : 1) : 0;
+        }
+        HIPCHK(ctx, hipEventRecord(L.ev[0], L.s));
+        HIPCHK(ctx, hipGraphLaunch(P->graph, L.s));
+        HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
+    } else {
+        rc = launch_pipeline(ctx, P, W, L, g, minimum, true, &launches);
+        if (rc) return rc;
+    }
+    rc = take_maxima_buffer(ctx, n_pairs, R);
+    if (rc) return rc;
+    if (n_pairs > 0)
+        HIPCHK(ctx, hipMemcpyAsync(R->maxima, W.maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost,
+                                   L.s));
+    HIPCHK(ctx, hipStreamSynchronize(L.s));
+    if (use_graph) {
+        ms_device = elapsed(L.ev[0], L.ev[5]);
+    } else {
+        ms_score = elapsed(L.ev[0], L.ev[1]);
+        ms_dir = elapsed(L.ev[2], L.ev[3]);
+        ms_tb = elapsed(L.ev[3], L.ev[4]);
+        ms_walk = elapsed(L.ev[4], L.ev[5]);
+    }
+    const Ctl c = *(const Ctl *)W.h_ctl.p;
+    const int ng = c.ngated;
+    if (ng < 0 || ng > n_pairs) {
+        ctx->err = "gate count out of range";
+        return PW_EDEVICE;
+    }
+    std::vector<PairOut> outs;
+    std::vector<PairTask> pending;
+    rc = read_tasks(ctx, W, L, ng, outs, pending);
+    if (rc) return rc;
+    TailStore store;
+    if (!pending.empty()) {
+        // the tail rounds reuse the staging buffers: keep copies of what they hold
+        for (PairOut &o : outs) {
+            store.sps.emplace_back(o.sps, o.sps + o.nsp);
+            store.hits.emplace_back(o.hits, o.hits + o.nsp);
+            store.ops.emplace_back(o.ops, o.ops + o.nops);
+            o.sps = store.sps.back().data();
+            o.hits = store.hits.back().data();
+            o.ops = store.ops.back().data();
+        }
+        adapt_workspace(ctx, W, c, pending);
+        rc = tail_rounds(ctx, P, W, L, g, minimum, pending, outs, store, &ms_dir, &ms_tb, &ms_walk);
+        if (rc) return rc;
+    }
+    assemble(outs, sp_cap, n_pairs, R);
     pw_result &O = R->pub;
     O.n_pairs = n_pairs;
     O.maxima = R->maxima;
@@ -972,9 +1351,8 @@ int execute(pw_ctx *ctx, pw_plan *P, double minimum, int32_t sp_cap, HostResult 
     O.n_ops = (int64_t)R->ops.size();
     O.ops = R->ops.data();
     O.n_traceback_pairs = ng;
-    O.sp_truncated = truncated;
     O.cells = P->cells;
-    O.ms_score = ms_score;
+    O.ms_score = use_graph ? ms_device : ms_score;
     O.ms_directions = ms_dir;
     O.ms_traceback = ms_tb;
     O.ms_walk = ms_walk;
@@ -993,7 +1371,16 @@ int ready(pw_ctx *ctx) {
     return PW_OK;
 }
 
+int new_result(HostResult **out) {
+    HostResult *R = new HostResult();
+    std::memset(&R->pub, 0, sizeof(R->pub));
+    *out = R;
+    return PW_OK;
+}
+
 }  // namespace
+
+extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res);
 
 extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
                           const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
@@ -1007,8 +1394,9 @@ extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff,
     }
     *out = nullptr;
     pw_plan *P = new pw_plan();
-    rc = prepare(ctx, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, min_start_score, same_id,
-                 mode);
+    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+                   min_start_score, same_id, mode);
+    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s);
     if (rc) {
         free_plan(P);
         return rc;
@@ -1017,17 +1405,18 @@ extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff,
     return PW_OK;
 }
 
-extern "C" int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out) {
+static int execute_entry(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, bool graph, pw_result **out,
+                         const char *what) {
     int rc = ready(ctx);
     if (rc) return rc;
     if (!plan || !out) {
-        ctx->err = "pw_execute: missing argument";
+        ctx->err = std::string(what) + ": missing argument";
         return PW_EINVAL;
     }
     *out = nullptr;
-    HostResult *R = new HostResult();
-    std::memset(&R->pub, 0, sizeof(R->pub));
-    rc = execute(ctx, plan, minimum, sp_cap, R);
+    HostResult *R = nullptr;
+    new_result(&R);
+    rc = execute(ctx, plan, plan->own, ctx->lane, minimum, sp_cap, graph, R);
     if (rc) {
         pw_free_result(ctx, &R->pub);
         return rc;
@@ -1036,10 +1425,19 @@ extern "C" int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp
     return PW_OK;
 }
 
+extern "C" int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out) {
+    return execute_entry(ctx, plan, minimum, sp_cap, false, out, "pw_execute");
+}
+
+extern "C" int pw_execute_graph(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out) {
+    return execute_entry(ctx, plan, minimum, sp_cap, true, out, "pw_execute_graph");
+}
+
 extern "C" int pw_release_plan(pw_ctx *ctx, pw_plan *plan) {
     if (!ctx || !plan) return PW_EINVAL;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->lane.s);
+    (void)hipStreamSynchronize(ctx->lane.s2);
     free_plan(plan);
     return PW_OK;
 }
@@ -1055,26 +1453,36 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         return PW_EINVAL;
     }
     pw_plan *P = new pw_plan();
-    rc = prepare(ctx, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, nullptr, nullptr, mode);
-    double ms = 0;
-    if (!rc) rc = run_score(ctx, P, &ms, nullptr);
+    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+                   nullptr, nullptr, mode);
+    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s);
+    Workspace &W = ctx->ws;
+    Lane &L = ctx->lane;
     if (!rc) {
-        hipError_t e = hipStreamSynchronize(ctx->stream);
+        const Geometry g = geometry(ctx, P);
+        rc = size_workspace(ctx, P, W, g);
+        if (!rc) {
+            hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(256), 0, L.s, (uint32_t *)W.ctl.p, (int)(sizeof(Ctl) / 4));
+            rc = launch_score(ctx, P, W, L, g, true, nullptr);
+        }
+    }
+    if (!rc && n_pairs > 0) {
+        hipError_t e = hipMemcpyAsync(maxima_out, W.maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost,
+                                      L.s);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipMemcpyAsync maxima: ") + hipGetErrorString(e);
+            rc = PW_EDEVICE;
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(L.s);
         if (e != hipSuccess) {
             ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
             rc = PW_EDEVICE;
         }
     }
-    if (!rc) rc = score_ms(ctx, &ms);
-    if (!rc && n_pairs > 0) {
-        hipError_t e = hipMemcpy(maxima_out, ctx->maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            ctx->err = std::string("hipMemcpy maxima: ") + hipGetErrorString(e);
-            rc = PW_EDEVICE;
-        }
-    }
+    if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
     free_plan(P);
-    if (!rc && ms_out) *ms_out = ms;
     return rc;
 }
 
@@ -1084,15 +1492,22 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
                               const uint8_t *same_id, double minimum, int32_t sp_cap, int32_t mode, pw_result **out) {
     int rc = ready(ctx);
     if (rc) return rc;
-    if (!out || (n_pairs > 0 && !min_start_score)) {
+    if (!out || nq < 0 || nt < 0 || (n_pairs > 0 && (!pq || !pt || !min_start_score))) {
         ctx->err = "pw_align_pairs: missing argument";
         return PW_EINVAL;
     }
-    pw_plan *P = nullptr;
-    rc = pw_prepare(ctx, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs, min_start_score, same_id, mode,
-                    &P);
-    if (rc) return rc;
-    rc = pw_execute(ctx, P, minimum, sp_cap, out);
+    *out = nullptr;
+    pw_plan *P = new pw_plan();
+    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+                   min_start_score, same_id, mode);
+    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s);
+    if (!rc) {
+        HostResult *R = nullptr;
+        new_result(&R);
+        rc = execute(ctx, P, ctx->ws, ctx->lane, minimum, sp_cap, false, R);
+        if (rc) pw_free_result(ctx, &R->pub);
+        else *out = &R->pub;
+    }
     free_plan(P);
     return rc;
 }

@@ -471,25 +471,33 @@ __global__ void __launch_bounds__(64) group_kernel(const uint8_t *__restrict__ r
     }
 }
 
-// Direction pass over pairs that passed the traceback gate.
+// Number of traceback tasks of the current execution (written by the gate
+// kernel or by the host before the kernel that reads it).
+__device__ __forceinline__ int task_count(const Ctl *ctl) {
+    return __hip_atomic_load(&ctl->ngated, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Direction pass over the traceback tasks; persistent workgroups take tasks
+// blockIdx.x, blockIdx.x + gridDim.x, ... up to the device-side task count.
 template <bool AFF>
 __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
-                                                 const DirItem *__restrict__ items, int nitems,
+                                                 PairTask *__restrict__ tasks, const Ctl *__restrict__ ctl,
                                                  const float *__restrict__ tables, int kp, int lkp, float G, float E,
                                                  float *__restrict__ scratch, int64_t scr_stride,
-                                                 uint8_t *__restrict__ dirs, int *__restrict__ cand_counts,
-                                                 Cand *__restrict__ cands) {
+                                                 uint8_t *__restrict__ dirs, Cand *__restrict__ cands) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
     uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
     float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        const DirItem it = items[item];
+    const int n = task_count(ctl);
+    for (int item = blockIdx.x; item < n; item += gridDim.x) {
+        const PairTask it = tasks[item];
+        if (it.status != TASK_OK) continue;
         const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
         uint8_t *d = dirs + it.dir_off;
-        int *cc = cand_counts + item;
+        int *cc = &tasks[item].ncand;
         Cand *cl = cands + it.cand_off;
         switch (it.R) {
         case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
@@ -500,25 +508,82 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
     }
 }
 
-// Traceback gate: pairs whose maximum reaches MINIMUM_SCORE and their minimum
-// start score get the direction pass.  One atomic per wave.
+// Clears n 32-bit words (control blocks), in place of a memset node so that
+// the execution can be captured in a graph.
+__global__ void __launch_bounds__(256) clear_kernel(uint32_t *__restrict__ p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__device__ __host__ __forceinline__ int dir_residues(int qlen) {
+    int r = 1;
+    while (r < 8 && WAVE * r < qlen) r *= 2;
+    return r;
+}
+
+// Bytes of a pair's tile-major direction area (16-byte aligned).
+__device__ __host__ __forceinline__ int64_t dir_area(int qlen, int tlen, int R) {
+    const int rows = WAVE * R;
+    const int64_t stripes = qlen > 0 ? (qlen + rows - 1) / rows : 1;
+    return (stripes * (int64_t)(tlen + WAVE - 1) * WAVE * R + 15) & ~(int64_t)15;
+}
+
+// Traceback gate of default_main_cpu.cl:408-410 (maximum >= MINIMUM_SCORE)
+// plus the pair's minimum start score, without which no cell can start
+// (:427).  Every passing pair gets a PairTask with its direction and
+// candidate areas taken from the bump allocators in ctl; a pair that does
+// not fit is marked TASK_DEFERRED for a later round.  One task-slot atomic
+// per wave.
 __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
-                                                   int64_t n, float minimum, int *__restrict__ count,
-                                                   int *__restrict__ out) {
+                                                   const int2 *__restrict__ pairs,
+                                                   const uint8_t *__restrict__ same_id,
+                                                   const SeqRef *__restrict__ seqs, int nq, int64_t n,
+                                                   float minimum, float ll, Ctl *__restrict__ ctl,
+                                                   PairTask *__restrict__ tasks, int64_t dir_cap,
+                                                   int64_t cand_cap_total, int cand0) {
     const int lane = threadIdx.x & 63;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = base + threadIdx.x;
         bool pass = false;
+        float m = 0.0f;
         if (i < n) {
-            const float m = maxima[i];
+            m = maxima[i];
             pass = m >= minimum && m >= mss[i];
         }
         const uint64_t mask = __ballot(pass);
         if (mask == 0) continue;
         int first = 0;
-        if (lane == 0) first = atomicAdd(count, __popcll(mask));
+        if (lane == 0) first = atomicAdd(&ctl->ngated, __popcll(mask));
         first = __shfl(first, 0, 64);
-        if (pass) out[first + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+        if (!pass) continue;
+        const int slot = first + __popcll(mask & ((1ull << lane) - 1ull));
+        const int2 pr = pairs[i];
+        const int ql = seqs[pr.x].len, tl = seqs[nq + pr.y].len;
+        PairTask t;
+        t.pair = (int32_t)i;
+        t.q_seq = pr.x;
+        t.t_seq = nq + pr.y;
+        t.R = dir_residues(ql);
+        const int64_t area = dir_area(ql, tl, t.R);
+        const unsigned long long doff = atomicAdd(&ctl->dir_used, (unsigned long long)area);
+        const unsigned long long coff = atomicAdd(&ctl->cand_used, (unsigned long long)cand0);
+        t.dir_off = (int64_t)doff;
+        t.cand_off = (int64_t)coff;
+        t.ops_off = 0;
+        t.nops = 0;
+        t.t1 = ll * m;
+        t.mp = mss[i];
+        t.maxv = m;
+        t.cand_cap = cand0;
+        t.ncand = 0;
+        t.nsp = 0;
+        t.same_id = same_id[i];
+        t.status = ((int64_t)doff + area <= dir_cap && (int64_t)coff + cand0 <= cand_cap_total) ? TASK_OK
+                                                                                                : TASK_DEFERRED;
+        t.qidx = pr.x;
+        t.tidx = pr.y;
+        t.out_off = 0;
+        t.pad = 0;
+        tasks[slot] = t;
     }
 }
 
@@ -526,7 +591,7 @@ __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ max
 // Start selection, marking and the walk: one thread per pair.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ int64_t cell_offset(const TbItem &it, int tlen, int x, int y) {
+__device__ __forceinline__ int64_t cell_offset(const PairTask &it, int tlen, int x, int y) {
     const int R = it.R, rows = WAVE * R;
     const int s = x / rows, rem = x - s * rows, k = rem / R, r = rem - k * R;
     const int64_t nsteps = (int64_t)tlen + WAVE - 1;
@@ -574,65 +639,71 @@ __device__ void heap_sort(T *a, int n, Less less) {
 // because they would need an existing I/J mark); a marked candidate becomes
 // a starting point only if its sign-marked value passes the tests.
 template <bool AFF>
-__global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs, const TbItem *__restrict__ items,
-                                                int nitems, uint8_t *__restrict__ dirs, Cand *__restrict__ cands,
-                                                SpRec *__restrict__ sps, int *__restrict__ sp_counts) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nitems) return;
-    const TbItem it = items[g];
-    const int tlen = seqs[it.t_seq].len;
-    Cand *c = cands + it.cand_off;
-    const int n = it.ncand;
-    heap_sort(c, n, [](const Cand &a, const Cand &b) { return order_key(a.x, a.y) > order_key(b.x, b.y); });
-    SpRec *out = sps + it.sp_off;
-    int nsp = 0;
-    for (int i = 0; i < n; ++i) {
-        int x = c[i].x, y = c[i].y;
-        const float v = c[i].v;
-        const int64_t off = cell_offset(it, tlen, x, y);
-        const uint8_t b = dirs[off];
-        float score;
-        bool emit = false;
-        if (!(b & DB_MARK)) {
-            if (b & DB_CAND) {
-                emit = true;
-                score = v;
-                // mark the path
-                int cx = x, cy = y;
-                for (;;) {
-                    const int64_t o = cell_offset(it, tlen, cx, cy);
-                    const uint8_t cb = dirs[o];
-                    dirs[o] = cb | DB_MARK;
-                    const uint32_t code = cb & 3u;
-                    if (cb & DB_STOP) break;
-                    if (code == 0u) break;
-                    if (AFF && code != 1u) break;
-                    const int px = cx - (code != 2u ? 1 : 0), py = cy - (code != 3u ? 1 : 0);
-                    if (px < 0 || py < 0) break;
-                    if (dirs[cell_offset(it, tlen, px, py)] & DB_MARK) break;
-                    cx = px;
-                    cy = py;
+__global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs, PairTask *__restrict__ tasks,
+                                                const Ctl *__restrict__ ctl, uint8_t *__restrict__ dirs,
+                                                Cand *__restrict__ cands, SpRec *__restrict__ sps) {
+    const int n = task_count(ctl);
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+        const PairTask it = tasks[g];
+        if (it.status != TASK_OK) continue;
+        if (it.ncand > it.cand_cap) {
+            tasks[g].status = TASK_CAND_OVERFLOW;
+            continue;
+        }
+        const int tlen = seqs[it.t_seq].len;
+        Cand *c = cands + it.cand_off;
+        const int nc = it.ncand;
+        heap_sort(c, nc, [](const Cand &a, const Cand &b) { return order_key(a.x, a.y) > order_key(b.x, b.y); });
+        SpRec *out = sps + it.cand_off;
+        int nsp = 0;
+        for (int i = 0; i < nc; ++i) {
+            int x = c[i].x, y = c[i].y;
+            const float v = c[i].v;
+            const int64_t off = cell_offset(it, tlen, x, y);
+            const uint8_t b = dirs[off];
+            float score;
+            bool emit = false;
+            if (!(b & DB_MARK)) {
+                if (b & DB_CAND) {
+                    emit = true;
+                    score = v;
+                    // mark the path
+                    int cx = x, cy = y;
+                    for (;;) {
+                        const int64_t o = cell_offset(it, tlen, cx, cy);
+                        const uint8_t cb = dirs[o];
+                        dirs[o] = cb | DB_MARK;
+                        const uint32_t code = cb & 3u;
+                        if (cb & DB_STOP) break;
+                        if (code == 0u) break;
+                        if (AFF && code != 1u) break;
+                        const int px = cx - (code != 2u ? 1 : 0), py = cy - (code != 3u ? 1 : 0);
+                        if (px < 0 || py < 0) break;
+                        if (dirs[cell_offset(it, tlen, px, py)] & DB_MARK) break;
+                        cx = px;
+                        cy = py;
+                    }
                 }
+            } else if (b & DB_CANDM) {
+                emit = true;
+                score = -v;
             }
-        } else if (b & DB_CANDM) {
-            emit = true;
-            score = -v;
+            if (emit) {
+                SpRec r;
+                r.sequence = (uint32_t)it.qidx;
+                r.target = (uint32_t)it.tidx;
+                r.blockX = (uint32_t)x >> 3;
+                r.blockY = (uint32_t)y >> 3;
+                r.valueX = (uint32_t)x & 7;
+                r.valueY = (uint32_t)y & 7;
+                r.score = score;
+                r.maxScore = it.maxv;
+                r.posScore = it.mp;
+                out[nsp++] = r;
+            }
         }
-        if (emit) {
-            SpRec r;
-            r.sequence = (uint32_t)it.qidx;
-            r.target = (uint32_t)it.tidx;
-            r.blockX = (uint32_t)x >> 3;
-            r.blockY = (uint32_t)y >> 3;
-            r.valueX = (uint32_t)x & 7;
-            r.valueY = (uint32_t)y & 7;
-            r.score = score;
-            r.maxScore = it.maxv;
-            r.posScore = it.mp;
-            out[nsp++] = r;
-        }
+        tasks[g].nsp = nsp;
     }
-    sp_counts[g] = nsp;
 }
 
 __device__ __forceinline__ uint32_t float_order(float f) {
@@ -652,91 +723,113 @@ __device__ __forceinline__ int eff_dir(uint8_t b) {
     return code;
 }
 
+// The walk of SmithWaterman.py:657-848 per pair: starting points in
+// ascending score order (stable), each walked until STOP / NONE / an
+// IN_ALIGNMENT cell.  The pair's op characters go to a range taken from the
+// ops bump allocator (capacity: every walk at most |q|+|t| columns and all
+// walks together at most |q||t| cells plus one column per walk); its starting
+// points (emission order) and hits (walk order) are compacted into out_sps /
+// out_hits at a range taken from the output allocator.
 template <bool AFF>
 __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
-                                                  const TbItem *__restrict__ items, int nitems,
+                                                  PairTask *__restrict__ tasks, Ctl *__restrict__ ctl,
                                                   uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
-                                                  const int *__restrict__ sp_counts, uint64_t *__restrict__ keys,
-                                                  HitRec *__restrict__ hits, char *__restrict__ ops,
-                                                  int64_t *__restrict__ ops_used, double minimum) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nitems) return;
-    const TbItem it = items[g];
-    const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
-    const int nsp = sp_counts[g];
-    const SpRec *sp = sps + it.sp_off;
-    uint64_t *key = keys + it.key_off;
-    for (int i = 0; i < nsp; ++i) key[i] = ((uint64_t)float_order(sp[i].score) << 32) | (uint32_t)i;
-    // ascending score, stable (SmithWaterman.py:699)
-    heap_sort(key, nsp, [](const uint64_t &a, const uint64_t &b) { return a < b; });
-    char *op = ops + it.ops_off;
-    int64_t nops = 0;
-    HitRec *h = hits + it.hit_off;
-    for (int k = 0; k < nsp; ++k) {
-        const int i = (int)(key[k] & 0xffffffffu);
-        const SpRec r = sp[i];
-        int x = (int)(r.blockX * 8 + r.valueX), y = (int)(r.blockY * 8 + r.valueY);
-        const int s_end = x, t_end = y;
-        int s_start = s_end + 1, t_start = t_end + 1;
-        int matches = 0, mismatches = 0, len = 0;
-        int direction = eff_dir<AFF>(dirs[cell_offset(it, t.len, x, y)]);
-        bool show = true;
-        if (it.same_id) {
-            direction = 64;
-            show = false;
+                                                  uint64_t *__restrict__ keys, SpRec *__restrict__ out_sps,
+                                                  HitRec *__restrict__ out_hits, char *__restrict__ ops,
+                                                  int64_t ops_cap, double minimum) {
+    const int n = task_count(ctl);
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+        const PairTask it = tasks[g];
+        if (it.status != TASK_OK) continue;
+        const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
+        const int nsp = it.nsp;
+        const int64_t ql = q.len, tl = t.len;
+        const int64_t cap = nsp > 0 ? min((int64_t)nsp * (ql + tl), ql * tl + nsp) : 0;
+        const unsigned long long ooff = cap > 0 ? atomicAdd(&ctl->ops_used, (unsigned long long)cap) : 0ull;
+        if ((int64_t)ooff + cap > ops_cap) {
+            tasks[g].status = TASK_OPS_OVERFLOW;
+            continue;
         }
-        if ((double)r.score < minimum) show = false;
-        const int64_t start_ops = nops;
-        bool term = false;
-        while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
-            const int64_t o = cell_offset(it, t.len, x, y);
-            const uint8_t b = dirs[o];
-            direction = eff_dir<AFF>(b);
-            dirs[o] = b | DB_WALK;
-            if (direction == 96) {
+        const unsigned long long out0 = nsp > 0 ? atomicAdd(&ctl->out_used, (unsigned long long)nsp) : 0ull;
+        const SpRec *sp = sps + it.cand_off;
+        uint64_t *key = keys + it.cand_off;
+        for (int i = 0; i < nsp; ++i) {
+            key[i] = ((uint64_t)float_order(sp[i].score) << 32) | (uint32_t)i;
+            out_sps[out0 + i] = sp[i];
+        }
+        // ascending score, stable (SmithWaterman.py:699)
+        heap_sort(key, nsp, [](const uint64_t &a, const uint64_t &b) { return a < b; });
+        char *op = ops + ooff;
+        int64_t nops = 0;
+        HitRec *h = out_hits + out0;
+        for (int k = 0; k < nsp; ++k) {
+            const int i = (int)(key[k] & 0xffffffffu);
+            const SpRec r = sp[i];
+            int x = (int)(r.blockX * 8 + r.valueX), y = (int)(r.blockY * 8 + r.valueY);
+            const int s_end = x, t_end = y;
+            int s_start = s_end + 1, t_start = t_end + 1;
+            int matches = 0, mismatches = 0, len = 0;
+            int direction = eff_dir<AFF>(dirs[cell_offset(it, t.len, x, y)]);
+            bool show = true;
+            if (it.same_id) {
+                direction = 64;
                 show = false;
-            } else if (direction == 1 || direction == 64) {
-                if (res[q.off + x] == res[t.off + y]) ++matches; else ++mismatches;
-                op[nops++] = 'D';
-                ++len;
-                --s_start;
-                --t_start;
-                if (direction == 1) {
-                    --x;
-                    --y;
-                }
-            } else if (direction == 3) {
-                op[nops++] = 'L';
-                ++len;
-                --s_start;
-                --x;
-            } else if (direction == 2) {
-                op[nops++] = 'U';
-                ++len;
-                --t_start;
-                --y;
-            } else {
-                term = true;
             }
+            if ((double)r.score < minimum) show = false;
+            const int64_t start_ops = nops;
+            bool term = false;
+            while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
+                const int64_t o = cell_offset(it, t.len, x, y);
+                const uint8_t b = dirs[o];
+                direction = eff_dir<AFF>(b);
+                dirs[o] = b | DB_WALK;
+                if (direction == 96) {
+                    show = false;
+                } else if (direction == 1 || direction == 64) {
+                    if (res[q.off + x] == res[t.off + y]) ++matches; else ++mismatches;
+                    op[nops++] = 'D';
+                    ++len;
+                    --s_start;
+                    --t_start;
+                    if (direction == 1) {
+                        --x;
+                        --y;
+                    }
+                } else if (direction == 3) {
+                    op[nops++] = 'L';
+                    ++len;
+                    --s_start;
+                    --x;
+                } else if (direction == 2) {
+                    op[nops++] = 'U';
+                    ++len;
+                    --t_start;
+                    --y;
+                } else {
+                    term = true;
+                }
+            }
+            if (!show) nops = start_ops;
+            HitRec hr;
+            hr.query = it.qidx;
+            hr.target = it.tidx;
+            hr.shown = show ? 1 : 0;
+            hr.s_start = s_start;
+            hr.s_end = s_end;
+            hr.t_start = t_start;
+            hr.t_end = t_end;
+            hr.matches = matches;
+            hr.mismatches = mismatches;
+            hr.length = show ? len : 0;
+            hr.score = r.score;
+            hr.sp_index = i;
+            hr.ops_offset = start_ops;
+            h[k] = hr;
         }
-        if (!show) nops = start_ops;
-        HitRec hr;
-        hr.query = it.qidx;
-        hr.target = it.tidx;
-        hr.shown = show ? 1 : 0;
-        hr.s_start = s_start;
-        hr.s_end = s_end;
-        hr.t_start = t_start;
-        hr.t_end = t_end;
-        hr.matches = matches;
-        hr.mismatches = mismatches;
-        hr.length = show ? len : 0;
-        hr.score = r.score;
-        hr.sp_index = i;
-        hr.ops_offset = start_ops;
-        h[k] = hr;
+        tasks[g].ops_off = (int64_t)ooff;
+        tasks[g].nops = nops;
+        tasks[g].out_off = (int32_t)out0;
     }
-    ops_used[g] = nops;
 }
 
 }  // namespace pw

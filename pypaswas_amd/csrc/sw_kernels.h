@@ -48,37 +48,51 @@ struct GroupEntry {
     int32_t pair;
 };
 
-// One pair of the direction pass (query along lanes, target swept).
-struct DirItem {
+// One pair of the traceback stages (direction pass, start selection, walk),
+// written by the gate kernel for every pair that passes the traceback gate
+// (or by the host for a tail round).  Candidates, the pair's starting points
+// and the walk's sort keys share the index range [cand_off, cand_off + cand_cap).
+struct PairTask {
+    int32_t pair;        // pair index
     int32_t q_seq, t_seq;
-    int32_t pair;    // pair index
-    int32_t R;
-    int64_t dir_off; // byte offset of the pair's tile-major direction area
-    float t1;        // LOWER_LIMIT_SCORE * maximum (f32 product)
-    float mp;        // minimum start score of the pair
+    int32_t R;           // query residues per lane of the direction pass
+    int64_t dir_off;     // byte offset of the pair's tile-major direction area
     int64_t cand_off;
+    int64_t ops_off;     // into the ops area (set by the walk)
+    int64_t nops;        // op characters written by the walk
+    float t1;            // LOWER_LIMIT_SCORE * maximum (f32 product)
+    float mp;            // minimum start score of the pair
+    float maxv;          // pair maximum
     int32_t cand_cap;
+    int32_t ncand;       // candidate cells found (may exceed cand_cap: overflow)
+    int32_t nsp;         // starting points emitted
+    int32_t same_id;
+    int32_t status;      // TASK_*
+    int32_t qidx, tidx;  // indices reported in the records
+    int32_t out_off;     // first compacted starting point / hit of the pair
     int32_t pad;
+};
+
+// status of a PairTask
+constexpr int32_t TASK_OK = 0;
+constexpr int32_t TASK_DEFERRED = 1;       // no room in the direction or candidate area
+constexpr int32_t TASK_CAND_OVERFLOW = 2;  // more candidates than cand_cap
+constexpr int32_t TASK_OPS_OVERFLOW = 3;   // no room in the ops area
+
+// Device control block of one execution: work counters of the scoring pass
+// (one 64-byte line each) and the bump allocators of the traceback stages.
+constexpr int NB_KINDS = 6;
+struct Ctl {
+    unsigned queue[16 * (1 + NB_KINDS)];
+    int ngated;
+    int pad0;
+    unsigned long long dir_used, cand_used, ops_used, out_used;
+    unsigned long long pad1[3];
 };
 
 struct Cand {
     int32_t x, y;
     float v;
-};
-
-// Per-pair traceback / walk bookkeeping.
-struct TbItem {
-    int32_t pair, q_seq, t_seq, R;
-    int64_t dir_off;
-    int64_t cand_off;
-    int64_t sp_off;      // into the starting-point area (capacity = candidate count)
-    int64_t ops_off;     // into the ops area
-    int64_t hit_off;     // into the hit area (capacity = starting points)
-    int64_t key_off;     // scratch keys for the walk order
-    float maxv, mp;
-    int32_t qidx, tidx;  // indices reported in the records
-    int32_t same_id;
-    int32_t ncand;
 };
 
 struct SpRec {
