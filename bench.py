@@ -65,6 +65,12 @@ def main():
     ap.add_argument("--cpu-reads", type=int, default=800, help="reads in the CPU-baseline sample")
     ap.add_argument("--no-protein", action="store_true", help="skip the uneven-length protein comparison")
     ap.add_argument("--protein", default="100x400", help="queries x targets of the protein sample")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--serving-jobs", type=int, default=3000, help="requests in the serving run")
+    ap.add_argument("--serving-qps", type=float, default=2000.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serving-lanes", type=int, default=4, help="stream lanes of the server")
+    ap.add_argument("--serving-arena-mib", type=int, default=16384, help="direction arena of the server")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -72,12 +78,18 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
-              "form (the sharding plan is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build targets a single MI355X and has no multi-GPU form; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from pypaswas_amd._native import Device
     from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
+
+    if args.serving_only:
+        dev = Device(0)
+        print(json.dumps(serving_run(dev, args.serving_jobs, args.serving_qps, args.serving_lanes,
+                                     args.serving_arena_mib)), flush=True)
+        return
 
     t0 = time.time()
     q, t, settings, score = build_job(args.workload)
@@ -142,6 +154,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(q, t, settings, score, dev, args.cpu_reads)
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_run(dev, args.serving_jobs, args.serving_qps, args.serving_lanes, args.serving_arena_mib)
+
     protein = None
     if not args.no_protein:  # last: it changes the device's scoring constants
         nq, nt = (int(v) for v in args.protein.split("x"))
@@ -161,6 +177,9 @@ def main():
         "plain_launch": plain,
         "batched_over_plain": round(gcups / plain["gcups"], 2) if plain else None,
         "protein_uneven": protein,
+        "latency_ms": ({"p50": serving["p50_ms"], "p99": serving["p99_ms"], "workload": "serving mix"}
+                       if serving else None),
+        "serving": serving,
         "traceback_pairs": last.n_traceback_pairs,
         "hits": int(last.hits["shown"].sum()) if len(last.hits) else 0,
         "stage_ms": {"score": round(last.ms_score, 3), "directions": round(last.ms_directions, 3),
@@ -168,6 +187,53 @@ def main():
                      "call_wall": round(last.ms_total, 3)},
     }
     print(json.dumps(out), flush=True)
+
+
+def serving_run(dev, n_jobs, qps, lanes, arena_mib, seed=5):
+    """Serving mix through the continuous-batching server at a fixed Poisson
+    rate: 70% reads (100-300 nt) x 8-adapter panels (+RC), 25% protein pairs
+    (100-2,000, BLOSUM62 -11/-1), 5% long DNA pairs (5,000-20,000 nt, +RC).
+    Latency = completion poll - arrival.  A short untimed burst first sizes
+    the lanes' work areas."""
+    from pypaswas_amd import workloads as W
+    from pypaswas_amd.serving import Job, Server, poisson_arrivals, replay
+    t0 = time.time()
+    schemes = W.serving_schemes()
+    raw = W.serving_jobs(seed, n_jobs)
+    jobs = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t) for k, q, t in raw]
+    warm = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t)
+            for k, q, t in W.serving_jobs(seed + 1000, 64)]
+    log("serving: %d jobs built in %.1fs" % (n_jobs, time.time() - t0))
+    srv = Server(dev, lanes=lanes, direction_bytes=int(arena_mib) << 20)
+    try:
+        for i, sc in enumerate(schemes):
+            srv.set_scheme(i, sc)
+        replay(srv, warm)
+        st0 = srv.stats()
+        arrivals = poisson_arrivals(n_jobs, qps, seed)
+        results, lat, wall = replay(srv, jobs, arrivals)
+        st = srv.stats()
+    finally:
+        srv.close()
+    cells = float(sum(j.cells for j in jobs))
+    kinds = np.array([k for k, _, _ in raw])
+    out = {"jobs": n_jobs, "qps": qps, "lanes": lanes, "arena_mib": arena_mib, "wall_s": round(wall, 3),
+           "gcups": round(cells / wall / 1e9, 2), "cells": cells,
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "max_ms": round(float(lat.max()) * 1e3, 3),
+           "batches": st["batches"] - st0["batches"],
+           "jobs_per_batch": round(n_jobs / max(1, st["batches"] - st0["batches"]), 2),
+           "peak_arena_mib": round(st["peak_arena_bytes"] / 2 ** 20, 1),
+           "hits": int(sum(int(r.hits["shown"].sum()) if len(r.hits) else 0 for r in results))}
+    for k in W.SERVING_KINDS:
+        m = kinds == k
+        if m.any():
+            out["p50_ms_" + k] = round(float(np.percentile(lat[m], 50)) * 1e3, 3)
+            out["p99_ms_" + k] = round(float(np.percentile(lat[m], 99)) * 1e3, 3)
+    log("serving: %d jobs at %.0f/s in %.2fs, %.1f GCUPS, p50 %.2f ms, p99 %.2f ms, %d batches" %
+        (n_jobs, qps, wall, out["gcups"], out["p50_ms"], out["p99_ms"], out["batches"]))
+    return out
 
 
 def copy_calibration(mib):

@@ -150,6 +150,48 @@ int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const 
                    const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
                    double *ms_out);
 
+/* ------------------------------------------------------------------------
+ * Serving: a request loop with continuous batching.  pyPaSWAS has no request
+ * loop; a job here is the work of one SmithWaterman.align_sequences pass
+ * (pyPaSWAS/Core/SmithWaterman.py:441-560) with the arguments of
+ * pw_align_pairs.  The caller's thread drives the loop through pw_server_poll;
+ * the server runs batches on `lanes` stream pairs of its own, concurrently on
+ * the device.  Whenever a lane is idle, the queue head and the later queued
+ * jobs with the same scheme and minimum score are merged into one batch while
+ * the direction bytes they could need (every pair passing the traceback gate)
+ * fit in the largest free range of the direction arena of `direction_bytes`.
+ * A job's records equal those of the same job run alone through
+ * pw_align_pairs (pairs are independent), with the job's own sequence indices.
+ * ------------------------------------------------------------------------ */
+typedef struct pw_server pw_server;
+
+int pw_server_create(pw_ctx *ctx, int32_t lanes, int64_t direction_bytes, pw_server **out);
+
+/* Scoring scheme `scheme` (0..7) of later jobs; same arguments as pw_set_scoring. */
+int pw_server_set_scoring(pw_server *srv, int32_t scheme, const float *subst, int dim, int char_offset, float gap,
+                          float gap_ext, float lower_limit, float minimum_score);
+
+/* Queues a job (inputs are copied); *job_id identifies it in poll and take. */
+int pw_server_submit(pw_server *srv, int32_t scheme, const uint8_t *qres, const int64_t *qoff,
+                     const int32_t *qlen, int32_t nq, const uint8_t *tres, const int64_t *toff,
+                     const int32_t *tlen, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs,
+                     const float *min_start_score, const uint8_t *same_id, double minimum, int32_t sp_cap,
+                     int64_t *job_id);
+
+/* Finishes batches whose device work ended, admits queued jobs onto idle
+ * lanes and returns up to max_ids ids of completed jobs.  block != 0 waits
+ * until at least one job has completed (or nothing is queued or running). */
+int pw_server_poll(pw_server *srv, int32_t block, int64_t *done_ids, int32_t max_ids, int32_t *n_done);
+
+/* The result of a completed job, owned by the caller until pw_free_result. */
+int pw_server_take(pw_server *srv, int64_t job_id, pw_result **out);
+
+/* stats[0..6]: batches launched, jobs admitted, jobs completed, peak arena
+ * bytes reserved, jobs queued, jobs running, arena bytes. */
+int pw_server_stats(pw_server *srv, int64_t *stats, int32_t n);
+
+int pw_server_destroy(pw_server *srv);
+
 /* Free a result of pw_align_pairs. */
 int pw_free_result(pw_ctx *ctx, pw_result *res);
 

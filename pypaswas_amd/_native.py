@@ -25,8 +25,9 @@ PW_EDEVICE = -3
 
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
-           "pw_execute", "pw_release_plan", "pw_execute_graph", "pw_set_limits", "pw_score_pairs", "pw_free_result",
-           "pw_last_error")
+           "pw_execute", "pw_release_plan", "pw_execute_graph", "pw_set_limits", "pw_score_pairs",
+           "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
+           "pw_server_stats", "pw_server_destroy", "pw_free_result", "pw_last_error")
 
 
 class StartingPointRec(ctypes.Structure):
@@ -92,6 +93,17 @@ def load() -> ctypes.CDLL:
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
+        L.pw_server_create.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(vp)]
+        L.pw_server_set_scoring.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                            ctypes.c_float]
+        L.pw_server_submit.argtypes = [vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32,
+                                       vp, vp, ctypes.c_int64, vp, vp, ctypes.c_double, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_int64)]
+        L.pw_server_poll.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.pw_server_take.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.POINTER(ResultRec))]
+        L.pw_server_stats.argtypes = [vp, vp, ctypes.c_int32]
+        L.pw_server_destroy.argtypes = [vp]
         L.pw_last_error.argtypes = [vp]
         L.pw_last_error.restype = ctypes.c_char_p
         for name in EXPORTS:

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libpypaswas_hip.so")
 SOURCES = [os.path.join(HERE, "sw_capi.hip")]
-DEPENDS = SOURCES + [os.path.join(HERE, "sw_device.hip"), os.path.join(HERE, "sw_kernels.h"),
+DEPENDS = SOURCES + [os.path.join(HERE, "sw_device.hip"), os.path.join(HERE, "sw_server.hip"), os.path.join(HERE, "sw_kernels.h"),
                      os.path.join(os.path.dirname(PKG), "include", "pypaswas_hip.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -77,7 +77,12 @@ struct Workspace {
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
     PinBuf h_ctl, h_tasks, h_sps, h_hits, h_ops;
+    // direction area lent by a caller (a range of a serving arena) instead of dirs
+    uint8_t *dir_ext = nullptr;
+    int64_t dir_ext_cap = 0;
 };
+
+uint8_t *dir_ptr(const Workspace &W) { return W.dir_ext ? W.dir_ext : (uint8_t *)W.dirs.p; }
 
 }  // namespace
 
@@ -244,9 +249,13 @@ void free_workspace(Workspace &W) {
     for (PinBuf *b : pins) free_pinned(*b);
 }
 
-int create_lane(Lane &L, std::string &err) {
+// single: one stream for everything (s2 == s), so that a lane occupies one
+// hardware queue; the tile kernels of a batch then follow its lane-per-pair
+// kernels instead of overlapping them.
+int create_lane(Lane &L, std::string &err, bool single = false) {
     hipError_t e = hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.s2, hipStreamNonBlocking);
+    if (single) L.s2 = L.s;
+    else if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.s2, hipStreamNonBlocking);
     for (auto &ev : L.ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fork, hipEventDisableTiming);
@@ -260,13 +269,13 @@ int create_lane(Lane &L, std::string &err) {
 
 void destroy_lane(Lane &L) {
     if (L.s) (void)hipStreamSynchronize(L.s);
-    if (L.s2) (void)hipStreamSynchronize(L.s2);
+    if (L.s2 && L.s2 != L.s) (void)hipStreamSynchronize(L.s2);
     for (auto &ev : L.ev)
         if (ev) (void)hipEventDestroy(ev);
     if (L.fork) (void)hipEventDestroy(L.fork);
     if (L.join) (void)hipEventDestroy(L.join);
+    if (L.s2 && L.s2 != L.s) (void)hipStreamDestroy(L.s2);
     if (L.s) (void)hipStreamDestroy(L.s);
-    if (L.s2) (void)hipStreamDestroy(L.s2);
     L = Lane();
 }
 
@@ -816,9 +825,13 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     ENSURE(ctx, W.ctl, sizeof(Ctl));
     ENSURE(ctx, W.maxima, (size_t)n * sizeof(float));
     ENSURE(ctx, W.tasks, (size_t)n * sizeof(PairTask));
-    const int64_t dcap = std::max<int64_t>(16, std::min(P->dir_worst, std::min(W.want_dir, dir_limit(ctx))));
-    ENSURE(ctx, W.dirs, (size_t)dcap);
-    W.dir_cap = (int64_t)W.dirs.cap;
+    if (W.dir_ext) {
+        W.dir_cap = W.dir_ext_cap;
+    } else {
+        const int64_t dcap = std::max<int64_t>(16, std::min(P->dir_worst, std::min(W.want_dir, dir_limit(ctx))));
+        ENSURE(ctx, W.dirs, (size_t)dcap);
+        W.dir_cap = (int64_t)W.dirs.cap;
+    }
     const int64_t ccap = std::max<int64_t>(16, std::min<int64_t>(n * (int64_t)W.cand0, W.want_cand));
     ENSURE(ctx, W.cands, (size_t)ccap * sizeof(Cand));
     ENSURE(ctx, W.spslot, (size_t)ccap * sizeof(SpRec));
@@ -913,19 +926,23 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
     PairTask *tasks = (PairTask *)W.tasks.p;
     const uint8_t *res = (const uint8_t *)P->res.p;
     const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+    if (!dir_ptr(W) || !W.cands.p || !W.tasks.p || !W.ops.p) {
+        ctx->err = "traceback work areas are not allocated";
+        return PW_EDEVICE;
+    }
     auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
     hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks, (const Ctl *)ctl,
                        (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.dscratch.p,
-                       g.dscr_stride, (uint8_t *)W.dirs.p, (Cand *)W.cands.p);
+                       g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p);
     HIPCHK(ctx, hipGetLastError());
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
     auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
-    hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)ctl, (uint8_t *)W.dirs.p,
+    hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)ctl, dir_ptr(W),
                        (Cand *)W.cands.p, (SpRec *)W.spslot.p);
     HIPCHK(ctx, hipGetLastError());
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
     auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
-    hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, (uint8_t *)W.dirs.p,
+    hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                        (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
                        (char *)W.ops.p, W.ops_cap, minimum);
     HIPCHK(ctx, hipGetLastError());
@@ -960,6 +977,7 @@ struct HostResult {
     pw_result pub;
     float *maxima = nullptr;  // pinned host memory of maxima_cap floats
     size_t maxima_cap = 0;
+    std::vector<float> maxima_vec;  // maxima of a served job (when maxima is NULL)
     std::vector<pw_starting_point> sps;
     std::vector<pw_hit> hits;
     std::vector<char> ops;
@@ -1060,8 +1078,16 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         // areas large enough for the first pair; then take pairs while they fit
         const PairTask &f = pending.front();
         const int64_t farea = dir_area(P->qlen[P->pq[f.pair]], P->tlen[P->pt[f.pair]], f.R);
-        ENSURE(ctx, W.dirs, (size_t)std::max<int64_t>(farea, W.dir_cap));
-        W.dir_cap = (int64_t)W.dirs.cap;
+        if (W.dir_ext && farea > W.dir_cap) {
+            // a lent area too small for this pair: continue in an own one
+            W.dir_ext = nullptr;
+            W.dir_ext_cap = 0;
+            W.dir_cap = 0;
+        }
+        if (!W.dir_ext) {
+            ENSURE(ctx, W.dirs, (size_t)std::max<int64_t>(farea, W.dir_cap));
+            W.dir_cap = (int64_t)W.dirs.cap;
+        }
         if (f.cand_cap > W.cand_cap) {
             const size_t c = (size_t)f.cand_cap;
             ENSURE(ctx, W.cands, c * sizeof(Cand));
@@ -1108,16 +1134,20 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         PairTask *tasks = (PairTask *)W.tasks.p;
         const uint8_t *res = (const uint8_t *)P->res.p;
         const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+        if (!dir_ptr(W)) {
+            ctx->err = "direction area is not allocated";
+            return PW_EDEVICE;
+        }
         HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
         auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
         hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks,
                            (const Ctl *)dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
-                           (float *)W.dscratch.p, g.dscr_stride, (uint8_t *)W.dirs.p, (Cand *)W.cands.p);
+                           (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
         auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
         hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)dctl,
-                           (uint8_t *)W.dirs.p, (Cand *)W.cands.p, (SpRec *)W.spslot.p);
+                           dir_ptr(W), (Cand *)W.cands.p, (SpRec *)W.spslot.p);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
         // exact ops capacity from the starting-point counts
@@ -1139,7 +1169,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         }
         HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
         auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
-        hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, dctl, (uint8_t *)W.dirs.p,
+        hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, dctl, dir_ptr(W),
                            (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
                            (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum);
         HIPCHK(ctx, hipGetLastError());
@@ -1237,6 +1267,7 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
         add(b->cap);
     }
     add((uint64_t)(uintptr_t)W.h_ctl.p);
+    add((uint64_t)(uintptr_t)dir_ptr(W));
     add((uint64_t)W.dir_cap);
     add((uint64_t)W.cand_cap);
     add((uint64_t)W.ops_cap);
@@ -1254,15 +1285,36 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     return s;
 }
 
-int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int32_t sp_cap, bool use_graph,
-            HostResult *R) {
-    const auto wall0 = std::chrono::steady_clock::now();
-    const int64_t n_pairs = P->n_pairs;
-    const Geometry g = geometry(ctx, P);
+// State of one execution between its launch and its read-back.
+struct Exec {
+    Geometry g;
+    int64_t launches = 0;
+    bool graph = false;
+    double minimum = 0;
+    std::chrono::steady_clock::time_point wall0;
+};
+
+// Results of a finished execution before they are split into host records.
+struct Finished {
+    std::vector<PairOut> outs;  // sorted by pair
+    TailStore store;
+    int ng = 0;
+    double ms_score = 0, ms_dir = 0, ms_tb = 0, ms_walk = 0;
+};
+
+// Sizes the work areas and issues the device work of one execution on L, the
+// pair maxima going to maxima_dst (pinned host memory, n_pairs floats).  Nothing
+// waits for the device.
+int execute_launch(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, bool use_graph,
+                   float *maxima_dst, Exec &X) {
+    X.wall0 = std::chrono::steady_clock::now();
+    X.graph = use_graph;
+    X.minimum = minimum;
+    X.launches = 0;
+    X.g = geometry(ctx, P);
+    const Geometry &g = X.g;
     int rc = size_workspace(ctx, P, W, g);
     if (rc) return rc;
-    int64_t launches = 0;
-    double ms_score = 0, ms_dir = 0, ms_tb = 0, ms_walk = 0, ms_device = 0;
     if (use_graph) {
         std::vector<uint64_t> sig = graph_signature(P, W, L, g, minimum);
         if (!P->graph || sig != P->graph_sig) {
@@ -1272,7 +1324,7 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
             }
             hipGraph_t graph = nullptr;
             HIPCHK(ctx, hipStreamBeginCapture(L.s, hipStreamCaptureModeThreadLocal));
-            rc = launch_pipeline(ctx, P, W, L, g, minimum, false, &launches);
+            rc = launch_pipeline(ctx, P, W, L, g, minimum, false, &X.launches);
             hipError_t e = hipStreamEndCapture(L.s, &graph);
             if (rc) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1291,59 +1343,66 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
             }
             P->graph_sig = sig;
         } else {
-            for (int w = 0; w < NB_KINDS; ++w) launches += P->nb_end[w] > P->nb_begin[w] ? 1 : 0;
-            launches += P->n_tile > 0 ? (P->mode == 0 ? (int64_t)P->batches.size() : 1) : 0;
+            for (int w = 0; w < NB_KINDS; ++w) X.launches += P->nb_end[w] > P->nb_begin[w] ? 1 : 0;
+            X.launches += P->n_tile > 0 ? (P->mode == 0 ? (int64_t)P->batches.size() : 1) : 0;
         }
         HIPCHK(ctx, hipEventRecord(L.ev[0], L.s));
         HIPCHK(ctx, hipGraphLaunch(P->graph, L.s));
         HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
     } else {
-        rc = launch_pipeline(ctx, P, W, L, g, minimum, true, &launches);
+        rc = launch_pipeline(ctx, P, W, L, g, minimum, true, &X.launches);
         if (rc) return rc;
     }
-    rc = take_maxima_buffer(ctx, n_pairs, R);
-    if (rc) return rc;
-    if (n_pairs > 0)
-        HIPCHK(ctx, hipMemcpyAsync(R->maxima, W.maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost,
-                                   L.s));
+    if (P->n_pairs > 0)
+        HIPCHK(ctx, hipMemcpyAsync(maxima_dst, W.maxima.p, (size_t)P->n_pairs * sizeof(float),
+                                   hipMemcpyDeviceToHost, L.s));
+    return PW_OK;
+}
+
+// Waits for the launched execution, reads its tasks and records back and runs
+// the tail rounds of the pairs that did not fit.
+int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X, Finished &F) {
     HIPCHK(ctx, hipStreamSynchronize(L.s));
-    if (use_graph) {
-        ms_device = elapsed(L.ev[0], L.ev[5]);
+    if (X.graph) {
+        F.ms_score = elapsed(L.ev[0], L.ev[5]);
     } else {
-        ms_score = elapsed(L.ev[0], L.ev[1]);
-        ms_dir = elapsed(L.ev[2], L.ev[3]);
-        ms_tb = elapsed(L.ev[3], L.ev[4]);
-        ms_walk = elapsed(L.ev[4], L.ev[5]);
+        F.ms_score = elapsed(L.ev[0], L.ev[1]);
+        F.ms_dir = elapsed(L.ev[2], L.ev[3]);
+        F.ms_tb = elapsed(L.ev[3], L.ev[4]);
+        F.ms_walk = elapsed(L.ev[4], L.ev[5]);
     }
     const Ctl c = *(const Ctl *)W.h_ctl.p;
-    const int ng = c.ngated;
-    if (ng < 0 || ng > n_pairs) {
+    F.ng = c.ngated;
+    if (F.ng < 0 || F.ng > P->n_pairs) {
         ctx->err = "gate count out of range";
         return PW_EDEVICE;
     }
-    std::vector<PairOut> outs;
     std::vector<PairTask> pending;
-    rc = read_tasks(ctx, W, L, ng, outs, pending);
+    int rc = read_tasks(ctx, W, L, F.ng, F.outs, pending);
     if (rc) return rc;
-    TailStore store;
     if (!pending.empty()) {
         // the tail rounds reuse the staging buffers: keep copies of what they hold
-        for (PairOut &o : outs) {
-            store.sps.emplace_back(o.sps, o.sps + o.nsp);
-            store.hits.emplace_back(o.hits, o.hits + o.nsp);
-            store.ops.emplace_back(o.ops, o.ops + o.nops);
-            o.sps = store.sps.back().data();
-            o.hits = store.hits.back().data();
-            o.ops = store.ops.back().data();
+        for (PairOut &o : F.outs) {
+            F.store.sps.emplace_back(o.sps, o.sps + o.nsp);
+            F.store.hits.emplace_back(o.hits, o.hits + o.nsp);
+            F.store.ops.emplace_back(o.ops, o.ops + o.nops);
+            o.sps = F.store.sps.back().data();
+            o.hits = F.store.hits.back().data();
+            o.ops = F.store.ops.back().data();
         }
         adapt_workspace(ctx, W, c, pending);
-        rc = tail_rounds(ctx, P, W, L, g, minimum, pending, outs, store, &ms_dir, &ms_tb, &ms_walk);
+        rc = tail_rounds(ctx, P, W, L, X.g, X.minimum, pending, F.outs, F.store, &F.ms_dir, &F.ms_tb, &F.ms_walk);
         if (rc) return rc;
     }
-    assemble(outs, sp_cap, n_pairs, R);
+    std::sort(F.outs.begin(), F.outs.end(), [](const PairOut &a, const PairOut &b) { return a.pair < b.pair; });
+    return PW_OK;
+}
+
+void fill_public(HostResult *R, const float *maxima, int64_t n_pairs, double cells, int64_t ng, const Finished &F,
+                 const Exec &X) {
     pw_result &O = R->pub;
     O.n_pairs = n_pairs;
-    O.maxima = R->maxima;
+    O.maxima = maxima;
     O.n_starting_points = (int64_t)R->sps.size();
     O.starting_points = R->sps.data();
     O.n_hits = (int64_t)R->hits.size();
@@ -1351,13 +1410,27 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
     O.n_ops = (int64_t)R->ops.size();
     O.ops = R->ops.data();
     O.n_traceback_pairs = ng;
-    O.cells = P->cells;
-    O.ms_score = use_graph ? ms_device : ms_score;
-    O.ms_directions = ms_dir;
-    O.ms_traceback = ms_tb;
-    O.ms_walk = ms_walk;
-    O.score_launches = launches;
-    O.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    O.cells = cells;
+    O.ms_score = F.ms_score;
+    O.ms_directions = F.ms_dir;
+    O.ms_traceback = F.ms_tb;
+    O.ms_walk = F.ms_walk;
+    O.score_launches = X.launches;
+    O.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - X.wall0).count();
+}
+
+int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int32_t sp_cap, bool use_graph,
+            HostResult *R) {
+    int rc = take_maxima_buffer(ctx, P->n_pairs, R);
+    if (rc) return rc;
+    Exec X;
+    rc = execute_launch(ctx, P, W, L, minimum, use_graph, R->maxima, X);
+    if (rc) return rc;
+    Finished F;
+    rc = execute_finish(ctx, P, W, L, X, F);
+    if (rc) return rc;
+    assemble(F.outs, sp_cap, P->n_pairs, R);
+    fill_public(R, R->maxima, P->n_pairs, P->cells, F.ng, F, X);
     return PW_OK;
 }
 
@@ -1524,3 +1597,5 @@ extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res) {
     delete R;
     return PW_OK;
 }
+
+#include "sw_server.hip"

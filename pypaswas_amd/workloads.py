@@ -135,29 +135,73 @@ def config4(seed: int = 4, n_pairs: int = 64, length: int = 10000):
     return queries, targets, settings
 
 
-def serving_jobs(seed: int = 5, n_jobs: int = 1000):
-    """Serving mix: 70% short DNA (reads 100-300 x adapters 20-80), 25% protein
-    pairs (100-2,000, BLOSUM62 -11/-1), 5% long DNA pairs (5,000-20,000).
-    Returns a list of (kind, query, target)."""
+# Serving mix: scoring settings per request kind (settings_dict overrides)
+SERVING_SETTINGS = {
+    # short reads against an adapter panel: the trimmer scoring
+    "dna": dict(program="trimmer", matrix_name="DNA-RNA", match_score="1", mismatch_score="-3", gap_score="-5",
+                gap_extension="-1", minimum_score="10", lower_limit_score="1.0", filter_factor="0.2",
+                **OPEN_FILTERS),
+    "protein": dict(matrix_name="BLOSUM62", gap_score="-11", gap_extension="-1", minimum_score="30",
+                    lower_limit_score="1.0", filter_factor="0.05", **OPEN_FILTERS),
+    "long": dict(matrix_name="DNA-RNA", match_score="1", mismatch_score="-3", gap_score="-5", gap_extension="0",
+                 minimum_score="10", lower_limit_score="1.0", filter_factor="0.2", **OPEN_FILTERS),
+}
+SERVING_KINDS = ("dna", "protein", "long")
+
+
+def serving_jobs(seed: int = 5, n_jobs: int = 1000, panel: int = 8, long_range=(5000, 20000)):
+    """Serving mix of requests, each a small pyPaSWAS job (records in reader order):
+      * 70% "dna": one read of 100-300 nt (uniform ACGT) against a panel of
+        `panel` adapters of 20-80 nt drawn for the request (plus their reverse
+        complements, as the DNA reader adds them), trimmer scoring; half the
+        reads carry a 20-nt adapter piece at 5% substitutions;
+      * 25% "protein": one pair of 100-2,000 residues (background
+        frequencies), BLOSUM62 -11/-1; half the pairs are homologs at 60%
+        identity with 2% indels;
+      * 5% "long": one DNA pair of 5,000-20,000 nt (long_range) at 90%
+        identity with 1% indels (plus the reverse complement target), linear
+        +1/-3/-5.
+    Returns a list of (kind, queries, targets)."""
     rng = np.random.default_rng(seed)
     jobs = []
     for j in range(n_jobs):
         r = rng.random()
         if r < 0.70:
-            jobs.append(("dna", _uniform_dna(rng, int(rng.integers(100, 301))),
-                         _uniform_dna(rng, int(rng.integers(20, 81)))))
+            adapters = [("a%d_%d" % (j, k), _uniform_dna(rng, int(rng.integers(20, 81)))) for k in range(panel)]
+            read = list(_uniform_dna(rng, int(rng.integers(100, 301))))
+            if rng.random() < 0.5:
+                a = adapters[int(rng.integers(0, panel))][1]
+                piece = _substitute(rng, a[:20], 0.05)
+                off = int(rng.integers(0, len(read) - len(piece) + 1))
+                read[off:off + len(piece)] = list(piece)
+            q, t = reference_order([("r%d" % j, "".join(read))], adapters, True)
+            jobs.append(("dna", q, t))
         elif r < 0.95:
             n = int(rng.integers(100, 2001))
-            q = "".join(rng.choice(AMINO, size=n, p=AMINO_FREQ))
-            t = _mutate_indel(rng, q, 0.6, 0.02, AMINO, AMINO_FREQ) if rng.random() < 0.5 else \
+            qs = "".join(rng.choice(AMINO, size=n, p=AMINO_FREQ))
+            ts = _mutate_indel(rng, qs, 0.6, 0.02, AMINO, AMINO_FREQ) if rng.random() < 0.5 else \
                 "".join(rng.choice(AMINO, size=int(rng.integers(100, 2001)), p=AMINO_FREQ))
-            jobs.append(("protein", q, t))
+            jobs.append(("protein", [("p%d" % j, qs)], [("h%d" % j, ts)]))
         else:
-            n = int(rng.integers(5000, 20001))
-            q = _uniform_dna(rng, n)
-            t = _mutate_indel(rng, q, 0.9, 0.01, DNA)
+            n = int(rng.integers(long_range[0], long_range[1] + 1))
+            qs = _uniform_dna(rng, n)
+            ts = _mutate_indel(rng, qs, 0.9, 0.01, DNA)
+            q, t = reference_order([("l%d" % j, qs)], [("m%d" % j, ts)], True)
             jobs.append(("long", q, t))
     return jobs
+
+
+def serving_schemes():
+    """(Scheme per kind, in SERVING_KINDS order) for pypaswas_amd.serving."""
+    import logging
+    from pypaswas_amd.Core.Scores import Blosum62Score, DnaRnaScore
+    from pypaswas_amd.serving import Scheme
+    out = []
+    for kind in SERVING_KINDS:
+        st = settings_dict(SERVING_SETTINGS[kind])
+        cls = Blosum62Score if st.matrix_name.upper() == "BLOSUM62" else DnaRnaScore
+        out.append(Scheme.from_score(cls(logging.getLogger("serving"), st), float(st.filter_factor)))
+    return out
 
 
 def reverse_complement(s: str) -> str:
