@@ -95,6 +95,11 @@ def main():
     q, t, settings, score = build_job(args.workload)
     log("workload built: %d queries x %d targets in %.1fs" % (len(q), len(t), time.time() - t0))
     qp, tp, pq, pt, mss, same = job_arrays(q, t, score.highest_score, float(settings.filter_factor))
+    if args.workload == "config4":
+        # 64 one-query jobs (query i vs target i and its reverse complement), run as one batch
+        from pypaswas_amd.workloads import config4_pair_mask
+        keep = config4_pair_mask(q, t, pq, pt)
+        pq, pt, mss, same = pq[keep], pt[keep], mss[keep], same[keep]
     dev = Device(0)
     dev.set_scoring(score.to_float32(), ord(score.char_offset), score.gap_score,
                     score.gap_extension if score.gap_extension is not None else 0.0, score.lower_limit_score,
@@ -202,7 +207,7 @@ def serving_run(dev, n_jobs, qps, lanes, arena_mib, seed=5):
     raw = W.serving_jobs(seed, n_jobs)
     jobs = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t) for k, q, t in raw]
     warm = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t)
-            for k, q, t in W.serving_jobs(seed + 1000, 64)]
+            for k, q, t in W.serving_jobs(seed + 1000, 400)]
     log("serving: %d jobs built in %.1fs" % (n_jobs, time.time() - t0))
     srv = Server(dev, lanes=lanes, direction_bytes=int(arena_mib) << 20)
     try:
@@ -225,7 +230,11 @@ def serving_run(dev, n_jobs, qps, lanes, arena_mib, seed=5):
            "batches": st["batches"] - st0["batches"],
            "jobs_per_batch": round(n_jobs / max(1, st["batches"] - st0["batches"]), 2),
            "peak_arena_mib": round(st["peak_arena_bytes"] / 2 ** 20, 1),
-           "hits": int(sum(int(r.hits["shown"].sum()) if len(r.hits) else 0 for r in results))}
+           "hits": int(sum(int(r.hits["shown"].sum()) if len(r.hits) else 0 for r in results)),
+           "host_ms_launch": round((st["host_us_launch"] - st0["host_us_launch"]) / 1e3, 1),
+           "host_ms_finish": round((st["host_us_finish"] - st0["host_us_finish"]) / 1e3, 1),
+           "device_ms_batches": round((st["device_us"] - st0["device_us"]) / 1e3, 1),
+           "reallocs": st["reallocs"] - st0["reallocs"]}
     for k in W.SERVING_KINDS:
         m = kinds == k
         if m.any():

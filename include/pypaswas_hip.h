@@ -186,8 +186,10 @@ int pw_server_poll(pw_server *srv, int32_t block, int64_t *done_ids, int32_t max
 /* The result of a completed job, owned by the caller until pw_free_result. */
 int pw_server_take(pw_server *srv, int64_t job_id, pw_result **out);
 
-/* stats[0..6]: batches launched, jobs admitted, jobs completed, peak arena
- * bytes reserved, jobs queued, jobs running, arena bytes. */
+/* stats[0..10]: batches launched, jobs admitted, jobs completed, peak arena
+ * bytes reserved, jobs queued, jobs running, arena bytes, host microseconds
+ * forming and launching batches, host microseconds reading them back, device
+ * microseconds of the batches, device or pinned buffers grown so far. */
 int pw_server_stats(pw_server *srv, int64_t *stats, int32_t n);
 
 int pw_server_destroy(pw_server *srv);

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -72,10 +73,12 @@ struct Limits {
 // Device work areas of one execution, plus the pinned host staging of the
 // read-back.  want_* are the sizes the next execution allocates.
 struct Workspace {
-    DevBuf ctl, maxima, tasks, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch, dscratch;
+    DevBuf ctl, maxima, tasks, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch, dscratch,
+        units, dlinks, dprog;
     int64_t want_dir = 0, want_cand = 0, want_ops = 0;
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
+    int64_t units_cap = 0, dlinks_cap = 0, dprog_cap = 0;
     PinBuf h_ctl, h_tasks, h_sps, h_hits, h_ops;
     // direction area lent by a caller (a range of a serving arena) instead of dirs
     uint8_t *dir_ext = nullptr;
@@ -98,6 +101,14 @@ struct pw_ctx {
     Workspace ws;  // shared by the one-shot calls (pw_align_pairs, pw_score_pairs)
     // pinned host buffers for result maxima, recycled by pw_free_result
     std::vector<std::pair<float *, size_t>> pinned_free;
+    int64_t reallocs = 0;  // device or pinned buffers grown (statistics)
+    // while a server has batches in flight, buffers replaced by larger ones
+    // are kept here instead of freed (hipFree and hipHostFree wait for the
+    // whole device) and released once the device is idle
+    bool defer_free = false;
+    std::vector<void *> dead_dev, dead_pinned;
+    std::map<std::pair<const void *, size_t>, int> occupancy;
+    PinBuf h_stage;  // upload staging of the plans of pw_prepare and the one-shot calls
 };
 
 static thread_local std::string g_init_err;
@@ -147,7 +158,12 @@ struct pw_plan {
     int64_t dir_worst = 0;      // direction bytes if every pair passed the gate
     int64_t dir_largest = 0;    // largest single pair's direction area
     double cells = 0;
-    int64_t n_tile = 0;        // pairs scored by the wave-per-pair tile kernel
+    int64_t n_tile = 0;        // items of the wave-per-pair tile kernel (one per stripe of a split pair)
+    int64_t score_links = 0;   // floats of stripe-boundary rows of the split pairs of the scoring pass
+    int64_t score_prog = 0;    // their progress counters
+    int64_t units_worst = 0;   // direction-pass work units if every pair passed the gate
+    int64_t dlinks_worst = 0;  // boundary-row floats of the direction pass in that case
+    int64_t dprog_worst = 0;   // and its progress counters
     int64_t n_groups = 0;      // lane-per-pair jobs
     int64_t group_rows = 0;    // longest row sequence of a multi-chunk job (scratch sizing)
     int32_t max_nb = 16;
@@ -157,7 +173,11 @@ struct pw_plan {
     std::vector<uint8_t> trans;
     std::vector<GroupItem> gitems;
     std::vector<GroupEntry> gents;
+    // per-plan arrays: views into one device allocation (blob) filled by one
+    // copy from pinned staging
     DevBuf res, seqs, tables, d_items, d_trans, d_gitems, d_gents, d_mss, d_pairs, d_same;
+    DevBuf blob, d_links, d_prog;
+    PinBuf h_blob;
     Workspace own;
     // graph of the execution and the launch parameters it was captured with
     hipGraphExec_t graph = nullptr;
@@ -173,19 +193,28 @@ struct pw_plan {
         }                                                                                   \
     } while (0)
 
+// Buffers grow to twice the request, so that a sequence of growing requests
+// reallocates rarely (each reallocation frees, and hipFree waits for the device).
 static int ensure(pw_ctx *ctx, DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return PW_OK;
+    ctx->reallocs += 1;
     if (b.p) {
-        hipError_t e = hipFree(b.p);
+        if (ctx->defer_free) {
+            ctx->dead_dev.push_back(b.p);
+        } else {
+            hipError_t e = hipFree(b.p);
+            if (e != hipSuccess) {
+                b.p = nullptr;
+                b.cap = 0;
+                ctx->err = std::string("hipFree: ") + hipGetErrorString(e);
+                return PW_EDEVICE;
+            }
+        }
         b.p = nullptr;
         b.cap = 0;
-        if (e != hipSuccess) {
-            ctx->err = std::string("hipFree: ") + hipGetErrorString(e);
-            return PW_EDEVICE;
-        }
     }
-    size_t want = bytes + bytes / 4;
+    size_t want = bytes < ((size_t)1 << 30) ? 2 * bytes : bytes + bytes / 4;
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -199,12 +228,14 @@ static int ensure(pw_ctx *ctx, DevBuf &b, size_t bytes) {
 static int ensure_pinned(pw_ctx *ctx, PinBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 64;
     if (b.cap >= bytes) return PW_OK;
+    ctx->reallocs += 1;
     if (b.p) {
-        (void)hipHostFree(b.p);
+        if (ctx->defer_free) ctx->dead_pinned.push_back(b.p);
+        else (void)hipHostFree(b.p);
         b.p = nullptr;
         b.cap = 0;
     }
-    size_t want = bytes + bytes / 4;
+    size_t want = 2 * bytes;
     hipError_t e = hipHostMalloc(&b.p, want, hipHostMallocDefault);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -227,6 +258,14 @@ static int ensure_pinned(pw_ctx *ctx, PinBuf &b, size_t bytes) {
         if (rc_ != PW_OK) return rc_;                   \
     } while (0)
 
+// Releases the buffers kept while frees were deferred.
+static void release_dead(pw_ctx *ctx) {
+    for (void *p : ctx->dead_dev) (void)hipFree(p);
+    for (void *p : ctx->dead_pinned) (void)hipHostFree(p);
+    ctx->dead_dev.clear();
+    ctx->dead_pinned.clear();
+}
+
 namespace {
 
 void free_buf(DevBuf &b) {
@@ -243,7 +282,7 @@ void free_pinned(PinBuf &b) {
 
 void free_workspace(Workspace &W) {
     DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
-                      &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch};
+                      &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks, &W.dprog};
     for (DevBuf *b : bufs) free_buf(*b);
     PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_sps, &W.h_hits, &W.h_ops};
     for (PinBuf *b : pins) free_pinned(*b);
@@ -328,6 +367,8 @@ extern "C" int pw_destroy(pw_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     destroy_lane(ctx->lane);
     free_workspace(ctx->ws);
+    free_pinned(ctx->h_stage);
+    release_dead(ctx);
     for (auto &pb : ctx->pinned_free) (void)hipHostFree(pb.first);
     delete ctx;
     return PW_OK;
@@ -535,9 +576,9 @@ int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, con
 }
 
 void free_plan_buffers(pw_plan *P) {
-    DevBuf *bufs[] = {&P->res, &P->seqs, &P->tables, &P->d_items, &P->d_trans, &P->d_gitems,
-                      &P->d_gents, &P->d_mss, &P->d_pairs, &P->d_same};
+    DevBuf *bufs[] = {&P->blob, &P->d_links, &P->d_prog};
     for (DevBuf *b : bufs) free_buf(*b);
+    free_pinned(P->h_blob);
     free_workspace(P->own);
     if (P->graph) (void)hipGraphExecDestroy(P->graph);
     P->graph = nullptr;
@@ -552,7 +593,7 @@ void free_plan(pw_plan *P) {
 
 // Host half of pw_prepare: encodes the residues and builds the scoring work
 // list.  No device call.
-int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P, const uint8_t *qres,
+int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus, pw_plan *P, const uint8_t *qres,
               const int64_t *qoff, const int32_t *qlen, int32_t nq, const uint8_t *tres, const int64_t *toff,
               const int32_t *tlen, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs,
               const float *mss, const uint8_t *same_id, int32_t mode) {
@@ -582,15 +623,24 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
     P->max_t = 0;
     P->dir_worst = 0;
     P->dir_largest = 0;
+    P->units_worst = P->dlinks_worst = P->dprog_worst = 0;
     for (int64_t i = 0; i < n_pairs; ++i) {
         const int ql = qlen[pq[i]], tl = tlen[pt[i]];
         P->cells += (double)ql * (double)tl;
         P->max_t = std::max<int64_t>(P->max_t, tl);
-        const int64_t a = dir_area(ql, tl, dir_residues(ql));
+        const int R = dir_residues(ql);
+        const int64_t a = dir_area(ql, tl, R);
         P->dir_worst += a;
         P->dir_largest = std::max(P->dir_largest, a);
+        const int ns = split_stripes(ql, tl, R);
+        P->units_worst += ns > 0 ? ns : 1;
+        if (ns > 0) {
+            P->dlinks_worst += (int64_t)(ns - 1) * link_row_floats(tl);
+            P->dprog_worst += ns;
+        }
     }
     P->batches.clear();
+    P->score_links = P->score_prog = 0;
     P->gitems.clear();
     P->gents.clear();
     P->n_groups = 0;
@@ -614,7 +664,8 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
         int64_t begin = 0;
         int bx = 0, by = 0;
         auto close = [&](int64_t end) {
-            for (int64_t k = begin; k < end; ++k) items[k] = ScoreItem{pq[k], nq + pt[k], (int32_t)k, 4, bx, by};
+            for (int64_t k = begin; k < end; ++k)
+                items[k] = ScoreItem{pq[k], nq + pt[k], (int32_t)k, 4, bx, by, 0, 1 << 30, -1, 0};
             P->batches.push_back({begin, end});
             max_b = std::max<int64_t>(max_b, by);
         };
@@ -634,6 +685,7 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
     } else {
         // lane-per-pair jobs for pairs whose target is shared by enough pairs;
         // the rest (long or unshared pairs) go to the wave-per-pair tile kernel
+        const double per_lane = P->cells / ((double)std::max(1, num_cus) * 4.0 * WAVE);
         std::vector<std::vector<int64_t>> by_target((size_t)nt);
         for (int64_t i = 0; i < n_pairs; ++i) by_target[pt[i]].push_back(i);
         std::vector<int64_t> tile_pairs;
@@ -650,17 +702,21 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
             std::stable_sort(cand.begin(), cand.end(),
                              [&](int64_t x, int64_t y) { return qlen[pq[x]] > qlen[pq[y]]; });
             // keep a 64-lane job only if its lanes are busy for most of its
-            // rows (sum of row lengths vs 64 x the longest); otherwise its
-            // pairs go to the tile kernel
+            // rows (sum of row lengths vs 64 x the longest) and it is not
+            // longer than twice the plan's work per lane of the device (one
+            // such job would set the makespan; split tiles finish it sooner);
+            // otherwise its pairs go to the tile kernel
+            const int tl = tlen[t];
             std::vector<int64_t> grp;
             for (size_t g0 = 0; g0 < cand.size(); g0 += 64) {
                 const size_t g1 = std::min(cand.size(), g0 + 64);
                 double sum = 0;
                 for (size_t u = g0; u < g1; ++u) sum += qlen[pq[cand[u]]];
                 const double fill = sum / (64.0 * std::max(1, qlen[pq[cand[g0]]]));
-                for (size_t u = g0; u < g1; ++u) (fill >= GROUP_MIN_FILL ? grp : tile_pairs).push_back(cand[u]);
+                const double job_steps = (double)qlen[pq[cand[g0]]] * tl;
+                const bool keep = fill >= GROUP_MIN_FILL && job_steps <= 2.0 * per_lane;
+                for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
             }
-            const int tl = tlen[t];
             const int nb = group_nb(tl);
             const int nch = tl > nb ? (tl + nb - 1) / nb : 1;
             for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
@@ -697,9 +753,13 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
         }
         P->n_groups = (int64_t)P->gitems.size();
 
+        // pairs with several stripes and long rows are split into one item
+        // per stripe (consecutive, in stripe order), run by as many waves in a
+        // pipeline; the others are one item each
         const int64_t ntile = (int64_t)tile_pairs.size();
         std::vector<ScoreItem> titems((size_t)ntile);
         std::vector<uint8_t> ttrans((size_t)ntile, 0);
+        std::vector<int> tsplit((size_t)ntile, 0);
         std::vector<std::pair<double, int64_t>> order((size_t)ntile);
         for (int64_t k = 0; k < ntile; ++k) {
             const int64_t i = tile_pairs[k];
@@ -708,21 +768,42 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
             choose_shape(ql, tl, tr, R);
             const int a = tr ? tl : ql, b = tr ? ql : tl;
             const int aseq = tr ? nq + pt[i] : pq[i], bseq = tr ? pq[i] : nq + pt[i];
-            titems[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b};
+            titems[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b, 0, 1 << 30, -1, 0};
             ttrans[k] = (uint8_t)tr;
             const int rows = 64 * R;
-            order[k] = {-(double)((a + rows - 1) / rows) * (b + 63) * (R + 1.5), k};
+            const int ns = (a + rows - 1) / rows;
+            tsplit[k] = split_stripes(a, b, R);
+            // critical path: all stripes in a row, or the pipeline of a split pair
+            const double path = tsplit[k] ? (double)(b + 63 + (ns - 1) * 128) : (double)ns * (b + 63);
+            order[k] = {-path * (R + 1.5), k};
             max_b = std::max<int64_t>(max_b, b);
         }
         std::stable_sort(order.begin(), order.end(),
                          [](const std::pair<double, int64_t> &x, const std::pair<double, int64_t> &y) {
                              return x.first < y.first;
                          });
-        items.resize((size_t)ntile);
-        trans.resize((size_t)ntile);
+        items.clear();
+        trans.clear();
+        P->score_links = P->score_prog = 0;
         for (int64_t k = 0; k < ntile; ++k) {
-            items[k] = titems[order[k].second];
-            trans[k] = ttrans[order[k].second];
+            const int64_t o = order[k].second;
+            const int ns = tsplit[o];
+            if (ns == 0) {
+                items.push_back(titems[o]);
+                trans.push_back(ttrans[o]);
+                continue;
+            }
+            ScoreItem it = titems[o];
+            it.rows_off = P->score_links;
+            it.prog_off = P->score_prog;
+            P->score_links += (int64_t)(ns - 1) * link_row_floats(it.blen);
+            P->score_prog += ns;
+            for (int st = 0; st < ns; ++st) {
+                it.s0 = st;
+                it.s1 = st + 1;
+                items.push_back(it);
+                trans.push_back(ttrans[o]);
+            }
         }
     }
     P->max_b = max_b;
@@ -730,39 +811,46 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, pw_plan *P,
     return PW_OK;
 }
 
-// Device half of pw_prepare: uploads everything plan_host built, on stream s.
-int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s) {
+// Device half of pw_prepare: uploads everything plan_host built, on stream s,
+// with one copy from the pinned staging buffer stage (which must not be in use
+// by an earlier copy); sync waits for the copy, so that stage can be reused.
+int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync) {
     const Encoded &E = P->E;
     const int64_t n = P->n_pairs;
-    ENSURE(ctx, P->res, E.codes.size());
-    ENSURE(ctx, P->seqs, E.refs.size() * sizeof(SeqRef));
-    ENSURE(ctx, P->tables, E.tables.size() * sizeof(float));
-    ENSURE(ctx, P->d_items, P->items.size() * sizeof(ScoreItem));
-    ENSURE(ctx, P->d_trans, P->trans.size());
-    ENSURE(ctx, P->d_gitems, P->gitems.size() * sizeof(GroupItem));
-    ENSURE(ctx, P->d_gents, P->gents.size() * sizeof(GroupEntry));
-    ENSURE(ctx, P->d_mss, (size_t)n * sizeof(float));
-    ENSURE(ctx, P->d_pairs, (size_t)n * sizeof(int2));
-    ENSURE(ctx, P->d_same, (size_t)n);
     std::vector<int2> pairs((size_t)n);
     for (int64_t i = 0; i < n; ++i) pairs[i] = make_int2(P->pq[i], P->pt[i]);
-    auto up = [&](DevBuf &d, const void *h, size_t bytes) -> int {
-        if (bytes) HIPCHK(ctx, hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
-        return PW_OK;
+    struct Part {
+        DevBuf *d;
+        const void *h;
+        size_t bytes;
+        size_t off;
     };
-    int rc = 0;
-    if ((rc = up(P->res, E.codes.data(), E.codes.size()))) return rc;
-    if ((rc = up(P->seqs, E.refs.data(), E.refs.size() * sizeof(SeqRef)))) return rc;
-    if ((rc = up(P->tables, E.tables.data(), E.tables.size() * sizeof(float)))) return rc;
-    if ((rc = up(P->d_items, P->items.data(), P->items.size() * sizeof(ScoreItem)))) return rc;
-    if ((rc = up(P->d_trans, P->trans.data(), P->trans.size()))) return rc;
-    if ((rc = up(P->d_gitems, P->gitems.data(), P->gitems.size() * sizeof(GroupItem)))) return rc;
-    if ((rc = up(P->d_gents, P->gents.data(), P->gents.size() * sizeof(GroupEntry)))) return rc;
-    if ((rc = up(P->d_mss, P->mss.data(), (size_t)n * sizeof(float)))) return rc;
-    if ((rc = up(P->d_pairs, pairs.data(), (size_t)n * sizeof(int2)))) return rc;
-    if ((rc = up(P->d_same, P->same.data(), (size_t)n))) return rc;
-    // the pageable sources above must stay alive until the copies finish
-    HIPCHK(ctx, hipStreamSynchronize(s));
+    Part parts[] = {{&P->res, E.codes.data(), E.codes.size(), 0},
+                    {&P->seqs, E.refs.data(), E.refs.size() * sizeof(SeqRef), 0},
+                    {&P->tables, E.tables.data(), E.tables.size() * sizeof(float), 0},
+                    {&P->d_items, P->items.data(), P->items.size() * sizeof(ScoreItem), 0},
+                    {&P->d_trans, P->trans.data(), P->trans.size(), 0},
+                    {&P->d_gitems, P->gitems.data(), P->gitems.size() * sizeof(GroupItem), 0},
+                    {&P->d_gents, P->gents.data(), P->gents.size() * sizeof(GroupEntry), 0},
+                    {&P->d_mss, P->mss.data(), (size_t)n * sizeof(float), 0},
+                    {&P->d_pairs, pairs.data(), (size_t)n * sizeof(int2), 0},
+                    {&P->d_same, P->same.data(), (size_t)n, 0}};
+    size_t total = 0;
+    for (Part &q : parts) {
+        q.off = total;
+        total += (q.bytes + 255) / 256 * 256;
+    }
+    ENSURE(ctx, P->blob, total);
+    ENSURE_PINNED(ctx, stage, total);
+    for (Part &q : parts) {
+        if (q.bytes) std::memcpy((char *)stage.p + q.off, q.h, q.bytes);
+        q.d->p = (char *)P->blob.p + q.off;
+        q.d->cap = q.bytes;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(P->blob.p, stage.p, total, hipMemcpyHostToDevice, s));
+    ENSURE(ctx, P->d_links, (size_t)P->score_links * sizeof(float));
+    ENSURE(ctx, P->d_prog, (size_t)P->score_prog * sizeof(unsigned));
+    if (sync) HIPCHK(ctx, hipStreamSynchronize(s));
     return PW_OK;
 }
 
@@ -777,11 +865,20 @@ const void *group_kernel_for(bool affine, int nb) {
     }
 }
 
-int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work) {
+// Workgroups of 64 threads resident per CU (cached per kernel and LDS size).
+int blocks_per_cu(pw_ctx *ctx, const void *kernel, size_t lds) {
+    const auto key = std::make_pair(kernel, lds);
+    auto it = ctx->occupancy.find(key);
+    if (it != ctx->occupancy.end()) return it->second;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds) != hipSuccess || per_cu <= 0)
         per_cu = 8;
-    const int64_t g = (int64_t)ctx->num_cus * per_cu;
+    ctx->occupancy[key] = per_cu;
+    return per_cu;
+}
+
+int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work) {
+    const int64_t g = (int64_t)ctx->num_cus * blocks_per_cu(ctx, kernel, lds);
     return (int)std::max<int64_t>(1, std::min<int64_t>(work, g));
 }
 
@@ -804,7 +901,11 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
     }
     const int64_t n = P->n_pairs;
     g.gate_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
-    g.dir_grid = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)ctx->num_cus * 8));
+    // persistent direction-pass workers: as many as the units of every pair
+    // passing the gate could use (one per stripe of a split pair), up to the
+    // resident limit; split stripes need their producer stripes running
+    g.dir_grid = persistent_grid(ctx, (const void *)(aff ? dir_kernel<true> : dir_kernel<false>),
+                                 lds_bytes(P->E.KP), std::max<int64_t>(1, P->units_worst));
     g.dscr_stride = 3 * (P->max_t + 64);
     g.task_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * 4));
     return g;
@@ -828,7 +929,10 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     if (W.dir_ext) {
         W.dir_cap = W.dir_ext_cap;
     } else {
-        const int64_t dcap = std::max<int64_t>(16, std::min(P->dir_worst, std::min(W.want_dir, dir_limit(ctx))));
+        // the whole worst case when it is at most half the limit, else the
+        // first size (grown after an execution that needed more)
+        const int64_t want = P->dir_worst <= dir_limit(ctx) / 2 ? std::max(W.want_dir, P->dir_worst) : W.want_dir;
+        const int64_t dcap = std::max<int64_t>(16, std::min(P->dir_worst, std::min(want, dir_limit(ctx))));
         ENSURE(ctx, W.dirs, (size_t)dcap);
         W.dir_cap = (int64_t)W.dirs.cap;
     }
@@ -844,6 +948,16 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
                                     (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
     ENSURE(ctx, W.ops, (size_t)std::max<int64_t>(16, W.want_ops));
     W.ops_cap = (int64_t)W.ops.cap;
+    // direction-pass work units: sized for every pair passing the gate; the
+    // split pairs' boundary rows up to a quarter of the direction limit (pairs
+    // beyond it run unsplit)
+    ENSURE(ctx, W.units, (size_t)std::max<int64_t>(1, P->units_worst) * sizeof(DirUnit));
+    W.units_cap = (int64_t)(W.units.cap / sizeof(DirUnit));
+    const int64_t lk = std::min<int64_t>(P->dlinks_worst, dir_limit(ctx) / 4 / (int64_t)sizeof(float));
+    ENSURE(ctx, W.dlinks, (size_t)std::max<int64_t>(1, lk) * sizeof(float));
+    W.dlinks_cap = lk > 0 ? (int64_t)(W.dlinks.cap / sizeof(float)) : 0;
+    ENSURE(ctx, W.dprog, (size_t)std::max<int64_t>(1, P->dprog_worst) * sizeof(unsigned));
+    W.dprog_cap = (int64_t)(W.dprog.cap / sizeof(unsigned));
     int gmax = 0;
     for (int w = 0; w < NB_KINDS; ++w) gmax = std::max(gmax, g.group_grid[w]);
     if (gmax > 0) ENSURE(ctx, W.gscratch, (size_t)gmax * g.gscr_stride * sizeof(float2));
@@ -892,7 +1006,8 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
                 const int grid = (int)std::min<int64_t>(n, g.tile_grid);
                 hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, L.s, res, seqs, items + b.first, (int)n,
                                    trans + b.first, tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
-                                   (float *)W.tscratch.p, g.tscr_stride, (unsigned *)nullptr, mx);
+                                   (float *)W.tscratch.p, g.tscr_stride, (unsigned *)nullptr, mx, (float *)nullptr,
+                                   (unsigned *)nullptr);
                 HIPCHK(ctx, hipGetLastError());
                 if (launches) *launches += 1;
             }
@@ -905,7 +1020,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             }
             hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
                                tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
-                               ctl->queue, mx);
+                               ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p);
             HIPCHK(ctx, hipGetLastError());
             if (launches) *launches += 1;
             if (st != L.s) {
@@ -931,9 +1046,10 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
         return PW_EDEVICE;
     }
     auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
-    hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks, (const Ctl *)ctl,
-                       (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.dscratch.p,
-                       g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p);
+    hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks,
+                       (const DirUnit *)W.units.p, ctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
+                       P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
+                       (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
     HIPCHK(ctx, hipGetLastError());
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
     auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
@@ -950,20 +1066,38 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
     return PW_OK;
 }
 
+int launch_clear(pw_ctx *ctx, void *p, int64_t words, hipStream_t s) {
+    if (words <= 0) return PW_OK;
+    const int grid = (int)std::min<int64_t>((words + 255) / 256, 1024);
+    hipLaunchKernelGGL(clear_kernel, dim3(grid), dim3(256), 0, s, (uint32_t *)p, (int)words);
+    HIPCHK(ctx, hipGetLastError());
+    return PW_OK;
+}
+
+// Zeroes the control block, the split pairs' progress counters and (when
+// split pairs combine their stripes' maxima) the pair maxima, with kernels.
+int launch_clears(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L) {
+    int rc = launch_clear(ctx, W.ctl.p, (int64_t)(sizeof(Ctl) / 4), L.s);
+    if (!rc && P->score_prog > 0) rc = launch_clear(ctx, P->d_prog.p, P->score_prog, L.s);
+    if (!rc && P->score_prog > 0) rc = launch_clear(ctx, W.maxima.p, P->n_pairs, L.s);
+    return rc;
+}
+
 // The whole device sequence of one execution (capturable: kernels, event
 // fork/join and one device-to-host copy into pinned memory).
 int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, bool events,
                     int64_t *launches) {
     Ctl *ctl = (Ctl *)W.ctl.p;
-    hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(256), 0, L.s, (uint32_t *)ctl, (int)(sizeof(Ctl) / 4));
-    HIPCHK(ctx, hipGetLastError());
-    int rc = launch_score(ctx, P, W, L, g, events, launches);
+    int rc = launch_clears(ctx, P, W, L);
+    if (rc) return rc;
+    rc = launch_score(ctx, P, W, L, g, events, launches);
     if (rc) return rc;
     if (P->n_pairs > 0) {
         hipLaunchKernelGGL(gate_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
                            (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
                            (const SeqRef *)P->seqs.p, P->nq, P->n_pairs, P->sc.minimum, P->sc.ll, ctl,
-                           (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0);
+                           (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
+                           W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap);
         HIPCHK(ctx, hipGetLastError());
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
@@ -1122,11 +1256,21 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
                 rest.push_back(t);
             }
         }
-        const int nc = (int)chunk.size();
+        const int nc = (int)std::min<int64_t>((int64_t)chunk.size(), W.units_cap);
+        for (int k = nc; k < (int)chunk.size(); ++k) rest.push_back(chunk[k]);
+        chunk.resize((size_t)nc);
+        std::vector<DirUnit> units((size_t)nc);
+        for (int k = 0; k < nc; ++k) {
+            chunk[k].split = 0;  // tail rounds run each pair's stripes in one unit
+            units[k] = DirUnit{k, -1};
+        }
         Ctl c;
         std::memset(&c, 0, sizeof c);
         c.ngated = nc;
+        c.dir_units = nc;
         HIPCHK(ctx, hipMemcpyAsync(W.tasks.p, chunk.data(), (size_t)nc * sizeof(PairTask), hipMemcpyHostToDevice,
+                                   L.s));
+        HIPCHK(ctx, hipMemcpyAsync(W.units.p, units.data(), (size_t)nc * sizeof(DirUnit), hipMemcpyHostToDevice,
                                    L.s));
         HIPCHK(ctx, hipMemcpyAsync(W.ctl.p, &c, sizeof c, hipMemcpyHostToDevice, L.s));
         const bool aff = P->sc.affine;
@@ -1141,8 +1285,9 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
         auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
         hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks,
-                           (const Ctl *)dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
-                           (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p);
+                           (const DirUnit *)W.units.p, dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
+                           P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
+                           (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
         auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
@@ -1261,7 +1406,8 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     std::vector<uint64_t> s;
     auto add = [&](uint64_t v) { s.push_back(v); };
     const DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
-                            &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch};
+                            &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks,
+                            &W.dprog, &P->d_links, &P->d_prog};
     for (const DevBuf *b : bufs) {
         add((uint64_t)(uintptr_t)b->p);
         add(b->cap);
@@ -1272,6 +1418,9 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     add((uint64_t)W.cand_cap);
     add((uint64_t)W.ops_cap);
     add((uint64_t)W.cand0);
+    add((uint64_t)W.units_cap);
+    add((uint64_t)W.dlinks_cap);
+    add((uint64_t)W.dprog_cap);
     add((uint64_t)(uintptr_t)L.s);
     add((uint64_t)(uintptr_t)L.s2);
     uint64_t mb;
@@ -1467,9 +1616,9 @@ extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff,
     }
     *out = nullptr;
     pw_plan *P = new pw_plan();
-    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, ctx->num_cus, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    min_start_score, same_id, mode);
-    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s);
+    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     if (rc) {
         free_plan(P);
         return rc;
@@ -1526,17 +1675,17 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         return PW_EINVAL;
     }
     pw_plan *P = new pw_plan();
-    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, ctx->num_cus, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    nullptr, nullptr, mode);
-    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s);
+    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     Workspace &W = ctx->ws;
     Lane &L = ctx->lane;
     if (!rc) {
         const Geometry g = geometry(ctx, P);
         rc = size_workspace(ctx, P, W, g);
         if (!rc) {
-            hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(256), 0, L.s, (uint32_t *)W.ctl.p, (int)(sizeof(Ctl) / 4));
-            rc = launch_score(ctx, P, W, L, g, true, nullptr);
+            rc = launch_clears(ctx, P, W, L);
+            if (!rc) rc = launch_score(ctx, P, W, L, g, true, nullptr);
         }
     }
     if (!rc && n_pairs > 0) {
@@ -1571,9 +1720,9 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     }
     *out = nullptr;
     pw_plan *P = new pw_plan();
-    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, ctx->num_cus, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    min_start_score, same_id, mode);
-    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s);
+    if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     if (!rc) {
         HostResult *R = nullptr;
         new_result(&R);

@@ -51,8 +51,38 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 __device__ __forceinline__ float load_scr(const float *p) {
-    // the scratch row is rewritten in place by the same wave; bypass the L1
+    // the scratch row is rewritten in place by the same wave (or written by
+    // another wave of a split pair); bypass the L1
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_scr(float *p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Stripe hand-over of a split pair.  Boundary rows and counters are only
+// accessed with device-scope (relaxed) atomics, which are coherent across the
+// XCDs' L2 caches without whole-cache write-back or invalidation (the cost of
+// an agent-scope release or acquire fence on this chip).  The producer wave
+// waits until its row stores are performed (s_waitcnt), then one lane stores
+// the new column count; the consumer wave spins (with s_sleep) on the count
+// and only then issues its row loads.  Waits end: the producer stripe was
+// dequeued earlier by a running wave, and it publishes every chunk up to the
+// last column.
+__device__ __forceinline__ void publish_progress(unsigned *p, unsigned v) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need) {
+    for (;;) {
+        const unsigned v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (v >= need) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -62,12 +92,18 @@ __device__ __forceinline__ float load_scr(const float *p) {
 //                 default_main_cpu.cl:188-220.
 //   DIRS = true : A must be the query; writes one direction byte per cell
 //                 (tile-major, see cell_offset) and appends candidate starts.
+// Stripes [s_first, s_last) are processed.  links == nullptr: one wave runs
+// them in order, handing each stripe's bottom row to the next through scr in
+// place.  Otherwise (a split pair, one stripe per wave) the row above comes
+// from boundary s-1 of links and the bottom row goes to boundary s, chunk by
+// chunk, synchronised through the progress counters prog.
 // ---------------------------------------------------------------------------
 template <bool AFF, int R, bool DIRS>
 __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int alen_it, int blen_it,
                       const float *subT, int lkp, uint8_t *bwin, float *__restrict__ scr, float G, float E,
                       int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
-                      Cand *__restrict__ cands, int cand_cap) {
+                      Cand *__restrict__ cands, int cand_cap, int s_first, int s_last, float *__restrict__ links,
+                      unsigned *__restrict__ prog) {
     const int lane = threadIdx.x & 63;
     const int rows = WAVE * R;
     const int nstripes = (alen_it + rows - 1) / rows;
@@ -75,8 +111,10 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     const char *subB = reinterpret_cast<const char *>(subT);
     float rmax = 0.0f;
     const float bJ = AFF ? G : 0.0f;  // J candidate entering row 1: max(-1e10 + E, 0 + G)
+    const int64_t row_floats = link_row_floats(blen_it);
+    if (s_last > nstripes) s_last = nstripes;
 
-    for (int s = 0; s < nstripes; ++s) {
+    for (int s = s_first; s < s_last; ++s) {
         int a4[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -92,12 +130,15 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         }
         const bool above = s > 0;
         const bool below = s + 1 < nstripes;
+        const float *sin = links ? links + (int64_t)(s - 1) * row_floats : scr;  // row above (when above)
+        float *sout = links ? links + (int64_t)s * row_floats : scr;             // bottom row (when below)
         float inT = 0.0f, inJ = bJ, inM = 0.0f;  // staged previous-stripe row, lane l = column c0+l
         float oT = 0.0f, oJ = 0.0f, oM = 0.0f;   // staged bottom row to hand down
+        if (links && above && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(WAVE, blen_it));
         if (above && lane < blen_it) {
-            inT = load_scr(&scr[3 * lane]);
-            if (AFF) inJ = load_scr(&scr[3 * lane + 1]);
-            if (DIRS && AFF) inM = load_scr(&scr[3 * lane + 2]);
+            inT = load_scr(&sin[3 * lane]);
+            if (AFF) inJ = load_scr(&sin[3 * lane + 1]);
+            if (DIRS && AFF) inM = load_scr(&sin[3 * lane + 2]);
         }
         // inputs of step 0: lane 0 reads column 0 of the row above; the other
         // lanes start in the virtual columns left of the matrix
@@ -236,10 +277,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                     if (sl == WAVE - 1 || c63 == blen_it - 1) {
                         const int c = (c63 & ~(WAVE - 1)) + lane;
                         if (c <= c63) {
-                            scr[3 * c] = oT;
-                            if (AFF) scr[3 * c + 1] = oJ;
-                            if (DIRS && AFF) scr[3 * c + 2] = oM;
+                            store_scr(&sout[3 * c], oT);
+                            if (AFF) store_scr(&sout[3 * c + 1], oJ);
+                            if (DIRS && AFF) store_scr(&sout[3 * c + 2], oM);
                         }
+                        if (links) publish_progress(prog + s, (unsigned)(c63 + 1));
                     }
                 }
             }
@@ -252,10 +294,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                     inT = 0.0f;
                     inJ = bJ;
                     inM = 0.0f;
+                    if (links && tn < blen_it) wait_progress(prog + (s - 1), (unsigned)min(tn + WAVE, blen_it));
                     if (c < blen_it) {
-                        inT = load_scr(&scr[3 * c]);
-                        if (AFF) inJ = load_scr(&scr[3 * c + 1]);
-                        if (DIRS && AFF) inM = load_scr(&scr[3 * c + 2]);
+                        inT = load_scr(&sin[3 * c]);
+                        if (AFF) inJ = load_scr(&sin[3 * c + 1]);
+                        if (DIRS && AFF) inM = load_scr(&sin[3 * c + 2]);
                     }
                 }
                 injT = rdlane(inT, tn & (WAVE - 1));
@@ -268,7 +311,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             if (AFF) upJ = shr1(injJ, botJ);
             if (DIRS && AFF) upM = shr1(injM, botM);
         }
-        __threadfence();
+        if (below && !links) __threadfence();  // hand-over through scr to the next stripe of this wave
     }
     return rmax;
 }
@@ -282,22 +325,29 @@ __device__ __forceinline__ void load_tables(float *lds, const float *__restrict_
 template <bool AFF, int R>
 __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *seqs, const ScoreItem &it,
                                             const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float *scr,
-                                            float G, float E, int fill, int trans) {
+                                            float G, float E, int fill, int trans, float *links, unsigned *prog) {
     const float *tab = lds_tab + (trans ? kp2 : 0);
+    float *lk = it.rows_off >= 0 ? links + it.rows_off : nullptr;
+    unsigned *pg = it.rows_off >= 0 ? prog + it.prog_off : nullptr;
     return tile<AFF, R, false>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, scr, G, E,
-                               fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0);
+                               fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg);
 }
 
 // Scoring pass.  queue != nullptr: persistent workers pull items from a
-// device counter (longest first, as the host orders them); otherwise each
-// workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...
+// device counter (longest first, as the host orders them; the stripes of a
+// split pair are consecutive items in stripe order, so the wave a stripe waits
+// for has always been dequeued before it); otherwise each workgroup takes
+// items blockIdx.x, blockIdx.x + gridDim.x, ... (no split pairs then).  A
+// split pair's maximum is the maximum over its stripes, combined with an
+// integer atomic max on the (non-negative) float bits into a zeroed slot.
 template <bool AFF>
 __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
                                                    const ScoreItem *__restrict__ items, int nitems,
                                                    const uint8_t *__restrict__ trans_flags,
                                                    const float *__restrict__ tables, int kp, int lkp, float G,
                                                    float E, float *__restrict__ scratch, int64_t scr_stride,
-                                                   unsigned *__restrict__ queue, float *__restrict__ maxima) {
+                                                   unsigned *__restrict__ queue, float *__restrict__ maxima,
+                                                   float *__restrict__ links, unsigned *__restrict__ prog) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
@@ -320,13 +370,16 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
         const int tr = trans_flags ? trans_flags[item] : 0;
         float m;
         switch (it.R) {
-        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
-        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
-        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
-        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr); break;
+        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
+        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
+        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
+        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
         }
         m = wave_max(m);
-        if (threadIdx.x == 0) maxima[it.out] = m;
+        if (threadIdx.x == 0) {
+            if (it.rows_off >= 0) atomicMax(reinterpret_cast<int *>(maxima) + it.out, __float_as_int(m));
+            else maxima[it.out] = m;
+        }
     }
 }
 
@@ -436,7 +489,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
             if (right_out) scr[r * WAVE + lane] = make_float2(Tp[NB - 1], ic);
             dl = lb.x;
         }
-        __threadfence();
+        if (right_out) __threadfence();  // scratch rows for the next chunk
     }
     return rmax;
 }
@@ -477,33 +530,46 @@ __device__ __forceinline__ int task_count(const Ctl *ctl) {
     return __hip_atomic_load(&ctl->ngated, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Direction pass over the traceback tasks; persistent workgroups take tasks
-// blockIdx.x, blockIdx.x + gridDim.x, ... up to the device-side task count.
+// Direction pass over the work units of the traceback tasks (one unit per
+// task, or one per stripe of a split task, consecutive in stripe order);
+// persistent workgroups pull units from a device counter up to the unit count.
 template <bool AFF>
 __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
-                                                 PairTask *__restrict__ tasks, const Ctl *__restrict__ ctl,
-                                                 const float *__restrict__ tables, int kp, int lkp, float G, float E,
-                                                 float *__restrict__ scratch, int64_t scr_stride,
-                                                 uint8_t *__restrict__ dirs, Cand *__restrict__ cands) {
+                                                 PairTask *__restrict__ tasks, const DirUnit *__restrict__ units,
+                                                 Ctl *__restrict__ ctl, const float *__restrict__ tables, int kp,
+                                                 int lkp, float G, float E, float *__restrict__ scratch,
+                                                 int64_t scr_stride, uint8_t *__restrict__ dirs,
+                                                 Cand *__restrict__ cands, float *__restrict__ links,
+                                                 unsigned *__restrict__ prog, int64_t units_cap) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
     uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
     float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
-    const int n = task_count(ctl);
-    for (int item = blockIdx.x; item < n; item += gridDim.x) {
-        const PairTask it = tasks[item];
+    const int n = (int)min((int64_t)__hip_atomic_load(&ctl->dir_units, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           units_cap);
+    for (;;) {
+        unsigned v = 0;
+        if (threadIdx.x == 0) v = atomicAdd(&ctl->dir_queue, 1u);
+        const int u = (int)__builtin_amdgcn_readfirstlane(v);
+        if (u >= n) break;
+        const DirUnit du = units[u];
+        const PairTask it = tasks[du.task];
         if (it.status != TASK_OK) continue;
         const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
         uint8_t *d = dirs + it.dir_off;
-        int *cc = &tasks[item].ncand;
+        int *cc = &tasks[du.task].ncand;
         Cand *cl = cands + it.cand_off;
+        const int s0 = du.stripe < 0 ? 0 : du.stripe;
+        const int s1 = du.stripe < 0 ? (1 << 30) : du.stripe + 1;
+        float *lk = du.stripe < 0 ? nullptr : links + it.rows_off;
+        unsigned *pg = du.stripe < 0 ? nullptr : prog + it.prog_off;
         switch (it.R) {
-        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
-        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
-        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
-        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap); break;
+        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
         }
     }
 }
@@ -527,19 +593,32 @@ __device__ __host__ __forceinline__ int64_t dir_area(int qlen, int tlen, int R) 
     return (stripes * (int64_t)(tlen + WAVE - 1) * WAVE * R + 15) & ~(int64_t)15;
 }
 
+// A pair is split into one work unit per stripe when it has several stripes
+// and enough columns for the stripes to overlap.
+constexpr int SPLIT_MIN_COLS = 256;
+__device__ __host__ __forceinline__ int split_stripes(int alen, int blen, int R) {
+    const int rows = WAVE * R;
+    const int ns = alen > 0 ? (alen + rows - 1) / rows : 1;
+    return (ns >= 2 && blen >= SPLIT_MIN_COLS) ? ns : 0;
+}
+
 // Traceback gate of default_main_cpu.cl:408-410 (maximum >= MINIMUM_SCORE)
 // plus the pair's minimum start score, without which no cell can start
 // (:427).  Every passing pair gets a PairTask with its direction and
-// candidate areas taken from the bump allocators in ctl; a pair that does
-// not fit is marked TASK_DEFERRED for a later round.  One task-slot atomic
-// per wave.
+// candidate areas taken from the bump allocators in ctl, and its direction-
+// pass work units (one per stripe for a split pair, whose boundary rows and
+// zeroed progress counters are allocated here too; without room it runs
+// unsplit).  A pair that does not fit is marked TASK_DEFERRED for a later
+// round.  One task-slot atomic per wave.
 __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
                                                    const int2 *__restrict__ pairs,
                                                    const uint8_t *__restrict__ same_id,
                                                    const SeqRef *__restrict__ seqs, int nq, int64_t n,
                                                    float minimum, float ll, Ctl *__restrict__ ctl,
                                                    PairTask *__restrict__ tasks, int64_t dir_cap,
-                                                   int64_t cand_cap_total, int cand0) {
+                                                   int64_t cand_cap_total, int cand0, DirUnit *__restrict__ units,
+                                                   int64_t units_cap, int64_t link_cap,
+                                                   unsigned *__restrict__ prog, int64_t prog_cap) {
     const int lane = threadIdx.x & 63;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = base + threadIdx.x;
@@ -582,7 +661,33 @@ __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ max
         t.qidx = pr.x;
         t.tidx = pr.y;
         t.out_off = 0;
-        t.pad = 0;
+        t.split = 0;
+        t.rows_off = 0;
+        t.prog_off = 0;
+        const int ns = split_stripes(ql, tl, t.R);
+        if (t.status == TASK_OK && ns > 0) {
+            const int64_t rows = (int64_t)(ns - 1) * link_row_floats(tl);
+            const unsigned long long lo = atomicAdd(&ctl->link_used, (unsigned long long)rows);
+            const unsigned long long po = atomicAdd(&ctl->prog_used, (unsigned long long)ns);
+            if ((int64_t)lo + rows <= link_cap && (int64_t)po + ns <= prog_cap) {
+                t.split = 1;
+                t.rows_off = (int64_t)lo;
+                t.prog_off = (int64_t)po;
+                for (int k = 0; k < ns; ++k) prog[po + k] = 0u;
+            }
+        }
+        if (t.status == TASK_OK) {
+            const int cnt = t.split ? ns : 1;
+            const int k0 = atomicAdd(&ctl->dir_units, cnt);
+            if ((int64_t)k0 + cnt <= units_cap) {
+                for (int k = 0; k < cnt; ++k) units[k0 + k] = DirUnit{slot, t.split ? k : -1};
+            } else {
+                // no unit slots (the host sizes them for every pair): leave the
+                // reserved units as no-ops and finish the pair in a tail round
+                t.status = TASK_DEFERRED;
+                for (int k = 0; k < cnt && (int64_t)k0 + k < units_cap; ++k) units[k0 + k] = DirUnit{slot, -1};
+            }
+        }
         tasks[slot] = t;
     }
 }

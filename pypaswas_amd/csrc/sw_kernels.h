@@ -30,6 +30,22 @@ struct ScoreItem {
     int32_t R;       // residues per lane: 1, 2, 4 or 8
     int32_t alen;    // rows to iterate (>= real length; FILL beyond it)
     int32_t blen;    // columns to iterate
+    int32_t s0, s1;  // stripes of this item: all of the pair's, or one stripe of a split pair
+    int64_t rows_off;  // split pair: its stripe-boundary rows in the link area (floats); -1 otherwise
+    int64_t prog_off;  // split pair: its progress counters in the counter area
+};
+
+// Stripe-boundary rows of a split pair: boundary k (between stripes k and
+// k+1) holds LINK_VALS floats per column for blen + 64 columns; counter k
+// holds the number of columns of boundary k already written.
+constexpr int LINK_VALS = 3;
+__host__ __device__ inline int64_t link_row_floats(int blen) { return (int64_t)LINK_VALS * (blen + 64); }
+
+// One work unit of the direction pass: a task, and the stripe of it this
+// unit runs (-1: all its stripes, one after the other).
+struct DirUnit {
+    int32_t task;
+    int32_t stripe;
 };
 
 // One job of the lane-per-pair scoring kernel: a column sequence shared by up
@@ -70,7 +86,9 @@ struct PairTask {
     int32_t status;      // TASK_*
     int32_t qidx, tidx;  // indices reported in the records
     int32_t out_off;     // first compacted starting point / hit of the pair
-    int32_t pad;
+    int32_t split;       // 1: the direction pass runs each stripe as its own unit
+    int64_t rows_off;    // split: stripe-boundary rows in the link area (floats)
+    int64_t prog_off;    // split: progress counters in the counter area
 };
 
 // status of a PairTask
@@ -85,9 +103,11 @@ constexpr int NB_KINDS = 6;
 struct Ctl {
     unsigned queue[16 * (1 + NB_KINDS)];
     int ngated;
-    int pad0;
+    int dir_units;       // work units of the direction pass
     unsigned long long dir_used, cand_used, ops_used, out_used;
-    unsigned long long pad1[3];
+    unsigned long long link_used, prog_used;  // split-pair areas of the direction pass
+    unsigned dir_queue;  // next direction-pass unit
+    unsigned pad0;
 };
 
 struct Cand {

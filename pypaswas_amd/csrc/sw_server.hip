@@ -132,7 +132,13 @@ struct pw_server {
     uint64_t launch_seq = 0;
     // statistics: batches launched, jobs admitted, jobs completed, arena bytes reserved (peak, now)
     int64_t n_batches = 0, n_admitted = 0, n_completed = 0, peak_reserved = 0, reserved = 0;
+    // host time spent forming and launching batches, and reading them back; device time of the batches
+    double us_launch = 0, us_finish = 0, us_device = 0;
 };
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 namespace {
 
@@ -144,6 +150,7 @@ int serve_fail(pw_server *S, const std::string &msg, int rc) {
 // Merges the jobs into the lane's plan, uploads it and launches the batch.
 int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, int64_t dir_off, int64_t dir_len) {
     pw_ctx *ctx = S->ctx;
+    const double t0 = now_us();
     const Scoring &sc = S->schemes[batch.front()->scheme];
     std::vector<uint8_t> qres, tres;
     std::vector<int64_t> qoff, toff;
@@ -175,11 +182,12 @@ int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, in
     if (qres.empty()) qres.push_back(0);
     if (tres.empty()) tres.push_back(0);
     pw_plan *P = ln.P;
-    int rc = plan_host(ctx->err, sc, ctx->total_mem, P, qres.data(), qoff.data(), qlen.data(), (int32_t)qlen.size(),
+    int rc = plan_host(ctx->err, sc, ctx->total_mem, ctx->num_cus, P, qres.data(), qoff.data(), qlen.data(), (int32_t)qlen.size(),
                        tres.data(), toff.data(), tlen.data(), (int32_t)tlen.size(), pq.data(), pt.data(),
                        (int64_t)pq.size(), mss.data(), same.data(), 1);
     if (rc) return rc;
-    rc = plan_upload(ctx, P, ln.L.s);
+    // the lane's previous batch (and with it the copy out of this staging) has finished
+    rc = plan_upload(ctx, P, ln.L.s, P->h_blob, false);
     if (rc) return rc;
     Workspace &W = P->own;
     W.dir_ext = (uint8_t *)S->arena.p + dir_off;
@@ -195,6 +203,7 @@ int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, in
     ln.seq = ++S->launch_seq;
     S->n_batches += 1;
     S->n_admitted += (int64_t)batch.size();
+    S->us_launch += now_us() - t0;
     return PW_OK;
 }
 
@@ -231,9 +240,11 @@ void split_job(const Finished &F, const ServeLane &ln, ServeJob *j, size_t &curs
 
 int finish_batch(pw_server *S, ServeLane &ln) {
     pw_ctx *ctx = S->ctx;
+    const double t0 = now_us();
     Finished F;
     int rc = execute_finish(ctx, ln.P, ln.P->own, ln.L, ln.X, F);
     if (rc) return rc;
+    S->us_device += 1e3 * elapsed(ln.L.ev[0], ln.L.ev[5]);
     size_t cursor = 0;
     for (ServeJob *j : ln.jobs) {
         split_job(F, ln, j, cursor);
@@ -247,6 +258,7 @@ int finish_batch(pw_server *S, ServeLane &ln) {
     S->reserved -= ln.dir_len;
     ln.jobs.clear();
     ln.busy = false;
+    S->us_finish += now_us() - t0;
     return PW_OK;
 }
 
@@ -344,6 +356,7 @@ extern "C" int pw_server_create(pw_ctx *ctx, int32_t lanes, int64_t direction_by
         S->ra[1].reset(cap - small, small);
     }
     S->lanes.resize((size_t)lanes);
+    ctx->defer_free = true;
     for (ServeLane &ln : S->lanes) {
         rc = create_lane(ln.L, ctx->err, true);
         hipError_t e = rc ? hipSuccess : hipEventCreateWithFlags(&ln.done_ev, hipEventDisableTiming);
@@ -355,6 +368,8 @@ extern "C" int pw_server_create(pw_ctx *ctx, int32_t lanes, int64_t direction_by
                 if (l2.P) free_plan(l2.P);
             }
             free_buf(S->arena);
+            ctx->defer_free = false;
+            release_dead(ctx);
             delete S;
             return PW_EDEVICE;
         }
@@ -439,14 +454,20 @@ extern "C" int pw_server_poll(pw_server *S, int32_t block, int64_t *done_ids, in
     *n_done = 0;
     for (;;) {
         // finish batches whose device work has ended, then refill the lanes
+        bool any_busy = false;
         for (ServeLane &ln : S->lanes) {
             if (!ln.busy) continue;
             hipError_t e = hipEventQuery(ln.done_ev);
-            if (e == hipErrorNotReady) continue;
+            if (e == hipErrorNotReady) {
+                any_busy = true;
+                continue;
+            }
             if (e != hipSuccess) return serve_fail(S, std::string("hipEventQuery: ") + hipGetErrorString(e), PW_EDEVICE);
             int rc = finish_batch(S, ln);
             if (rc) return rc;
         }
+        // buffers outgrown during the batches are freed while nothing runs
+        if (!any_busy && (!ctx->dead_dev.empty() || !ctx->dead_pinned.empty())) release_dead(ctx);
         int rc = admit(S);
         if (rc) return rc;
         if (!block || !S->completed.empty()) break;
@@ -480,9 +501,10 @@ extern "C" int pw_server_stats(pw_server *S, int64_t *stats, int32_t n) {
     if (!S || (n > 0 && !stats)) return PW_EINVAL;
     int64_t running = 0;
     for (const ServeLane &ln : S->lanes) running += ln.busy ? (int64_t)ln.jobs.size() : 0;
-    const int64_t v[7] = {S->n_batches, S->n_admitted, S->n_completed, S->peak_reserved,
-                          (int64_t)S->queue.size(), running, S->arena_cap};
-    for (int32_t i = 0; i < n && i < 7; ++i) stats[i] = v[i];
+    const int64_t v[11] = {S->n_batches, S->n_admitted, S->n_completed, S->peak_reserved,
+                           (int64_t)S->queue.size(), running, S->arena_cap, (int64_t)S->us_launch,
+                           (int64_t)S->us_finish, (int64_t)S->us_device, S->ctx->reallocs};
+    for (int32_t i = 0; i < n && i < 11; ++i) stats[i] = v[i];
     return PW_OK;
 }
 
@@ -502,6 +524,8 @@ extern "C" int pw_server_destroy(pw_server *S) {
     }
     // queued jobs are in the map too
     free_buf(S->arena);
+    ctx->defer_free = false;
+    release_dead(ctx);
     delete S;
     return PW_OK;
 }

@@ -107,9 +107,10 @@ class Server:
         return self.dev._take_result(out)
 
     def stats(self) -> Dict[str, int]:
-        v = np.zeros(7, dtype=np.int64)
-        self.dev._check(self.lib.pw_server_stats(self.handle, v.ctypes.data, 7), "pw_server_stats")
-        keys = ("batches", "admitted", "completed", "peak_arena_bytes", "queued", "running", "arena_bytes")
+        v = np.zeros(11, dtype=np.int64)
+        self.dev._check(self.lib.pw_server_stats(self.handle, v.ctypes.data, 11), "pw_server_stats")
+        keys = ("batches", "admitted", "completed", "peak_arena_bytes", "queued", "running", "arena_bytes",
+                "host_us_launch", "host_us_finish", "device_us", "reallocs")
         return {k: int(x) for k, x in zip(keys, v)}
 
     def close(self):

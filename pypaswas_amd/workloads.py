@@ -118,7 +118,10 @@ def config3(seed: int = 3, n_queries: int = 2000, n_targets: int = 20000, famili
 def config4(seed: int = 4, n_pairs: int = 64, length: int = 10000):
     """64 DNA pairs of 10,000 x 10,000 (uniform ACGT) sharing a planted
     2,000-nt region at 85% identity with 2% indels; linear +1/-3/-5.  Pair i
-    is query i vs target i (and its reverse complement)."""
+    is query i vs target i (and its reverse complement): 64 one-query jobs
+    (see config4_pair_mask).  The filter factor is 0.05 so that the planted
+    pairs reach the traceback (at the default 0.2 the minimum start score,
+    0.2 x 10,000, exceeds every pair maximum)."""
     rng = np.random.default_rng(seed)
     queries, targets = [], []
     for i in range(n_pairs):
@@ -131,8 +134,16 @@ def config4(seed: int = 4, n_pairs: int = 64, length: int = 10000):
         queries.append(("long%02d_a" % i, "".join(q)))
         targets.append(("long%02d_b" % i, "".join(t)))
     settings = dict(matrix_name="DNA-RNA", match_score="1", mismatch_score="-3", gap_score="-5", gap_extension="0",
-                    minimum_score="10", lower_limit_score="1.0", filter_factor="0.2", **OPEN_FILTERS)
+                    minimum_score="10", lower_limit_score="1.0", filter_factor="0.05", **OPEN_FILTERS)
     return queries, targets, settings
+
+
+def config4_pair_mask(queries: Records, targets: Records, pq: np.ndarray, pt: np.ndarray) -> np.ndarray:
+    """Pairs of the 64 config-4 jobs among all query x target pairs: query
+    longNN_a with target longNN_b and its reverse complement longNN_b_RC."""
+    qk = np.array([i[:6] for i, _ in queries])
+    tk = np.array([i[:6] for i, _ in targets])
+    return qk[pq] == tk[pt]
 
 
 # Serving mix: scoring settings per request kind (settings_dict overrides)

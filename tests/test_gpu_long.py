@@ -1,0 +1,90 @@
+"""Long pairs: stripes split over several waves (scoring and direction pass) match the oracle.
+
+Pairs with several stripes and at least 256 swept columns run one stripe per
+wave, each stripe waiting for the bottom row of the one above.  These tests
+cover such pairs against the oracle (linear, affine, protein), and compare the
+split runs with runs where the split is impossible (plain launch for the
+scoring pass; no room for boundary rows in the direction pass).
+"""
+import numpy as np
+import pytest
+
+from tests._parity import AMINO, DNA, compare, hip_all_vs_all, mutate, rand_seq
+from tests.test_gpu_parity import blosum_params, dna_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from pypaswas_amd._native import Device
+    d = Device(0)
+    yield d
+    d.close()
+
+
+def long_pairs(rng, n, length, alphabet, region=800, ident=0.85, indel=0.02):
+    """n queries and n targets of about `length`, each target carrying a
+    mutated region of the query with the same index."""
+    qs, ts = [], []
+    for _ in range(n):
+        q = rand_seq(rng, int(rng.integers(length[0], length[1] + 1)), alphabet)
+        t = list(rand_seq(rng, int(rng.integers(length[0], length[1] + 1)), alphabet))
+        a = int(rng.integers(0, len(q) - region))
+        piece = mutate(rng, q[a:a + region], 1.0 - ident, indel, alphabet)[:region]
+        off = int(rng.integers(0, len(t) - len(piece)))
+        t[off:off + len(piece)] = list(piece)
+        qs.append(q)
+        ts.append("".join(t))
+    return qs, ts
+
+
+def same_result(a, b):
+    assert np.array_equal(a.maxima.view(np.uint32), b.maxima.view(np.uint32))
+    assert np.array_equal(a.starting_points, b.starting_points)
+    assert np.array_equal(a.hits, b.hits)
+    assert a.ops == b.ops
+
+
+@pytest.mark.parametrize("affine", [False, True])
+def test_long_dna_vs_oracle(dev, oracle, affine):
+    print("long dna affine", affine, flush=True)
+    rng = np.random.default_rng(40 + int(affine))
+    qs, ts = long_pairs(rng, 2, (2500, 4000), DNA)
+    # filter factor low enough that the planted pairs pass the traceback gate
+    p = dna_params(oracle, affine, ff=0.05)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs >= 2
+    plain = hip_all_vs_all(dev, oracle, qs, ts, p, mode=0)
+    same_result(plain, res)
+
+
+def test_long_protein_vs_oracle(dev, oracle):
+    rng = np.random.default_rng(44)
+    qs, ts = long_pairs(rng, 2, (1500, 2500), AMINO, region=500, ident=0.6)
+    p = blosum_params(oracle, ff=0.02)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs >= 2
+
+
+@pytest.fixture()
+def nosplit_dev():
+    from pypaswas_amd._native import Device
+    d = Device(0)
+    # a direction limit this small leaves no room for boundary rows: every pair runs unsplit
+    d.set_limits(max_direction_bytes=1 << 16)
+    yield d
+    d.close()
+
+
+@pytest.mark.parametrize("affine", [False, True])
+def test_split_equals_unsplit(dev, nosplit_dev, oracle, affine):
+    rng = np.random.default_rng(50 + int(affine))
+    qs, ts = long_pairs(rng, 3, (1200, 2200), DNA, region=500)
+    p = dna_params(oracle, affine, ff=0.05)
+    split = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    unsplit = hip_all_vs_all(nosplit_dev, oracle, qs, ts, p, mode=1)
+    same_result(split, unsplit)
+    assert split.n_traceback_pairs >= 3
