@@ -64,13 +64,14 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-batch launch")
     ap.add_argument("--cpu-reads", type=int, default=800, help="reads in the CPU-baseline sample")
     ap.add_argument("--no-protein", action="store_true", help="skip the uneven-length protein comparison")
-    ap.add_argument("--protein", default="100x400", help="queries x targets of the protein sample")
+    ap.add_argument("--protein", default="200x2000", help="queries x targets of the protein sample")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--serving-jobs", type=int, default=3000, help="requests in the serving run")
     ap.add_argument("--serving-qps", type=float, default=2000.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serving-lanes", type=int, default=4, help="stream lanes of the server")
     ap.add_argument("--serving-arena-mib", type=int, default=16384, help="direction arena of the server")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving measurement")
+    ap.add_argument("--protein-only", action="store_true", help="run only the protein comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -84,6 +85,11 @@ def main():
 
     from pypaswas_amd._native import Device
     from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
+
+    if args.protein_only:
+        nq, nt = (int(v) for v in args.protein.split("x"))
+        print(json.dumps(protein_comparison(Device(0), nq, nt)), flush=True)
+        return
 
     if args.serving_only:
         dev = Device(0)
@@ -189,7 +195,8 @@ def main():
         "hits": int(last.hits["shown"].sum()) if len(last.hits) else 0,
         "stage_ms": {"score": round(last.ms_score, 3), "directions": round(last.ms_directions, 3),
                      "traceback": round(last.ms_traceback, 3), "walk": round(last.ms_walk, 3),
-                     "call_wall": round(last.ms_total, 3)},
+                     "host_readback": round(last.ms_readback, 3), "host_tail": round(last.ms_tail, 3),
+                     "host_assemble": round(last.ms_assemble, 3), "call_wall": round(last.ms_total, 3)},
     }
     print(json.dumps(out), flush=True)
 
@@ -278,14 +285,23 @@ def protein_comparison(dev, nq, nt):
     res = {}
     for mode, name in ((1, "batched"), (0, "plain")):
         plan = dev.prepare(qp, tp, pq, pt, mss, same, mode=mode)
-        dev.execute(plan, minimum)
-        t0 = time.perf_counter()
-        r = dev.execute(plan, minimum)
-        el = time.perf_counter() - t0
+        for _ in range(2):  # the first execution sizes the work areas
+            dev.execute(plan, minimum)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = dev.execute(plan, minimum)
+            times.append(time.perf_counter() - t0)
+        el = float(np.median(times))
         plan.release()
         res[name] = r
         out[name + "_gcups"] = round(r.cells / el / 1e9, 2)
         out[name + "_score_kernel_ms"] = round(r.ms_score, 3)
+        out[name + "_stage_ms"] = {"score": round(r.ms_score, 3), "directions": round(r.ms_directions, 3),
+                                   "traceback": round(r.ms_traceback, 3), "walk": round(r.ms_walk, 3),
+                                   "host_readback": round(r.ms_readback, 3), "host_tail": round(r.ms_tail, 3),
+                                   "host_assemble": round(r.ms_assemble, 3), "call_wall": round(r.ms_total, 3)}
+        out["traceback_pairs"] = r.n_traceback_pairs
         log("protein %s: %.1f ms, %.2f GCUPS (score kernels %.2f ms)" % (name, el * 1000, r.cells / el / 1e9,
                                                                         r.ms_score))
     out["cells"] = res["batched"].cells

@@ -65,6 +65,9 @@ typedef struct {
     double ms_walk;                      /* walk */
     double ms_total;                     /* wall time of the call, host clock */
     int64_t score_launches;
+    double ms_readback;                  /* host: reading tasks and records back (incl. copies) */
+    double ms_tail;                      /* host: tail rounds of pairs that did not fit the work areas */
+    double ms_assemble;                  /* host: records into pair order */
 } pw_result;
 
 /* SmithWatermanOcl._initialize_device (pyPaSWAS/Core/SmithWatermanOcl.py:116-129)

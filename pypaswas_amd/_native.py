@@ -52,7 +52,8 @@ class ResultRec(ctypes.Structure):
                 ("ops", ctypes.POINTER(ctypes.c_char)), ("n_traceback_pairs", ctypes.c_int64),
                 ("sp_truncated", ctypes.c_int64), ("cells", ctypes.c_double), ("ms_score", ctypes.c_double),
                 ("ms_directions", ctypes.c_double), ("ms_traceback", ctypes.c_double), ("ms_walk", ctypes.c_double),
-                ("ms_total", ctypes.c_double), ("score_launches", ctypes.c_int64)]
+                ("ms_total", ctypes.c_double), ("score_launches", ctypes.c_int64),
+                ("ms_readback", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_assemble", ctypes.c_double)]
 
 
 _lib = None
@@ -145,6 +146,9 @@ class AlignResult:
     ms_walk: float
     ms_total: float
     score_launches: int
+    ms_readback: float = 0.0
+    ms_tail: float = 0.0
+    ms_assemble: float = 0.0
 
 
 SP_DTYPE = np.dtype([("sequence", "<u4"), ("target", "<u4"), ("blockX", "<u4"), ("blockY", "<u4"),
@@ -239,7 +243,8 @@ class Device:
             ops = ctypes.string_at(r.ops, int(r.n_ops)) if r.n_ops else b""
             return AlignResult(maxima, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
                                float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
-                               float(r.ms_walk), float(r.ms_total), int(r.score_launches))
+                               float(r.ms_walk), float(r.ms_total), int(r.score_launches), float(r.ms_readback),
+                               float(r.ms_tail), float(r.ms_assemble))
         finally:
             self.lib.pw_free_result(self.ctx, out)
 

@@ -551,6 +551,7 @@ void choose_shape(int qlen, int tlen, int &trans, int &R) {
 // a 64-lane job is used when its lanes are busy for at least this fraction
 // of its rows; the longest row sequence such a job takes
 constexpr double GROUP_MIN_FILL = 0.6;
+constexpr double GROUP_MAX_SHARE = 0.25;
 constexpr int GROUP_MAX_ROWS = 1 << 15;
 
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
@@ -702,9 +703,11 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus
             std::stable_sort(cand.begin(), cand.end(),
                              [&](int64_t x, int64_t y) { return qlen[pq[x]] > qlen[pq[y]]; });
             // keep a 64-lane job only if its lanes are busy for most of its
-            // rows (sum of row lengths vs 64 x the longest) and it is not
-            // longer than twice the plan's work per lane of the device (one
-            // such job would set the makespan; split tiles finish it sooner);
+            // rows (sum of row lengths vs 64 x the longest) and its cells per
+            // lane stay below a quarter of the plan's cells per lane of the
+            // device: the job runs as one wave, which alone on a SIMD needs as
+            // long per cell as the whole SIMD needs per lane-cell, so a longer
+            // job would set the makespan (split tiles finish it sooner);
             // otherwise its pairs go to the tile kernel
             const int tl = tlen[t];
             std::vector<int64_t> grp;
@@ -714,7 +717,7 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus
                 for (size_t u = g0; u < g1; ++u) sum += qlen[pq[cand[u]]];
                 const double fill = sum / (64.0 * std::max(1, qlen[pq[cand[g0]]]));
                 const double job_steps = (double)qlen[pq[cand[g0]]] * tl;
-                const bool keep = fill >= GROUP_MIN_FILL && job_steps <= 2.0 * per_lane;
+                const bool keep = fill >= GROUP_MIN_FILL && job_steps <= GROUP_MAX_SHARE * per_lane;
                 for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
             }
             const int nb = group_nb(tl);
@@ -1449,7 +1452,12 @@ struct Finished {
     TailStore store;
     int ng = 0;
     double ms_score = 0, ms_dir = 0, ms_tb = 0, ms_walk = 0;
+    double ms_readback = 0, ms_tail = 0, ms_assemble = 0;  // host phases
 };
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
 
 // Sizes the work areas and issues the device work of one execution on L, the
 // pair maxima going to maxima_dst (pinned host memory, n_pairs floats).  Nothing
@@ -1527,8 +1535,11 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
         return PW_EDEVICE;
     }
     std::vector<PairTask> pending;
+    auto t0 = std::chrono::steady_clock::now();
     int rc = read_tasks(ctx, W, L, F.ng, F.outs, pending);
     if (rc) return rc;
+    F.ms_readback = ms_since(t0);
+    t0 = std::chrono::steady_clock::now();
     if (!pending.empty()) {
         // the tail rounds reuse the staging buffers: keep copies of what they hold
         for (PairOut &o : F.outs) {
@@ -1543,7 +1554,10 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
         rc = tail_rounds(ctx, P, W, L, X.g, X.minimum, pending, F.outs, F.store, &F.ms_dir, &F.ms_tb, &F.ms_walk);
         if (rc) return rc;
     }
+    F.ms_tail = ms_since(t0);
+    t0 = std::chrono::steady_clock::now();
     std::sort(F.outs.begin(), F.outs.end(), [](const PairOut &a, const PairOut &b) { return a.pair < b.pair; });
+    F.ms_assemble = ms_since(t0);
     return PW_OK;
 }
 
@@ -1565,6 +1579,9 @@ void fill_public(HostResult *R, const float *maxima, int64_t n_pairs, double cel
     O.ms_traceback = F.ms_tb;
     O.ms_walk = F.ms_walk;
     O.score_launches = X.launches;
+    O.ms_readback = F.ms_readback;
+    O.ms_tail = F.ms_tail;
+    O.ms_assemble = F.ms_assemble;
     O.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - X.wall0).count();
 }
 
@@ -1578,7 +1595,9 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
     Finished F;
     rc = execute_finish(ctx, P, W, L, X, F);
     if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
     assemble(F.outs, sp_cap, P->n_pairs, R);
+    F.ms_assemble += ms_since(t0);
     fill_public(R, R->maxima, P->n_pairs, P->cells, F.ng, F, X);
     return PW_OK;
 }

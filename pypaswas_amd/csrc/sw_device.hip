@@ -50,9 +50,11 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Scratch rows handed from one stripe or chunk to the next (by the same wave,
+// or by another wave for a split pair) are written and read only with
+// device-scope relaxed atomics: coherent per location, they need no fence and
+// no cache maintenance.
 __device__ __forceinline__ float load_scr(const float *p) {
-    // the scratch row is rewritten in place by the same wave (or written by
-    // another wave of a split pair); bypass the L1
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -311,7 +313,6 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             if (AFF) upJ = shr1(injJ, botJ);
             if (DIRS && AFF) upM = shr1(injM, botM);
         }
-        if (below && !links) __threadfence();  // hand-over through scr to the next stripe of this wave
     }
     return rmax;
 }
@@ -486,10 +487,12 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                 }
                 asm volatile("" : "+v"(rmax), "+v"(ic), "+v"(diag));
             }
-            if (right_out) scr[r * WAVE + lane] = make_float2(Tp[NB - 1], ic);
+            if (right_out) {
+                store_scr(&scr[r * WAVE + lane].x, Tp[NB - 1]);
+                store_scr(&scr[r * WAVE + lane].y, ic);
+            }
             dl = lb.x;
         }
-        if (right_out) __threadfence();  // scratch rows for the next chunk
     }
     return rmax;
 }

@@ -233,6 +233,9 @@ void split_job(const Finished &F, const ServeLane &ln, ServeJob *j, size_t &curs
     stats.ms_dir = F.ms_dir;
     stats.ms_tb = F.ms_tb;
     stats.ms_walk = F.ms_walk;
+    stats.ms_readback = F.ms_readback;
+    stats.ms_tail = F.ms_tail;
+    stats.ms_assemble = F.ms_assemble;
     fill_public(R, R->maxima_vec.data(), (int64_t)j->pq.size(), j->cells, (int64_t)mine.size(), stats, ln.X);
     j->result = R;
     j->done = true;
