@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--no-protein", action="store_true", help="skip the uneven-length protein comparison")
     ap.add_argument("--protein", default="200x2000", help="queries x targets of the protein sample")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-autotune", action="store_true", help="use the built-in shape cost model")
     ap.add_argument("--serving-jobs", type=int, default=3000, help="requests in the serving run")
     ap.add_argument("--serving-qps", type=float, default=2000.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serving-lanes", type=int, default=4, help="stream lanes of the server")
@@ -88,7 +89,7 @@ def main():
 
     if args.protein_only:
         nq, nt = (int(v) for v in args.protein.split("x"))
-        print(json.dumps(protein_comparison(Device(0), nq, nt)), flush=True)
+        print(json.dumps(protein_comparison(Device(0), nq, nt, not args.no_autotune)), flush=True)
         return
 
     if args.serving_only:
@@ -112,10 +113,27 @@ def main():
                     score.minimum_score)
     minimum = float(settings.minimum_score)
     kind = "affine" if score.gap_extension is not None else "linear"
+    tuning = None
+    if not args.no_autotune:
+        # measured kernel shapes for this gap model, before the plans are built (untimed)
+        sec = dev.autotune()
+        tab = dev.tuning(kind == "affine")
+        best = tab["rates"].argmax(axis=2)
+        tuning = {"seconds": round(sec, 2), "group_chunk_width": tab["group_nb_long"],
+                  "group_rates_cells_per_us": {w: round(float(r), 1) for w, r in
+                                               zip((32, 48, 64, 80, 96), tab["group_rates"])},
+                  "residues_per_lane": [[int([1, 2, 4, 8][k]) for k in row] for row in best],
+                  "buckets": "rows: lane-side length <=64,128,256,512,1024,2048,more; "
+                             "columns: swept length <=32,64,128,256,512,1024,more"}
+        log("autotune: %.2fs, multi-chunk width %d" % (sec, tab["group_nb_long"]))
 
     t0 = time.time()
     plan = dev.prepare(qp, tp, pq, pt, mss, same, mode=1)
     log("batched plan prepared in %.1fs" % (time.time() - t0))
+    plan_tuning = None
+    if not args.no_autotune:
+        plan_tuning = dev.tune_plan(plan)
+        log("plan tuned in %.1fs: %s" % (plan_tuning["seconds"], plan_tuning))
     for i in range(args.warmup):
         r = dev.execute(plan, minimum)
         log("warmup %d: %.1f ms (score %.1f ms, dirs %.1f, tb %.1f, walk %.1f), %d traceback pairs, %d hits" %
@@ -172,7 +190,7 @@ def main():
     protein = None
     if not args.no_protein:  # last: it changes the device's scoring constants
         nq, nt = (int(v) for v in args.protein.split("x"))
-        protein = protein_comparison(dev, nq, nt)
+        protein = protein_comparison(dev, nq, nt, not args.no_autotune)
 
     out = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
@@ -193,6 +211,8 @@ def main():
         "serving": serving,
         "traceback_pairs": last.n_traceback_pairs,
         "hits": int(last.hits["shown"].sum()) if len(last.hits) else 0,
+        "autotune": tuning,
+        "plan_tuning": plan_tuning,
         "stage_ms": {"score": round(last.ms_score, 3), "directions": round(last.ms_directions, 3),
                      "traceback": round(last.ms_traceback, 3), "walk": round(last.ms_walk, 3),
                      "host_readback": round(last.ms_readback, 3), "host_tail": round(last.ms_tail, 3),
@@ -264,7 +284,7 @@ def copy_calibration(mib):
     log("copied %d MiB x 3" % mib)
 
 
-def protein_comparison(dev, nq, nt):
+def protein_comparison(dev, nq, nt, tune=True):
     """Uneven-length protein all-vs-all sample (BLOSUM62, -11/-1; lengths lognormal
     median 350, sigma 0.6, clipped 30-5000): batched persistent vs plain launch,
     same pairs, results compared."""
@@ -285,6 +305,8 @@ def protein_comparison(dev, nq, nt):
     res = {}
     for mode, name in ((1, "batched"), (0, "plain")):
         plan = dev.prepare(qp, tp, pq, pt, mss, same, mode=mode)
+        if mode == 1 and tune:
+            out["plan_tuning"] = dev.tune_plan(plan)
         for _ in range(2):  # the first execution sizes the work areas
             dev.execute(plan, minimum)
         times = []

@@ -153,6 +153,41 @@ int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const 
                    const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
                    double *ms_out);
 
+/* Autotuner.  Times the scoring kernels on synthetic pairs for the gap model
+ * of the current scoring scheme: the wave-per-pair tile kernel with 1, 2, 4
+ * and 8 residues per lane for every bucket of lane-side length (<= 64, 128,
+ * 256, 512, 1024, 2048, more) and swept length (<= 32, 64, 128, 256, 512,
+ * 1024, more), and the chunk widths 32..96 of lane-per-pair jobs whose target
+ * exceeds 96 residues.  Plans prepared afterwards take, per pair, the
+ * orientation and residues per lane with the highest measured rate of its
+ * buckets (instead of the built-in cost model), and the fastest chunk width.
+ * Replaces nothing in pyPaSWAS, whose kernels have one fixed shape
+ * (8x8 blocks of 2x2 work-items x 4x4 cells, default_main_cpu.cl:108-222);
+ * results never depend on the shape.  *seconds: time the tuning took. */
+int pw_autotune(pw_ctx *ctx, double *seconds);
+
+/* The tuning table of a gap model (affine != 0: affine): rates[7][7][4] in
+ * cells per microsecond (lane-side bucket, swept bucket, 1/2/4/8 residues per
+ * lane), group_rates[5] for chunk widths 32, 48, 64, 80, 96, the chosen width
+ * and whether the table is in use. */
+int pw_get_tuning(pw_ctx *ctx, int32_t affine, float *rates, float *group_rates, int32_t *group_nb_long,
+                  int32_t *valid);
+
+/* Installs a saved table (rates as pw_get_tuning returns them); rates == NULL
+ * drops the table and restores the cost model. */
+int pw_set_tuning(pw_ctx *ctx, int32_t affine, const float *rates, int32_t group_nb_long);
+
+/* Plan-level tuning of a batched plan (pw_prepare, mode 1): its scoring pass
+ * is timed under each combination of three routing choices -- the longest
+ * lane-per-pair job allowed (1/8, 1/4 or 1/2 of the plan's cells per device
+ * lane), the chunk width of lane-per-pair jobs whose target exceeds 96
+ * residues (64 or 96), and tile shapes from the pw_autotune table or from the
+ * cost model (when a table exists) -- and the fastest work list is kept for
+ * the plan's executions.  Results never depend on these choices.
+ * *seconds: tuning time; *score_ms: the kept scoring pass; choice[3]: share x
+ * 1000, chunk width, table used.  Plain plans (mode 0) are left as they are. */
+int pw_tune_plan(pw_ctx *ctx, pw_plan *plan, double *seconds, double *score_ms, int32_t *choice);
+
 /* ------------------------------------------------------------------------
  * Serving: a request loop with continuous batching.  pyPaSWAS has no request
  * loop; a job here is the work of one SmithWaterman.align_sequences pass

@@ -26,7 +26,7 @@ PW_EDEVICE = -3
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
            "pw_execute", "pw_release_plan", "pw_execute_graph", "pw_set_limits", "pw_score_pairs",
-           "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
+           "pw_autotune", "pw_get_tuning", "pw_set_tuning", "pw_tune_plan", "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
            "pw_server_stats", "pw_server_destroy", "pw_free_result", "pw_last_error")
 
 
@@ -94,6 +94,11 @@ def load() -> ctypes.CDLL:
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
+        L.pw_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.pw_get_tuning.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_int32)]
+        L.pw_set_tuning.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32]
+        L.pw_tune_plan.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), vp]
         L.pw_server_create.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(vp)]
         L.pw_server_set_scoring.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -269,6 +274,39 @@ class Device:
         self._check(fn(self.ctx, plan.handle, float(minimum), int(sp_cap), ctypes.byref(out)),
                     "pw_execute_graph" if graph else "pw_execute")
         return self._take_result(out)
+
+    def autotune(self) -> float:
+        """Measures the kernel shapes for the current scheme's gap model (pw_autotune); returns seconds."""
+        sec = ctypes.c_double()
+        self._check(self.lib.pw_autotune(self.ctx, ctypes.byref(sec)), "pw_autotune")
+        return float(sec.value)
+
+    def tuning(self, affine: bool) -> dict:
+        """The tuning table of a gap model (pw_get_tuning)."""
+        rates = np.zeros((7, 7, 4), dtype=np.float32)
+        grates = np.zeros(5, dtype=np.float32)
+        nb = ctypes.c_int32()
+        valid = ctypes.c_int32()
+        self._check(self.lib.pw_get_tuning(self.ctx, 1 if affine else 0, rates.ctypes.data, grates.ctypes.data,
+                                           ctypes.byref(nb), ctypes.byref(valid)), "pw_get_tuning")
+        return {"valid": bool(valid.value), "rates": rates, "group_rates": grates, "group_nb_long": int(nb.value)}
+
+    def set_tuning(self, affine: bool, rates: Optional[np.ndarray], group_nb_long: int = 64):
+        """Installs a saved table (pw_set_tuning); rates=None restores the cost model."""
+        r = None if rates is None else np.ascontiguousarray(rates, dtype=np.float32).reshape(7, 7, 4)
+        self._check(self.lib.pw_set_tuning(self.ctx, 1 if affine else 0, None if r is None else r.ctypes.data,
+                                           int(group_nb_long)), "pw_set_tuning")
+
+    def tune_plan(self, plan: "Plan") -> dict:
+        """Times the plan's scoring pass under the routing choices and keeps the fastest (pw_tune_plan)."""
+        sec = ctypes.c_double()
+        ms = ctypes.c_double()
+        choice = np.zeros(3, dtype=np.int32)
+        self._check(self.lib.pw_tune_plan(self.ctx, plan.handle, ctypes.byref(sec), ctypes.byref(ms),
+                                          choice.ctypes.data), "pw_tune_plan")
+        return {"seconds": round(float(sec.value), 3), "score_ms": round(float(ms.value), 3),
+                "group_max_share": choice[0] / 1000.0, "group_chunk_width": int(choice[1]),
+                "measured_shapes": bool(choice[2])}
 
     def set_limits(self, max_direction_bytes: int = 0, candidates_per_pair: int = 0, ops_bytes: int = 0,
                    grow: bool = True):

@@ -89,6 +89,47 @@ uint8_t *dir_ptr(const Workspace &W) { return W.dir_ext ? W.dir_ext : (uint8_t *
 
 }  // namespace
 
+namespace {
+
+// Measured shape table (pw_autotune), one per gap model.  Buckets of the
+// lane-side length a (<= 64, 128, 256, 512, 1024, 2048, more) and of the
+// swept length b (<= 32, 64, 128, 256, 512, 1024, more); rate = cells per
+// microsecond of the tile kernel with 1, 2, 4 or 8 residues per lane on a
+// device full of such pairs.
+constexpr int TUNE_A = 7, TUNE_B = 7, TUNE_R = 4;
+constexpr int TUNE_RS[TUNE_R] = {1, 2, 4, 8};
+constexpr int TUNE_REP_A[TUNE_A] = {48, 96, 192, 384, 768, 1536, 3072};
+constexpr int TUNE_REP_B[TUNE_B] = {24, 48, 96, 192, 384, 768, 1536};
+struct Tuning {
+    bool valid = false;
+    float rate[TUNE_A][TUNE_B][TUNE_R] = {};
+    int group_nb_long = 64;  // chunk width of lane-per-pair jobs whose target exceeds 96 residues
+    float group_rate[5] = {};  // measured for widths 32, 48, 64, 80, 96
+};
+
+// Planning choices besides the inputs: device facts, the tuned shapes, and
+// the overrides the autotuner uses to time one shape.
+struct PlanOpts {
+    size_t total_mem = 0;
+    int num_cus = 256;
+    const Tuning *tune = nullptr;
+    int force_R = 0;        // > 0: every tile item uses these residues per lane, untransposed
+    bool force_tile = false;  // every pair to the tile kernel
+    bool force_group = false; // every pair to lane-per-pair jobs (no fill or length rule)
+    int force_nb = 0;       // > 0: chunk width of multi-chunk lane-per-pair jobs
+    double group_max_share = 0.25;  // longest lane-per-pair job, as a share of the cells per device lane
+};
+
+// Bucket k holds lengths up to 4/3 of its representative length rep[k]
+// (64, 128, ... for the lane side; 32, 64, ... for the swept side).
+int tune_bucket(int v, const int *rep, int n) {
+    for (int k = 0; k < n - 1; ++k)
+        if (v <= rep[k] * 4 / 3) return k;
+    return n - 1;
+}
+
+}  // namespace
+
 struct pw_ctx {
     int device = 0;
     int num_cus = 0;
@@ -109,6 +150,7 @@ struct pw_ctx {
     std::vector<void *> dead_dev, dead_pinned;
     std::map<std::pair<const void *, size_t>, int> occupancy;
     PinBuf h_stage;  // upload staging of the plans of pw_prepare and the one-shot calls
+    Tuning tune[2];  // linear, affine
 };
 
 static thread_local std::string g_init_err;
@@ -179,6 +221,7 @@ struct pw_plan {
     DevBuf blob, d_links, d_prog;
     PinBuf h_blob;
     Workspace own;
+    PlanOpts opts;  // routing choices the work list was built with
     // graph of the execution and the launch parameters it was captured with
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_sig;
@@ -527,9 +570,47 @@ int encode(std::string &err, const Scoring &sc, const uint8_t *qres, const int64
 
 inline int pad8(int n) { return (n + 7) / 8 * 8; }
 
-// Tile shape for the batched pass: orientation and residues per lane that
-// minimise the modelled cost stripes * (columns + 63) * (R + overhead).
-void choose_shape(int qlen, int tlen, int &trans, int &R) {
+// Tile shape for the batched pass: orientation and residues per lane with the
+// highest measured rate of the pair's length buckets when the device has been
+// tuned, else those that minimise the modelled cost
+// stripes * (columns + 63) * (R + overhead).
+void choose_shape_model(int qlen, int tlen, int &trans, int &R);
+
+int tune_rk(int R) { return R == 1 ? 0 : R == 2 ? 1 : R == 4 ? 2 : 3; }
+
+// Measured rate of a shape (0 when the table has none).
+double tuned_rate(const Tuning *tu, int qlen, int tlen, int o, int R) {
+    const int a = o ? tlen : qlen, b = o ? qlen : tlen;
+    return tu->rate[tune_bucket(a, TUNE_REP_A, TUNE_A)][tune_bucket(b, TUNE_REP_B, TUNE_B)][tune_rk(R)];
+}
+
+// Measured rates decide only when they beat the cost model's shape clearly
+// (by TUNE_MARGIN): a shape timed alone on uniform pairs can lose a few per
+// cent inside a mixed batch that shares the device with other kernels.
+constexpr double TUNE_MARGIN = 1.08;
+
+void choose_shape(int qlen, int tlen, int &trans, int &R, const Tuning *tu) {
+    choose_shape_model(qlen, tlen, trans, R);
+    if (!tu || !tu->valid) return;
+    const double base = tuned_rate(tu, qlen, tlen, trans, R);
+    double rbest = base;
+    int ob = trans, Rb = R;
+    for (int o = 0; o < 2; ++o)
+        for (int k = 0; k < TUNE_R; ++k) {
+            const double r = tuned_rate(tu, qlen, tlen, o, TUNE_RS[k]);
+            if (r > rbest) {
+                rbest = r;
+                ob = o;
+                Rb = TUNE_RS[k];
+            }
+        }
+    if (rbest > TUNE_MARGIN * base) {
+        trans = ob;
+        R = Rb;
+    }
+}
+
+void choose_shape_model(int qlen, int tlen, int &trans, int &R) {
     double best = 1e300;
     trans = 0;
     R = 4;
@@ -551,15 +632,15 @@ void choose_shape(int qlen, int tlen, int &trans, int &R) {
 // a 64-lane job is used when its lanes are busy for at least this fraction
 // of its rows; the longest row sequence such a job takes
 constexpr double GROUP_MIN_FILL = 0.6;
-constexpr double GROUP_MAX_SHARE = 0.25;
 constexpr int GROUP_MAX_ROWS = 1 << 15;
 
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
-// chunks of 64 beyond
-int group_nb(int clen) {
+// chunks of 64 (or the tuned width) beyond
+int group_nb(int clen, const Tuning *tu) {
     if (clen <= 96) return std::max(16, (clen + 15) / 16 * 16);
-    return 64;
+    return (tu && tu->valid) ? tu->group_nb_long : 64;
 }
+
 
 size_t lds_bytes(int KP) { return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE; }
 
@@ -594,7 +675,9 @@ void free_plan(pw_plan *P) {
 
 // Host half of pw_prepare: encodes the residues and builds the scoring work
 // list.  No device call.
-int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus, pw_plan *P, const uint8_t *qres,
+int plan_worklist(const PlanOpts &opt, pw_plan *P);
+
+int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan *P, const uint8_t *qres,
               const int64_t *qoff, const int32_t *qlen, int32_t nq, const uint8_t *tres, const int64_t *toff,
               const int32_t *tlen, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs,
               const float *mss, const uint8_t *same_id, int32_t mode) {
@@ -640,6 +723,20 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus
             P->dprog_worst += ns;
         }
     }
+    P->opts = opt;
+    return plan_worklist(opt, P);
+}
+
+// Builds the scoring work list of a plan whose inputs are in place (plain
+// batches, or lane-per-pair jobs and tile items under the routing choices of opt).
+int plan_worklist(const PlanOpts &opt, pw_plan *P) {
+    const size_t total_mem = opt.total_mem;
+    const int num_cus = opt.num_cus;
+    const Scoring &sc = P->sc;
+    const int32_t nq = P->nq, nt = P->nt;
+    const int64_t n_pairs = P->n_pairs;
+    const int32_t mode = P->mode;
+    const int32_t *pq = P->pq.data(), *pt = P->pt.data(), *qlen = P->qlen.data(), *tlen = P->tlen.data();
     P->batches.clear();
     P->score_links = P->score_prog = 0;
     P->gitems.clear();
@@ -698,7 +795,7 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus
             if (L.empty()) continue;
             std::vector<int64_t> cand;
             for (int64_t i : L)
-                if (qlen[pq[i]] <= GROUP_MAX_ROWS) cand.push_back(i);
+                if (!opt.force_tile && qlen[pq[i]] <= GROUP_MAX_ROWS) cand.push_back(i);
                 else tile_pairs.push_back(i);
             std::stable_sort(cand.begin(), cand.end(),
                              [&](int64_t x, int64_t y) { return qlen[pq[x]] > qlen[pq[y]]; });
@@ -717,10 +814,11 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus
                 for (size_t u = g0; u < g1; ++u) sum += qlen[pq[cand[u]]];
                 const double fill = sum / (64.0 * std::max(1, qlen[pq[cand[g0]]]));
                 const double job_steps = (double)qlen[pq[cand[g0]]] * tl;
-                const bool keep = fill >= GROUP_MIN_FILL && job_steps <= GROUP_MAX_SHARE * per_lane;
+                const bool keep =
+                    opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
                 for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
             }
-            const int nb = group_nb(tl);
+            const int nb = (opt.force_nb > 0 && tl > 96) ? opt.force_nb : group_nb(tl, opt.tune);
             const int nch = tl > nb ? (tl + nb - 1) / nb : 1;
             for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
                 const int cnt = (int)std::min<size_t>(64, grp.size() - g0);
@@ -768,7 +866,12 @@ int plan_host(std::string &err, const Scoring &sc, size_t total_mem, int num_cus
             const int64_t i = tile_pairs[k];
             const int ql = qlen[pq[i]], tl = tlen[pt[i]];
             int tr, R;
-            choose_shape(ql, tl, tr, R);
+            if (opt.force_R > 0) {
+                tr = 0;
+                R = opt.force_R;
+            } else {
+                choose_shape(ql, tl, tr, R, opt.tune);
+            }
             const int a = tr ? tl : ql, b = tr ? ql : tl;
             const int aseq = tr ? nq + pt[i] : pq[i], bseq = tr ? pq[i] : nq + pt[i];
             titems[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b, 0, 1 << 30, -1, 0};
@@ -1602,6 +1705,14 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
     return PW_OK;
 }
 
+PlanOpts plan_opts(pw_ctx *ctx, const Scoring &sc) {
+    PlanOpts o;
+    o.total_mem = ctx->total_mem;
+    o.num_cus = ctx->num_cus;
+    o.tune = &ctx->tune[sc.affine ? 1 : 0];
+    return o;
+}
+
 int ready(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
     if (!ctx->scoring_set) {
@@ -1623,6 +1734,238 @@ int new_result(HostResult **out) {
 
 extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res);
 
+namespace {
+
+// Times the scoring pass of a synthetic job: nq queries of length A and nt
+// targets of length B over the first residues of the scoring alphabet, pairs
+// from pairs_of; returns the best of two device times in milliseconds.
+int time_synthetic(pw_ctx *ctx, const PlanOpts &opt, int nq, int A, int nt, int B,
+                   const std::vector<int32_t> &pq, const std::vector<int32_t> &pt, uint64_t &seed, double &ms) {
+    const Scoring &sc = ctx->sc;
+    const int K = std::max(1, std::min(sc.dim, 20));
+    std::vector<uint8_t> qres((size_t)nq * A + 1), tres((size_t)nt * B + 1);
+    auto rnd = [&]() {
+        seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+        return (uint32_t)(seed >> 33);
+    };
+    for (auto &c : qres) c = (uint8_t)(sc.char_offset + rnd() % K);
+    for (auto &c : tres) c = (uint8_t)(sc.char_offset + rnd() % K);
+    std::vector<int64_t> qoff(nq), toff(nt);
+    std::vector<int32_t> qlen(nq, A), tlen(nt, B);
+    for (int i = 0; i < nq; ++i) qoff[i] = (int64_t)i * A;
+    for (int i = 0; i < nt; ++i) toff[i] = (int64_t)i * B;
+    pw_plan P;
+    int rc = plan_host(ctx->err, sc, opt, &P, qres.data(), qoff.data(), qlen.data(), nq, tres.data(), toff.data(),
+                       tlen.data(), nt, pq.data(), pt.data(), (int64_t)pq.size(), nullptr, nullptr, 1);
+    if (!rc) rc = plan_upload(ctx, &P, ctx->lane.s, ctx->h_stage, true);
+    Workspace &W = P.own;
+    const Geometry g = geometry(ctx, &P);
+    if (!rc) rc = size_workspace(ctx, &P, W, g);
+    ms = 1e300;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {
+        rc = launch_clears(ctx, &P, W, ctx->lane);
+        if (!rc) rc = launch_score(ctx, &P, W, ctx->lane, g, true, nullptr);
+        if (!rc) {
+            hipError_t e = hipStreamSynchronize(ctx->lane.s);
+            if (e != hipSuccess) {
+                ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+                rc = PW_EDEVICE;
+            }
+        }
+        if (!rc && rep > 0) ms = std::min(ms, elapsed(ctx->lane.ev[0], ctx->lane.ev[1]));
+    }
+    free_plan_buffers(&P);
+    return rc;
+}
+
+// Measures the tile kernel for every length bucket and residues per lane, and
+// the chunk widths of multi-chunk lane-per-pair jobs, for the current scoring
+// scheme's gap model.
+int autotune(pw_ctx *ctx, double *seconds) {
+    const auto t0 = std::chrono::steady_clock::now();
+    Tuning tu;
+    uint64_t seed = 12345;
+    PlanOpts opt = plan_opts(ctx, ctx->sc);
+    opt.tune = nullptr;
+    for (int ia = 0; ia < TUNE_A; ++ia)
+        for (int ib = 0; ib < TUNE_B; ++ib) {
+            const int A = TUNE_REP_A[ia], B = TUNE_REP_B[ib];
+            // enough pairs to fill the device several times over, at most 8192
+            const int n = (int)std::max<int64_t>(1024, std::min<int64_t>(8192, (int64_t)3e8 / ((int64_t)A * B)));
+            std::vector<int32_t> pq(n), pt(n);
+            for (int i = 0; i < n; ++i) pq[i] = pt[i] = i;
+            for (int k = 0; k < TUNE_R; ++k) {
+                opt.force_tile = true;
+                opt.force_R = TUNE_RS[k];
+                double ms = 0;
+                int rc = time_synthetic(ctx, opt, n, A, n, B, pq, pt, seed, ms);
+                if (rc) return rc;
+                tu.rate[ia][ib][k] = (float)((double)n * A * B / (ms * 1e3));
+            }
+        }
+    // multi-chunk lane-per-pair jobs: 512 targets of 1024 residues, each with 64 queries of 384
+    {
+        const int NT = 512, A = 384, B = 1024;
+        std::vector<int32_t> pq, pt;
+        for (int t = 0; t < NT; ++t)
+            for (int q = 0; q < 64; ++q) {
+                pq.push_back(t * 64 + q);
+                pt.push_back(t);
+            }
+        const int widths[5] = {32, 48, 64, 80, 96};
+        opt = plan_opts(ctx, ctx->sc);
+        opt.tune = nullptr;
+        opt.force_group = true;
+        for (int w = 0; w < 5; ++w) {
+            opt.force_nb = widths[w];
+            double ms = 0;
+            int rc = time_synthetic(ctx, opt, NT * 64, A, NT, B, pq, pt, seed, ms);
+            if (rc) return rc;
+            tu.group_rate[w] = (float)((double)NT * 64 * A * B / (ms * 1e3));
+        }
+        // the built-in width 64 unless another is clearly faster
+        float best = tu.group_rate[2];
+        tu.group_nb_long = 64;
+        for (int w = 0; w < 5; ++w)
+            if (tu.group_rate[w] > TUNE_MARGIN * tu.group_rate[2] && tu.group_rate[w] > best) {
+                best = tu.group_rate[w];
+                tu.group_nb_long = widths[w];
+            }
+    }
+    tu.valid = true;
+    ctx->tune[ctx->sc.affine ? 1 : 0] = tu;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PW_OK;
+}
+
+// Plan-level tuning: the plan's own scoring pass timed under a few routing
+// choices (the longest lane-per-pair job as a share of the per-lane work, the
+// chunk width of multi-chunk lane-per-pair jobs, tile shapes from the measured
+// table or the cost model); the fastest work list is kept.
+int tune_plan(pw_ctx *ctx, pw_plan *P, double *seconds, double *best_ms, int32_t *choice) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (P->mode != 1) {
+        if (seconds) *seconds = 0.0;
+        if (best_ms) *best_ms = 0.0;
+        return PW_OK;
+    }
+    struct Cand {
+        double share;
+        int nb;
+        bool table;
+    };
+    const Tuning *tu = &ctx->tune[P->sc.affine ? 1 : 0];
+    std::vector<Cand> cands;
+    for (double share : {0.125, 0.25, 0.5})
+        for (int nb : {64, 96})
+            for (int table = 0; table < (tu->valid ? 2 : 1); ++table) cands.push_back(Cand{share, nb, table != 0});
+    auto opts_of = [&](const Cand &c) {
+        PlanOpts o = plan_opts(ctx, P->sc);
+        o.group_max_share = c.share;
+        o.force_nb = c.nb;
+        o.tune = c.table ? tu : nullptr;
+        return o;
+    };
+    double best = 1e300;
+    size_t bi = 0;
+    Lane &L = ctx->lane;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        int rc = plan_worklist(opts_of(cands[i]), P);
+        if (!rc) rc = plan_upload(ctx, P, L.s, ctx->h_stage, true);
+        const Geometry g = geometry(ctx, P);
+        if (!rc) rc = size_workspace(ctx, P, P->own, g);
+        double ms = 1e300;
+        for (int rep = 0; rep < 3 && !rc; ++rep) {
+            rc = launch_clears(ctx, P, P->own, L);
+            if (!rc) rc = launch_score(ctx, P, P->own, L, g, true, nullptr);
+            if (!rc) {
+                hipError_t e = hipStreamSynchronize(L.s);
+                if (e != hipSuccess) {
+                    ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+                    rc = PW_EDEVICE;
+                }
+            }
+            if (!rc && rep > 0) ms = std::min(ms, elapsed(L.ev[0], L.ev[1]));
+        }
+        if (rc) return rc;
+        if (ms < best) {
+            best = ms;
+            bi = i;
+        }
+    }
+    int rc = plan_worklist(opts_of(cands[bi]), P);
+    if (!rc) rc = plan_upload(ctx, P, L.s, ctx->h_stage, true);
+    if (rc) return rc;
+    if (P->graph) {
+        (void)hipGraphExecDestroy(P->graph);
+        P->graph = nullptr;
+        P->graph_sig.clear();
+    }
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (best_ms) *best_ms = best;
+    if (choice) {
+        choice[0] = (int32_t)(cands[bi].share * 1000.0 + 0.5);
+        choice[1] = cands[bi].nb;
+        choice[2] = cands[bi].table ? 1 : 0;
+    }
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" int pw_tune_plan(pw_ctx *ctx, pw_plan *plan, double *seconds, double *score_ms, int32_t *choice) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!plan) {
+        ctx->err = "pw_tune_plan: missing plan";
+        return PW_EINVAL;
+    }
+    return tune_plan(ctx, plan, seconds, score_ms, choice);
+}
+
+extern "C" int pw_autotune(pw_ctx *ctx, double *seconds) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    return autotune(ctx, seconds);
+}
+
+extern "C" int pw_get_tuning(pw_ctx *ctx, int32_t affine, float *rates, float *group_rates, int32_t *group_nb_long,
+                             int32_t *valid) {
+    if (!ctx || !rates || !group_rates || !group_nb_long || !valid) return PW_EINVAL;
+    const Tuning &tu = ctx->tune[affine ? 1 : 0];
+    std::memcpy(rates, tu.rate, sizeof tu.rate);
+    std::memcpy(group_rates, tu.group_rate, sizeof tu.group_rate);
+    *group_nb_long = tu.group_nb_long;
+    *valid = tu.valid ? 1 : 0;
+    return PW_OK;
+}
+
+extern "C" int pw_set_tuning(pw_ctx *ctx, int32_t affine, const float *rates, int32_t group_nb_long) {
+    if (!ctx) return PW_EINVAL;
+    if (!rates) {
+        ctx->tune[affine ? 1 : 0] = Tuning();  // back to the cost model
+        return PW_OK;
+    }
+    if (group_nb_long != 32 && group_nb_long != 48 && group_nb_long != 64 && group_nb_long != 80 &&
+        group_nb_long != 96) {
+        ctx->err = "pw_set_tuning: chunk width must be 32, 48, 64, 80 or 96";
+        return PW_EINVAL;
+    }
+    Tuning tu;
+    std::memcpy(tu.rate, rates, sizeof tu.rate);
+    for (int a = 0; a < TUNE_A; ++a)
+        for (int b = 0; b < TUNE_B; ++b)
+            for (int k = 0; k < TUNE_R; ++k)
+                if (!std::isfinite(tu.rate[a][b][k]) || tu.rate[a][b][k] < 0.0f) {
+                    ctx->err = "pw_set_tuning: rates must be finite and non-negative";
+                    return PW_EINVAL;
+                }
+    tu.group_nb_long = group_nb_long;
+    tu.valid = true;
+    ctx->tune[affine ? 1 : 0] = tu;
+    return PW_OK;
+}
+
 extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
                           const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
                           const int32_t *pq, const int32_t *pt, int64_t n_pairs, const float *min_start_score,
@@ -1635,7 +1978,7 @@ extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff,
     }
     *out = nullptr;
     pw_plan *P = new pw_plan();
-    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, ctx->num_cus, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+    rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    min_start_score, same_id, mode);
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     if (rc) {
@@ -1694,7 +2037,7 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         return PW_EINVAL;
     }
     pw_plan *P = new pw_plan();
-    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, ctx->num_cus, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+    rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    nullptr, nullptr, mode);
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     Workspace &W = ctx->ws;
@@ -1739,7 +2082,7 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     }
     *out = nullptr;
     pw_plan *P = new pw_plan();
-    rc = plan_host(ctx->err, ctx->sc, ctx->total_mem, ctx->num_cus, P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
+    rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    min_start_score, same_id, mode);
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     if (!rc) {

@@ -182,7 +182,7 @@ int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, in
     if (qres.empty()) qres.push_back(0);
     if (tres.empty()) tres.push_back(0);
     pw_plan *P = ln.P;
-    int rc = plan_host(ctx->err, sc, ctx->total_mem, ctx->num_cus, P, qres.data(), qoff.data(), qlen.data(), (int32_t)qlen.size(),
+    int rc = plan_host(ctx->err, sc, plan_opts(ctx, sc), P, qres.data(), qoff.data(), qlen.data(), (int32_t)qlen.size(),
                        tres.data(), toff.data(), tlen.data(), (int32_t)tlen.size(), pq.data(), pt.data(),
                        (int64_t)pq.size(), mss.data(), same.data(), 1);
     if (rc) return rc;
