@@ -206,6 +206,7 @@ struct pw_plan {
     int64_t units_worst = 0;   // direction-pass work units if every pair passed the gate
     int64_t dlinks_worst = 0;  // boundary-row floats of the direction pass in that case
     int64_t dprog_worst = 0;   // and its progress counters
+    int32_t dir_rmax = 8;      // largest residues per lane of the direction pass
     int64_t n_groups = 0;      // lane-per-pair jobs
     int64_t group_rows = 0;    // longest row sequence of a multi-chunk job (scratch sizing)
     int32_t max_nb = 16;
@@ -708,11 +709,27 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
     P->dir_worst = 0;
     P->dir_largest = 0;
     P->units_worst = P->dlinks_worst = P->dprog_worst = 0;
+    // residues per lane of the direction pass: the largest of 8, 4, 2 for
+    // which the pairs, all gated, would give at least 4 work units per SIMD
+    // (more, shorter stripes for few long pairs: their waves run latency-bound)
+    P->dir_rmax = 2;
+    for (int r : {8, 4}) {
+        int64_t units = 0;
+        for (int64_t i = 0; i < n_pairs; ++i) {
+            const int ql = qlen[pq[i]], tl = tlen[pt[i]];
+            const int ns = split_stripes(ql, tl, dir_residues(ql, r));
+            units += ns > 0 ? ns : 1;
+        }
+        if (units >= (int64_t)std::max(1, opt.num_cus) * 4 * 4) {
+            P->dir_rmax = r;
+            break;
+        }
+    }
     for (int64_t i = 0; i < n_pairs; ++i) {
         const int ql = qlen[pq[i]], tl = tlen[pt[i]];
         P->cells += (double)ql * (double)tl;
         P->max_t = std::max<int64_t>(P->max_t, tl);
-        const int R = dir_residues(ql);
+        const int R = dir_residues(ql, P->dir_rmax);
         const int64_t a = dir_area(ql, tl, R);
         P->dir_worst += a;
         P->dir_largest = std::max(P->dir_largest, a);
@@ -1203,7 +1220,7 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
                            (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
                            (const SeqRef *)P->seqs.p, P->nq, P->n_pairs, P->sc.minimum, P->sc.ll, ctl,
                            (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
-                           W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap);
+                           W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax);
         HIPCHK(ctx, hipGetLastError());
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));

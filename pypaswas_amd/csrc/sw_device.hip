@@ -115,6 +115,8 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     const float bJ = AFF ? G : 0.0f;  // J candidate entering row 1: max(-1e10 + E, 0 + G)
     const int64_t row_floats = link_row_floats(blen_it);
     if (s_last > nstripes) s_last = nstripes;
+    // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
+    const float thr = __builtin_fmaxf(t1, mp);
 
     for (int s = s_first; s < s_last; ++s) {
         int a4[R];
@@ -179,15 +181,13 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                     rmax = __builtin_fmaxf(rmax, T);
                     if (DIRS) {
                         // default_main_cpu.cl:283-345: source = M, I, J in that
-                        // priority among equal positive values, none when 0
-                        uint32_t code = T > 0.0f ? (T == M ? 1u : (T == I ? 2u : 3u)) : 0u;
-                        uint32_t b = code;
-                        if (code == 1u && md == 0.0f) b |= DB_STOP;
-                        if (code == 1u) {
-                            if (M >= t1 && M >= mp) b |= DB_CAND;
-                            if (-M >= t1 && -M >= mp) b |= DB_CANDM;
-                        }
-                        bits[r] = b;
+                        // priority among equal positive values, none when 0; a
+                        // main-matrix cell stops on a zero diagonal predecessor
+                        // and is a candidate start when M (or -M, for a marked
+                        // cell) reaches both thresholds
+                        const uint32_t main = DB_MAIN | (md == 0.0f ? DB_STOP : 0u) | (M >= thr ? DB_CAND : 0u) |
+                                              (M <= -thr ? DB_CANDM : 0u);
+                        bits[r] = T > 0.0f ? (T == M ? main : (T == I ? 2u : 3u)) : 0u;
                     }
                     const float jout = __builtin_fmaxf(jin + E, M + G);
                     td = Tp[r];
@@ -207,21 +207,18 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         tup = h;
                     } else {
                         // default_main_cpu.cl:162-177: LEFT (x-1), UP (y-1), UL,
-                        // first strict improvement over 0 wins
+                        // first strict improvement over 0 wins: the value is
+                        // max(0, l, u, d) and the code that of the first of
+                        // l, u, d equal to it (when it is positive)
                         const float l = tup + G, u = Tp[r] + G;
-                        float cur = 0.0f;
-                        uint32_t code = 0u;
-                        if (cur < l) { cur = l; code = 3u; }
-                        if (cur < u) { cur = u; code = 2u; }
-                        if (cur < d) { cur = d; code = 1u; }
-                        uint32_t b = code;
-                        const float pv = code == 1u ? hd : (code == 3u ? tup : Tp[r]);
-                        if (code != 0u && pv == 0.0f) b |= DB_STOP;
-                        if (code == 1u) {
-                            if (cur >= t1 && cur >= mp) b |= DB_CAND;
-                            if (-cur >= t1 && -cur >= mp) b |= DB_CANDM;
-                        }
-                        bits[r] = b;
+                        const float cur = fmax3(__builtin_fmaxf(l, u), d, 0.0f);
+                        // STOP when the predecessor along the code is 0; only
+                        // diagonal cells are candidate starts
+                        const bool isl = cur == l, isu = cur == u;
+                        const float pv = isl ? tup : (isu ? Tp[r] : hd);
+                        const uint32_t diag = DB_MAIN | (cur >= thr ? DB_CAND : 0u) | (cur <= -thr ? DB_CANDM : 0u);
+                        const uint32_t b = (isl ? 3u : (isu ? 2u : diag)) | (pv == 0.0f ? DB_STOP : 0u);
+                        bits[r] = cur > 0.0f ? b : 0u;
                         rmax = __builtin_fmaxf(rmax, cur);
                         td = Tp[r];
                         Tp[r] = cur;
@@ -583,9 +580,12 @@ __global__ void __launch_bounds__(256) clear_kernel(uint32_t *__restrict__ p, in
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-__device__ __host__ __forceinline__ int dir_residues(int qlen) {
+// Query residues per lane of the direction pass: the smallest power of two
+// whose 64-lane stripe covers the query, at most rmax (a plan lowers rmax when
+// its pairs would otherwise give too few stripe units to fill the device).
+__device__ __host__ __forceinline__ int dir_residues(int qlen, int rmax = 8) {
     int r = 1;
-    while (r < 8 && WAVE * r < qlen) r *= 2;
+    while (r < rmax && WAVE * r < qlen) r *= 2;
     return r;
 }
 
@@ -621,7 +621,7 @@ __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ max
                                                    PairTask *__restrict__ tasks, int64_t dir_cap,
                                                    int64_t cand_cap_total, int cand0, DirUnit *__restrict__ units,
                                                    int64_t units_cap, int64_t link_cap,
-                                                   unsigned *__restrict__ prog, int64_t prog_cap) {
+                                                   unsigned *__restrict__ prog, int64_t prog_cap, int rmax) {
     const int lane = threadIdx.x & 63;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = base + threadIdx.x;
@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ max
         t.pair = (int32_t)i;
         t.q_seq = pr.x;
         t.t_seq = nq + pr.y;
-        t.R = dir_residues(ql);
+        t.R = dir_residues(ql, rmax);
         const int64_t area = dir_area(ql, tl, t.R);
         const unsigned long long doff = atomicAdd(&ctl->dir_used, (unsigned long long)area);
         const unsigned long long coff = atomicAdd(&ctl->cand_used, (unsigned long long)cand0);

@@ -138,6 +138,6 @@ struct HitRec {
 //   bit 4     candidate start with the sign-marked value
 //   bit 5     marked by the traceback
 //   bit 6     consumed by the walk (IN_ALIGNMENT)
-constexpr uint8_t DB_STOP = 4, DB_CAND = 8, DB_CANDM = 16, DB_MARK = 32, DB_WALK = 64;
+constexpr uint8_t DB_MAIN = 1, DB_STOP = 4, DB_CAND = 8, DB_CANDM = 16, DB_MARK = 32, DB_WALK = 64;
 
 }  // namespace pw
