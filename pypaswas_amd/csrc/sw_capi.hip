@@ -73,8 +73,8 @@ struct Limits {
 // Device work areas of one execution, plus the pinned host staging of the
 // read-back.  want_* are the sizes the next execution allocates.
 struct Workspace {
-    DevBuf ctl, maxima, tasks, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch, dscratch,
-        units, dlinks, dprog;
+    DevBuf ctl, maxima, gated, tasks, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch,
+        dscratch, units, dlinks, dprog;
     int64_t want_dir = 0, want_cand = 0, want_ops = 0;
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
@@ -176,6 +176,7 @@ struct Geometry {
     int tile_grid = 0;
     int64_t tscr_stride = 0;  // floats per workgroup
     int gate_grid = 0;
+    int build_grid = 0;  // workgroups of 256 threads of the task builder
     int dir_grid = 0;
     int64_t dscr_stride = 0;
     int task_grid = 0;  // workgroups of 64 threads of the start-selection and walk kernels
@@ -325,7 +326,7 @@ void free_pinned(PinBuf &b) {
 }
 
 void free_workspace(Workspace &W) {
-    DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
+    DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
                       &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks, &W.dprog};
     for (DevBuf *b : bufs) free_buf(*b);
     PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_sps, &W.h_hits, &W.h_ops};
@@ -1023,7 +1024,9 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
         g.tscr_stride = 3 * (P->max_b + 64);
     }
     const int64_t n = P->n_pairs;
-    g.gate_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
+    // gate: contiguous ranges of at least 2048 pairs per workgroup (one atomic each)
+    g.gate_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 2047) / 2048, (int64_t)ctx->num_cus * 4));
+    g.build_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 4));
     // persistent direction-pass workers: as many as the units of every pair
     // passing the gate could use (one per stripe of a split pair), up to the
     // resident limit; split stripes need their producer stripes running
@@ -1049,6 +1052,7 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     ENSURE(ctx, W.ctl, sizeof(Ctl));
     ENSURE(ctx, W.maxima, (size_t)n * sizeof(float));
     ENSURE(ctx, W.tasks, (size_t)n * sizeof(PairTask));
+    ENSURE(ctx, W.gated, (size_t)n * sizeof(int));
     if (W.dir_ext) {
         W.dir_cap = W.dir_ext_cap;
     } else {
@@ -1217,8 +1221,11 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
     if (rc) return rc;
     if (P->n_pairs > 0) {
         hipLaunchKernelGGL(gate_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
+                           (const float *)P->d_mss.p, P->n_pairs, P->sc.minimum, ctl, (int *)W.gated.p);
+        HIPCHK(ctx, hipGetLastError());
+        hipLaunchKernelGGL(task_kernel, dim3(g.build_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
                            (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
-                           (const SeqRef *)P->seqs.p, P->nq, P->n_pairs, P->sc.minimum, P->sc.ll, ctl,
+                           (const SeqRef *)P->seqs.p, P->nq, P->sc.ll, ctl, (const int *)W.gated.p,
                            (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
                            W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax);
         HIPCHK(ctx, hipGetLastError());
@@ -1390,7 +1397,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         Ctl c;
         std::memset(&c, 0, sizeof c);
         c.ngated = nc;
-        c.dir_units = nc;
+        c.units_used = nc;
         HIPCHK(ctx, hipMemcpyAsync(W.tasks.p, chunk.data(), (size_t)nc * sizeof(PairTask), hipMemcpyHostToDevice,
                                    L.s));
         HIPCHK(ctx, hipMemcpyAsync(W.units.p, units.data(), (size_t)nc * sizeof(DirUnit), hipMemcpyHostToDevice,
@@ -1528,7 +1535,7 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
                                       double minimum) {
     std::vector<uint64_t> s;
     auto add = [&](uint64_t v) { s.push_back(v); };
-    const DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
+    const DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
                             &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks,
                             &W.dprog, &P->d_links, &P->d_prog};
     for (const DevBuf *b : bufs) {

@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
     uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
     float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
-    const int n = (int)min((int64_t)__hip_atomic_load(&ctl->dir_units, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+    const int n = (int)min((int64_t)__hip_atomic_load(&ctl->units_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
                            units_cap);
     for (;;) {
         unsigned v = 0;
@@ -607,91 +607,151 @@ __device__ __host__ __forceinline__ int split_stripes(int alen, int blen, int R)
 
 // Traceback gate of default_main_cpu.cl:408-410 (maximum >= MINIMUM_SCORE)
 // plus the pair's minimum start score, without which no cell can start
-// (:427).  Every passing pair gets a PairTask with its direction and
-// candidate areas taken from the bump allocators in ctl, and its direction-
-// pass work units (one per stripe for a split pair, whose boundary rows and
-// zeroed progress counters are allocated here too; without room it runs
-// unsplit).  A pair that does not fit is marked TASK_DEFERRED for a later
-// round.  One task-slot atomic per wave.
+// (:427): the indices of the passing pairs are compacted into `gated`.
+// Passing pairs are sparse, so each workgroup counts its contiguous range
+// first, takes its output range with one atomic, then writes (one LDS atomic
+// per wave for the slots inside the range).
 __global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
+                                                   int64_t n, float minimum, Ctl *__restrict__ ctl,
+                                                   int *__restrict__ gated) {
+    __shared__ int count, base;
+    const int lane = threadIdx.x & 63;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool pass = i < hi && maxima[i] >= minimum && maxima[i] >= mss[i];
+        const uint64_t mask = __ballot(pass);
+        if (lane == 0 && mask) atomicAdd(&count, __popcll(mask));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        base = count ? atomicAdd(&ctl->ngated, count) : 0;
+        count = 0;
+    }
+    __syncthreads();
+    for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool pass = i < hi && maxima[i] >= minimum && maxima[i] >= mss[i];
+        const uint64_t mask = __ballot(pass);
+        if (!mask) continue;
+        int first = 0;
+        if (lane == 0) first = atomicAdd(&count, __popcll(mask));
+        first = __shfl(first, 0, 64);
+        if (pass) gated[base + first + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+    }
+}
+
+// Exclusive prefix sum over the wave of a 64-bit amount, and the wave total.
+__device__ __forceinline__ uint64_t wave_excl_scan(uint64_t v, uint64_t &total) {
+    const int lane = threadIdx.x & 63;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// One amount per lane from a shared counter: one atomic per wave.
+__device__ __forceinline__ unsigned long long wave_alloc(unsigned long long *counter, uint64_t amount) {
+    uint64_t total = 0;
+    const uint64_t off = wave_excl_scan(amount, total);
+    unsigned long long b = 0;
+    if ((threadIdx.x & 63) == 0 && total) b = atomicAdd(counter, (unsigned long long)total);
+    b = __shfl(b, 0, 64);
+    return b + off;
+}
+
+// Builds the task of every gated pair (slot = its position in `gated`), with
+// its direction and candidate areas taken from the bump allocators in ctl,
+// and its direction-pass work units (one per stripe for a split pair, whose
+// boundary rows and zeroed progress counters are allocated here too; without
+// room it runs unsplit).  A pair that does not fit is marked TASK_DEFERRED for
+// a later round.  Allocations are summed over the wave: one atomic per wave
+// and counter.
+__global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
                                                    const int2 *__restrict__ pairs,
                                                    const uint8_t *__restrict__ same_id,
-                                                   const SeqRef *__restrict__ seqs, int nq, int64_t n,
-                                                   float minimum, float ll, Ctl *__restrict__ ctl,
+                                                   const SeqRef *__restrict__ seqs, int nq, float ll,
+                                                   Ctl *__restrict__ ctl, const int *__restrict__ gated,
                                                    PairTask *__restrict__ tasks, int64_t dir_cap,
                                                    int64_t cand_cap_total, int cand0, DirUnit *__restrict__ units,
                                                    int64_t units_cap, int64_t link_cap,
                                                    unsigned *__restrict__ prog, int64_t prog_cap, int rmax) {
-    const int lane = threadIdx.x & 63;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        bool pass = false;
-        float m = 0.0f;
-        if (i < n) {
-            m = maxima[i];
-            pass = m >= minimum && m >= mss[i];
-        }
-        const uint64_t mask = __ballot(pass);
-        if (mask == 0) continue;
-        int first = 0;
-        if (lane == 0) first = atomicAdd(&ctl->ngated, __popcll(mask));
-        first = __shfl(first, 0, 64);
-        if (!pass) continue;
-        const int slot = first + __popcll(mask & ((1ull << lane) - 1ull));
-        const int2 pr = pairs[i];
-        const int ql = seqs[pr.x].len, tl = seqs[nq + pr.y].len;
+    const int ng = __hip_atomic_load(&ctl->ngated, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int stride = gridDim.x * blockDim.x;
+    // whole waves iterate together (the scans need every lane)
+    for (int g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); g0 < ng; g0 += stride) {
+        const int g = g0 + (threadIdx.x & 63);
+        const bool on = g < ng;
         PairTask t;
-        t.pair = (int32_t)i;
-        t.q_seq = pr.x;
-        t.t_seq = nq + pr.y;
-        t.R = dir_residues(ql, rmax);
-        const int64_t area = dir_area(ql, tl, t.R);
-        const unsigned long long doff = atomicAdd(&ctl->dir_used, (unsigned long long)area);
-        const unsigned long long coff = atomicAdd(&ctl->cand_used, (unsigned long long)cand0);
+        int ql = 0, tl = 0, ns = 0;
+        int64_t area = 0;
+        float m = 0.0f;
+        if (on) {
+            const int i = gated[g];
+            const int2 pr = pairs[i];
+            ql = seqs[pr.x].len;
+            tl = seqs[nq + pr.y].len;
+            m = maxima[i];
+            t.pair = i;
+            t.q_seq = pr.x;
+            t.t_seq = nq + pr.y;
+            t.R = dir_residues(ql, rmax);
+            area = dir_area(ql, tl, t.R);
+            ns = split_stripes(ql, tl, t.R);
+            t.ops_off = 0;
+            t.nops = 0;
+            t.t1 = ll * m;
+            t.mp = mss[i];
+            t.maxv = m;
+            t.cand_cap = cand0;
+            t.ncand = 0;
+            t.nsp = 0;
+            t.same_id = same_id[i];
+            t.qidx = pr.x;
+            t.tidx = pr.y;
+            t.out_off = 0;
+            t.split = 0;
+            t.rows_off = 0;
+            t.prog_off = 0;
+        }
+        const unsigned long long doff = wave_alloc(&ctl->dir_used, on ? (uint64_t)area : 0ull);
+        const unsigned long long coff = wave_alloc(&ctl->cand_used, on ? (uint64_t)cand0 : 0ull);
+        const bool ok = on && (int64_t)doff + area <= dir_cap && (int64_t)coff + cand0 <= cand_cap_total;
+        const int64_t rows = (ok && ns > 0) ? (int64_t)(ns - 1) * link_row_floats(tl) : 0;
+        const unsigned long long lo = wave_alloc(&ctl->link_used, (uint64_t)rows);
+        const unsigned long long po = wave_alloc(&ctl->prog_used, (ok && ns > 0) ? (uint64_t)ns : 0ull);
+        const bool split = ok && ns > 0 && (int64_t)lo + rows <= link_cap && (int64_t)po + ns <= prog_cap;
+        const int cnt = ok ? (split ? ns : 1) : 0;
+        const unsigned long long k0 = wave_alloc(reinterpret_cast<unsigned long long *>(&ctl->units_used),
+                                                 (uint64_t)cnt);
+        if (!on) continue;
         t.dir_off = (int64_t)doff;
         t.cand_off = (int64_t)coff;
-        t.ops_off = 0;
-        t.nops = 0;
-        t.t1 = ll * m;
-        t.mp = mss[i];
-        t.maxv = m;
-        t.cand_cap = cand0;
-        t.ncand = 0;
-        t.nsp = 0;
-        t.same_id = same_id[i];
-        t.status = ((int64_t)doff + area <= dir_cap && (int64_t)coff + cand0 <= cand_cap_total) ? TASK_OK
-                                                                                                : TASK_DEFERRED;
-        t.qidx = pr.x;
-        t.tidx = pr.y;
-        t.out_off = 0;
-        t.split = 0;
-        t.rows_off = 0;
-        t.prog_off = 0;
-        const int ns = split_stripes(ql, tl, t.R);
-        if (t.status == TASK_OK && ns > 0) {
-            const int64_t rows = (int64_t)(ns - 1) * link_row_floats(tl);
-            const unsigned long long lo = atomicAdd(&ctl->link_used, (unsigned long long)rows);
-            const unsigned long long po = atomicAdd(&ctl->prog_used, (unsigned long long)ns);
-            if ((int64_t)lo + rows <= link_cap && (int64_t)po + ns <= prog_cap) {
-                t.split = 1;
-                t.rows_off = (int64_t)lo;
-                t.prog_off = (int64_t)po;
-                for (int k = 0; k < ns; ++k) prog[po + k] = 0u;
-            }
+        t.status = ok ? TASK_OK : TASK_DEFERRED;
+        if (split) {
+            t.split = 1;
+            t.rows_off = (int64_t)lo;
+            t.prog_off = (int64_t)po;
+            for (int k = 0; k < ns; ++k) prog[po + k] = 0u;
         }
-        if (t.status == TASK_OK) {
-            const int cnt = t.split ? ns : 1;
-            const int k0 = atomicAdd(&ctl->dir_units, cnt);
+        if (ok) {
             if ((int64_t)k0 + cnt <= units_cap) {
-                for (int k = 0; k < cnt; ++k) units[k0 + k] = DirUnit{slot, t.split ? k : -1};
+                for (int k = 0; k < cnt; ++k) units[k0 + k] = DirUnit{g, split ? k : -1};
             } else {
-                // no unit slots (the host sizes them for every pair): leave the
-                // reserved units as no-ops and finish the pair in a tail round
+                // no unit slots (the host sizes them for every pair): finish the
+                // pair in a tail round; its in-range slots become no-ops
                 t.status = TASK_DEFERRED;
-                for (int k = 0; k < cnt && (int64_t)k0 + k < units_cap; ++k) units[k0 + k] = DirUnit{slot, -1};
+                for (int k = 0; k < cnt && (int64_t)k0 + k < units_cap; ++k) units[k0 + k] = DirUnit{g, -1};
             }
         }
-        tasks[slot] = t;
+        tasks[g] = t;
     }
 }
 

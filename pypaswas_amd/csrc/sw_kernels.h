@@ -103,11 +103,10 @@ constexpr int NB_KINDS = 6;
 struct Ctl {
     unsigned queue[16 * (1 + NB_KINDS)];
     int ngated;
-    int dir_units;       // work units of the direction pass
+    unsigned dir_queue;  // next direction-pass unit
     unsigned long long dir_used, cand_used, ops_used, out_used;
     unsigned long long link_used, prog_used;  // split-pair areas of the direction pass
-    unsigned dir_queue;  // next direction-pass unit
-    unsigned pad0;
+    long long units_used;                     // work units of the direction pass
 };
 
 struct Cand {
