@@ -28,9 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS batched-persistent vs plain launch, bit-exact hits; p50/p99 latency"
-# Published FP32 vector peak of MI355X, 157.3 TFLOP/s with an FMA counted as 2: 78.6 T lane-ops/s
-# (256 CUs x 128 FP32 lanes/clk x 2.4 GHz).  The score pass issues adds and maxima, one op each.
-VALU_PEAK_TOPS = 157.3 / 2
+# Published FP32 peak of MI355X: 157.3 TFLOP/s, an FMA counted as 2 ops.  The score pass issues FP32 adds and
+# maxima (one op each; v_max3 does two), never an FMA.  Its instruction-issue ceiling is half that figure,
+# 78.6 T lane-instructions/s (256 CUs x 128 FP32 lanes/clk x 2.4 GHz), reported beside it.
+FP32_PEAK_TFLOPS = 157.3
+VALU_ISSUE_PEAK_TOPS = 157.3 / 2
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "score_config2_traffic.txt")
 OPS_PER_CELL = {"linear": 7, "affine": 17}
 
@@ -172,8 +174,10 @@ def main():
     if args.workload == "config2" and os.path.exists(TRAFFIC_FILE):
         # HBM bytes per score pass from the committed FETCH_SIZE / WRITE_SIZE runs (see the file for the command)
         traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_pass")
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic,
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "issue_peak": round(VALU_ISSUE_PEAK_TOPS, 2),
+                "issue_frac": round(achieved / VALU_ISSUE_PEAK_TOPS, 4),
                 "kernel": "score pass (group_kernel<%s,NB> + score_kernel<%s>)" % (
                     ("true",) * 2 if kind == "affine" else ("false",) * 2),
                 "ops_per_cell": OPS_PER_CELL[kind], "cells_per_launch": cells,

@@ -107,9 +107,11 @@ bool same_key(const ServeJob *a, const ServeJob *b) { return a->scheme == b->sch
 
 bool is_large(const ServeJob *j) { return j->cells > LARGE_JOB_CELLS; }
 
-// Lane 0 takes any job; the other lanes (when there are several) take small
-// jobs only.
-bool lane_takes(size_t lane, size_t nlanes, const ServeJob *j) { return lane == 0 || nlanes == 1 || !is_large(j); }
+// Lane 0 takes any job, the others small jobs only.
+size_t large_lanes(size_t nlanes) { return nlanes < 1 ? 0 : 1; }
+bool lane_takes(size_t lane, size_t nlanes, const ServeJob *j) { return lane < large_lanes(nlanes) || !is_large(j); }
+// arena allocator of a lane: large-capable lanes share ra[0]
+int lane_pool(size_t lane, size_t nlanes) { return lane < large_lanes(nlanes) ? 0 : 1; }
 
 }  // namespace
 
@@ -257,7 +259,7 @@ int finish_batch(pw_server *S, ServeLane &ln) {
     // the lent range goes back to the arena; the plan keeps its own buffers for the next batch
     ln.P->own.dir_ext = nullptr;
     ln.P->own.dir_ext_cap = 0;
-    S->ra[&ln == &S->lanes[0] ? 0 : 1].give(ln.dir_off, ln.dir_len);
+    S->ra[lane_pool((size_t)(&ln - &S->lanes[0]), S->lanes.size())].give(ln.dir_off, ln.dir_len);
     S->reserved -= ln.dir_len;
     ln.jobs.clear();
     ln.busy = false;
@@ -267,10 +269,12 @@ int finish_batch(pw_server *S, ServeLane &ln) {
 
 // Admits queued jobs onto idle lanes while the arena has room.
 int admit(pw_server *S) {
-    // lanes 1.. first, so that lane 0 stays free for large jobs when small ones suffice
+    // small-job lanes first, so that the large-capable lanes stay free for
+    // large jobs when the others suffice
     const size_t nl = S->lanes.size();
+    const size_t nlarge = large_lanes(nl);
     for (size_t li = 0; li < nl; ++li) {
-        const size_t lane = (li + 1) % nl;
+        const size_t lane = (li + nlarge) % nl;
         ServeLane &ln = S->lanes[lane];
         if (ln.busy) continue;
         if (S->queue.empty()) break;
@@ -281,7 +285,7 @@ int admit(pw_server *S) {
                 break;
             }
         if (!head) continue;
-        RangeAlloc &ra = S->ra[lane == 0 ? 0 : 1];
+        RangeAlloc &ra = S->ra[lane_pool(lane, nl)];
         const int64_t largest = ra.largest();
         const int64_t cap = ra.cap;
         // a job larger than the whole arena runs alone once the arena is free;
