@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 import threading
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -166,6 +167,13 @@ assert SP_DTYPE.itemsize == ctypes.sizeof(StartingPointRec) == 36
 assert HIT_DTYPE.itemsize == ctypes.sizeof(HitRec)
 
 
+def _release_result(dev_ref, lib, out):
+    """Hands a result back to the library (its pinned buffers are recycled by a live device)."""
+    dev = dev_ref()
+    ctx = dev.ctx if dev is not None and dev.ctx else None
+    lib.pw_free_result(ctx, out)
+
+
 def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data
 
@@ -235,10 +243,19 @@ class Device:
         return self._take_result(out)
 
     def _take_result(self, out) -> AlignResult:
+        free_now = True
         try:
             r = out.contents
             n = int(r.n_pairs)
-            maxima = np.ctypeslib.as_array(r.maxima, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+            if n:
+                # a view of the library's (pinned) maxima buffer, not a copy: the result
+                # goes back to the library when the array is collected
+                maxima = np.ctypeslib.as_array(r.maxima, shape=(n,))
+                fin = weakref.finalize(maxima, _release_result, weakref.ref(self), self.lib, out)
+                fin.atexit = False
+                free_now = False
+            else:
+                maxima = np.zeros(0, np.float32)
             nsp = int(r.n_starting_points)
             sps = (np.frombuffer(ctypes.string_at(r.starting_points, nsp * SP_DTYPE.itemsize), dtype=SP_DTYPE).copy()
                    if nsp else np.zeros(0, SP_DTYPE))
@@ -251,7 +268,8 @@ class Device:
                                float(r.ms_walk), float(r.ms_total), int(r.score_launches), float(r.ms_readback),
                                float(r.ms_tail), float(r.ms_assemble))
         finally:
-            self.lib.pw_free_result(self.ctx, out)
+            if free_now:
+                self.lib.pw_free_result(self.ctx, out)
 
     def prepare(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                 min_start_score: Optional[np.ndarray], same_id: Optional[np.ndarray], mode: int = 1) -> "Plan":

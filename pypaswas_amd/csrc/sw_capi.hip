@@ -59,6 +59,7 @@ struct Lane {
     hipStream_t s = nullptr, s2 = nullptr;
     hipEvent_t ev[6] = {};
     hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork2 = nullptr, join2 = nullptr;  // maxima copy on s2 beside the traceback stages
 };
 
 // Work-area sizing rules (pw_set_limits).
@@ -343,6 +344,8 @@ int create_lane(Lane &L, std::string &err, bool single = false) {
     for (auto &ev : L.ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fork2, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.join2, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&L.join, hipEventDisableTiming);
     if (e != hipSuccess) {
         err = std::string("stream/event creation: ") + hipGetErrorString(e);
@@ -357,6 +360,8 @@ void destroy_lane(Lane &L) {
     for (auto &ev : L.ev)
         if (ev) (void)hipEventDestroy(ev);
     if (L.fork) (void)hipEventDestroy(L.fork);
+    if (L.fork2) (void)hipEventDestroy(L.fork2);
+    if (L.join2) (void)hipEventDestroy(L.join2);
     if (L.join) (void)hipEventDestroy(L.join);
     if (L.s2 && L.s2 != L.s) (void)hipStreamDestroy(L.s2);
     if (L.s) (void)hipStreamDestroy(L.s);
@@ -1213,12 +1218,25 @@ int launch_clears(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L) {
 // The whole device sequence of one execution (capturable: kernels, event
 // fork/join and one device-to-host copy into pinned memory).
 int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, bool events,
-                    int64_t *launches) {
+                    int64_t *launches, float *maxima_dst = nullptr) {
     Ctl *ctl = (Ctl *)W.ctl.p;
     int rc = launch_clears(ctx, P, W, L);
     if (rc) return rc;
     rc = launch_score(ctx, P, W, L, g, events, launches);
     if (rc) return rc;
+    // the pair maxima go to the host on the second stream while the traceback
+    // stages run (a lane with one stream copies them in order)
+    const bool copy_beside = maxima_dst && P->n_pairs > 0 && L.s2 != L.s;
+    if (copy_beside) {
+        HIPCHK(ctx, hipEventRecord(L.fork2, L.s));
+        HIPCHK(ctx, hipStreamWaitEvent(L.s2, L.fork2, 0));
+        HIPCHK(ctx, hipMemcpyAsync(maxima_dst, W.maxima.p, (size_t)P->n_pairs * sizeof(float),
+                                   hipMemcpyDeviceToHost, L.s2));
+        HIPCHK(ctx, hipEventRecord(L.join2, L.s2));
+    } else if (maxima_dst && P->n_pairs > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(maxima_dst, W.maxima.p, (size_t)P->n_pairs * sizeof(float),
+                                   hipMemcpyDeviceToHost, L.s));
+    }
     if (P->n_pairs > 0) {
         hipLaunchKernelGGL(gate_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
                            (const float *)P->d_mss.p, P->n_pairs, P->sc.minimum, ctl, (int *)W.gated.p);
@@ -1233,6 +1251,7 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
     rc = launch_traceback(ctx, P, W, L, g, minimum, events);
     if (rc) return rc;
+    if (copy_beside) HIPCHK(ctx, hipStreamWaitEvent(L.s, L.join2, 0));
     HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
     return PW_OK;
 }
@@ -1633,13 +1652,14 @@ int execute_launch(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimu
         HIPCHK(ctx, hipEventRecord(L.ev[0], L.s));
         HIPCHK(ctx, hipGraphLaunch(P->graph, L.s));
         HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
+        // the destination differs per call, so the maxima copy stays out of the graph
+        if (P->n_pairs > 0)
+            HIPCHK(ctx, hipMemcpyAsync(maxima_dst, W.maxima.p, (size_t)P->n_pairs * sizeof(float),
+                                       hipMemcpyDeviceToHost, L.s));
     } else {
-        rc = launch_pipeline(ctx, P, W, L, g, minimum, true, &X.launches);
+        rc = launch_pipeline(ctx, P, W, L, g, minimum, true, &X.launches, maxima_dst);
         if (rc) return rc;
     }
-    if (P->n_pairs > 0)
-        HIPCHK(ctx, hipMemcpyAsync(maxima_dst, W.maxima.p, (size_t)P->n_pairs * sizeof(float),
-                                   hipMemcpyDeviceToHost, L.s));
     return PW_OK;
 }
 
