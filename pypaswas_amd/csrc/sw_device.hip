@@ -447,9 +447,13 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
             const float *prow = prof + a * NBP;
             float diag = dl;   // T[r-1][c-1] for the first column of the block
             float ic = lb.y;   // affine: I candidate entering the column; linear: H[r][c-1]
+            // the profile of the next 4-column block is read one block ahead,
+            // so that the LDS latency overlaps the current block's arithmetic
+            float4 s4n = *reinterpret_cast<const float4 *>(prow);
 #pragma unroll
             for (int b = 0; b < NB; b += 4) {
-                const float4 s4 = *reinterpret_cast<const float4 *>(prow + b);
+                const float4 s4 = s4n;
+                if (b + 4 < NB) s4n = *reinterpret_cast<const float4 *>(prow + b + 4);
                 const float dg0 = diag + s4.x;
                 const float dg1 = Tp[b] + s4.y;
                 const float dg2 = Tp[b + 1] + s4.z;
