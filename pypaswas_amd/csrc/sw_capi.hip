@@ -131,6 +131,18 @@ int tune_bucket(int v, const int *rep, int n) {
 
 }  // namespace
 
+// A result as the library owns it until pw_free_result; pub is its public
+// view (first member, so a pw_result pointer converts back).
+struct HostResult {
+    pw_result pub;
+    float *maxima = nullptr;  // pinned host memory of maxima_cap floats
+    size_t maxima_cap = 0;
+    std::vector<float> maxima_vec;  // maxima of a served job (when maxima is NULL)
+    std::vector<pw_starting_point> sps;
+    std::vector<pw_hit> hits;
+    std::vector<char> ops;
+};
+
 struct pw_ctx {
     int device = 0;
     int num_cus = 0;
@@ -152,6 +164,7 @@ struct pw_ctx {
     std::map<std::pair<const void *, size_t>, int> occupancy;
     PinBuf h_stage;  // upload staging of the plans of pw_prepare and the one-shot calls
     Tuning tune[2];  // linear, affine
+    std::vector<HostResult *> result_pool;  // freed results kept for reuse
 };
 
 static thread_local std::string g_init_err;
@@ -419,6 +432,7 @@ extern "C" int pw_destroy(pw_ctx *ctx) {
     free_workspace(ctx->ws);
     free_pinned(ctx->h_stage);
     release_dead(ctx);
+    for (HostResult *R : ctx->result_pool) delete R;
     for (auto &pb : ctx->pinned_free) (void)hipHostFree(pb.first);
     delete ctx;
     return PW_OK;
@@ -1256,15 +1270,6 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
     return PW_OK;
 }
 
-struct HostResult {
-    pw_result pub;
-    float *maxima = nullptr;  // pinned host memory of maxima_cap floats
-    size_t maxima_cap = 0;
-    std::vector<float> maxima_vec;  // maxima of a served job (when maxima is NULL)
-    std::vector<pw_starting_point> sps;
-    std::vector<pw_hit> hits;
-    std::vector<char> ops;
-};
 
 // Finished traceback work of one gated pair: where its records sit in the
 // read-back arrays.
@@ -1504,8 +1509,33 @@ void adapt_workspace(pw_ctx *ctx, Workspace &W, const Ctl &c, const std::vector<
 }
 
 // Per-pair records in pair order into the host result.
+// Sorts finished pairs by pair index: a stable LSD radix sort on
+// (pair << 32 | position) keys, one byte of the pair index per pass.
+void sort_by_pair(std::vector<PairOut> &outs) {
+    const size_t n = outs.size();
+    bool sorted = true;
+    for (size_t i = 1; i < n && sorted; ++i) sorted = outs[i - 1].pair < outs[i].pair;
+    if (sorted) return;
+    std::vector<uint64_t> keys(n), tmp(n);
+    uint32_t maxp = 0;
+    for (size_t i = 0; i < n; ++i) {
+        keys[i] = ((uint64_t)(uint32_t)outs[i].pair << 32) | (uint64_t)i;
+        maxp = std::max(maxp, (uint32_t)outs[i].pair);
+    }
+    for (int shift = 32; shift < 64 && (maxp >> (shift - 32)) != 0; shift += 8) {
+        size_t cnt[257] = {};
+        for (uint64_t k : keys) ++cnt[((k >> shift) & 255u) + 1];
+        for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+        for (uint64_t k : keys) tmp[cnt[(k >> shift) & 255u]++] = k;
+        keys.swap(tmp);
+    }
+    std::vector<PairOut> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = outs[keys[i] & 0xffffffffu];
+    outs.swap(out);
+}
+
 void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostResult *R) {
-    std::sort(outs.begin(), outs.end(), [](const PairOut &a, const PairOut &b) { return a.pair < b.pair; });
+    sort_by_pair(outs);
     int64_t nsp = 0, nops = 0;
     for (const PairOut &o : outs) {
         nsp += o.nsp;
@@ -1703,7 +1733,7 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
     }
     F.ms_tail = ms_since(t0);
     t0 = std::chrono::steady_clock::now();
-    std::sort(F.outs.begin(), F.outs.end(), [](const PairOut &a, const PairOut &b) { return a.pair < b.pair; });
+    sort_by_pair(F.outs);
     F.ms_assemble = ms_since(t0);
     return PW_OK;
 }
@@ -1767,9 +1797,21 @@ int ready(pw_ctx *ctx) {
     return PW_OK;
 }
 
-int new_result(HostResult **out) {
-    HostResult *R = new HostResult();
+// A result record, recycled from the context's pool when one is there (its
+// vectors keep their capacity, so filling them touches no new pages).
+int new_result(pw_ctx *ctx, HostResult **out) {
+    HostResult *R = nullptr;
+    if (ctx && !ctx->result_pool.empty()) {
+        R = ctx->result_pool.back();
+        ctx->result_pool.pop_back();
+    } else {
+        R = new HostResult();
+    }
     std::memset(&R->pub, 0, sizeof(R->pub));
+    R->sps.clear();
+    R->hits.clear();
+    R->ops.clear();
+    R->maxima_vec.clear();
     *out = R;
     return PW_OK;
 }
@@ -2043,7 +2085,7 @@ static int execute_entry(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_
     }
     *out = nullptr;
     HostResult *R = nullptr;
-    new_result(&R);
+    new_result(ctx, &R);
     rc = execute(ctx, plan, plan->own, ctx->lane, minimum, sp_cap, graph, R);
     if (rc) {
         pw_free_result(ctx, &R->pub);
@@ -2131,7 +2173,7 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     if (!rc) {
         HostResult *R = nullptr;
-        new_result(&R);
+        new_result(ctx, &R);
         rc = execute(ctx, P, ctx->ws, ctx->lane, minimum, sp_cap, false, R);
         if (rc) pw_free_result(ctx, &R->pub);
         else *out = &R->pub;
@@ -2148,8 +2190,10 @@ extern "C" int pw_free_result(pw_ctx *ctx, pw_result *res) {
         if (ctx && ctx->pinned_free.size() < 4) ctx->pinned_free.push_back({R->maxima, R->maxima_cap});
         else (void)hipHostFree(R->maxima);
         R->maxima = nullptr;
+        R->maxima_cap = 0;
     }
-    delete R;
+    if (ctx && ctx->result_pool.size() < 4) ctx->result_pool.push_back(R);
+    else delete R;
     return PW_OK;
 }
 

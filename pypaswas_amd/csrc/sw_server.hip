@@ -210,9 +210,9 @@ int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, in
 }
 
 // One job's records out of its batch's (pair-sorted) records.
-void split_job(const Finished &F, const ServeLane &ln, ServeJob *j, size_t &cursor) {
-    HostResult *R = new HostResult();
-    std::memset(&R->pub, 0, sizeof(R->pub));
+void split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j, size_t &cursor) {
+    HostResult *R = nullptr;
+    new_result(ctx, &R);
     const int64_t p1 = j->p0 + (int64_t)j->pq.size();
     std::vector<PairOut> mine;
     while (cursor < F.outs.size() && F.outs[cursor].pair < p1) {
@@ -252,7 +252,7 @@ int finish_batch(pw_server *S, ServeLane &ln) {
     S->us_device += 1e3 * elapsed(ln.L.ev[0], ln.L.ev[5]);
     size_t cursor = 0;
     for (ServeJob *j : ln.jobs) {
-        split_job(F, ln, j, cursor);
+        split_job(ctx, F, ln, j, cursor);
         S->completed.push_back(j->id);
         S->n_completed += 1;
     }
