@@ -183,7 +183,9 @@ int pw_set_tuning(pw_ctx *ctx, int32_t affine, const float *rates, int32_t group
  * lane), the chunk width of lane-per-pair jobs whose target exceeds 96
  * residues (64 or 96), and tile shapes from the pw_autotune table or from the
  * cost model (when a table exists) -- and the fastest work list is kept for
- * the plan's executions.  Results never depend on these choices.
+ * the plan's executions; the defaults (1/4, 64, cost model) stay unless
+ * another combination is at least 3% faster.  Results never depend on these
+ * choices.
  * *seconds: tuning time; *score_ms: the kept scoring pass; choice[3]: share x
  * 1000, chunk width, table used.  Plain plans (mode 0) are left as they are. */
 int pw_tune_plan(pw_ctx *ctx, pw_plan *plan, double *seconds, double *score_ms, int32_t *choice);

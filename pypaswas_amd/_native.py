@@ -323,7 +323,7 @@ class Device:
         self._check(self.lib.pw_tune_plan(self.ctx, plan.handle, ctypes.byref(sec), ctypes.byref(ms),
                                           choice.ctypes.data), "pw_tune_plan")
         return {"seconds": round(float(sec.value), 3), "score_ms": round(float(ms.value), 3),
-                "group_max_share": choice[0] / 1000.0, "group_chunk_width": int(choice[1]),
+                "group_max_share": float(choice[0]) / 1000.0, "group_chunk_width": int(choice[1]),
                 "measured_shapes": bool(choice[2])}
 
     def set_limits(self, max_direction_bytes: int = 0, candidates_per_pair: int = 0, ops_bytes: int = 0,

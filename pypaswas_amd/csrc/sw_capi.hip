@@ -160,7 +160,11 @@ struct pw_ctx {
     // are kept here instead of freed (hipFree and hipHostFree wait for the
     // whole device) and released once the device is idle
     bool defer_free = false;
-    std::vector<void *> dead_dev, dead_pinned;
+    std::vector<std::pair<void *, size_t>> dead_dev;
+    std::vector<void *> dead_pinned;
+    // device buffers given back, by size, kept for reuse (dev_alloc, dev_release)
+    std::multimap<size_t, void *> dev_cache;
+    size_t dev_cache_bytes = 0;
     std::map<std::pair<const void *, size_t>, int> occupancy;
     PinBuf h_stage;  // upload staging of the plans of pw_prepare and the one-shot calls
     Tuning tune[2];  // linear, affine
@@ -252,36 +256,69 @@ struct pw_plan {
         }                                                                                   \
     } while (0)
 
+// Device memory is not given back to the runtime while the context lives:
+// after one hipFree of a large allocation (256 MiB and up, measured on
+// MI355X with ROCm 7) every later device-to-host copy of the process runs
+// about three times slower (a 5 MiB copy into pinned memory, 0.10 -> 0.35 ms),
+// and hipFree waits for the whole device besides.  Released buffers go to a
+// cache by size instead; dev_alloc takes a cached buffer that is large enough
+// and not much larger, else allocates.  The cache is emptied only when an
+// allocation fails, or by pw_destroy.  A buffer enters the cache only once no
+// queued work uses it (the callers release after the stream synchronized, or
+// through the deferred list while a server runs).
+static void dev_cache_flush(pw_ctx *ctx) {
+    for (auto &kv : ctx->dev_cache) (void)hipFree(kv.second);
+    ctx->dev_cache.clear();
+    ctx->dev_cache_bytes = 0;
+}
+
+static void dev_release(pw_ctx *ctx, void *p, size_t cap) {
+    if (!p) return;
+    ctx->dev_cache.emplace(cap, p);
+    ctx->dev_cache_bytes += cap;
+}
+
+// A buffer of at least need bytes; want bytes when a new one is allocated.
+static int dev_alloc(pw_ctx *ctx, size_t need, size_t want, void **p, size_t *cap) {
+    auto it = ctx->dev_cache.lower_bound(need);
+    if (it != ctx->dev_cache.end() && it->first <= std::max(4 * need, need + ((size_t)16 << 20))) {
+        *p = it->second;
+        *cap = it->first;
+        ctx->dev_cache_bytes -= it->first;
+        ctx->dev_cache.erase(it);
+        return PW_OK;
+    }
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess && !ctx->dev_cache.empty()) {
+        (void)hipGetLastError();
+        dev_cache_flush(ctx);
+        e = hipMalloc(p, want);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        *cap = 0;
+        ctx->err = "hipMalloc of " + std::to_string(want) + " bytes: " + hipGetErrorString(e);
+        return PW_ENOMEM;
+    }
+    *cap = want;
+    return PW_OK;
+}
+
 // Buffers grow to twice the request, so that a sequence of growing requests
-// reallocates rarely (each reallocation frees, and hipFree waits for the device).
+// reallocates rarely.
 static int ensure(pw_ctx *ctx, DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return PW_OK;
     ctx->reallocs += 1;
     if (b.p) {
-        if (ctx->defer_free) {
-            ctx->dead_dev.push_back(b.p);
-        } else {
-            hipError_t e = hipFree(b.p);
-            if (e != hipSuccess) {
-                b.p = nullptr;
-                b.cap = 0;
-                ctx->err = std::string("hipFree: ") + hipGetErrorString(e);
-                return PW_EDEVICE;
-            }
-        }
+        if (ctx->defer_free) ctx->dead_dev.emplace_back(b.p, b.cap);
+        else dev_release(ctx, b.p, b.cap);
         b.p = nullptr;
         b.cap = 0;
     }
-    size_t want = bytes < ((size_t)1 << 30) ? 2 * bytes : bytes + bytes / 4;
-    hipError_t e = hipMalloc(&b.p, want);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        ctx->err = "hipMalloc of " + std::to_string(want) + " bytes: " + hipGetErrorString(e);
-        return PW_ENOMEM;
-    }
-    b.cap = want;
-    return PW_OK;
+    const size_t want = bytes < ((size_t)1 << 30) ? 2 * bytes : bytes + bytes / 4;
+    return dev_alloc(ctx, bytes, want, &b.p, &b.cap);
 }
 
 static int ensure_pinned(pw_ctx *ctx, PinBuf &b, size_t bytes) {
@@ -319,7 +356,7 @@ static int ensure_pinned(pw_ctx *ctx, PinBuf &b, size_t bytes) {
 
 // Releases the buffers kept while frees were deferred.
 static void release_dead(pw_ctx *ctx) {
-    for (void *p : ctx->dead_dev) (void)hipFree(p);
+    for (auto &d : ctx->dead_dev) dev_release(ctx, d.first, d.second);
     for (void *p : ctx->dead_pinned) (void)hipHostFree(p);
     ctx->dead_dev.clear();
     ctx->dead_pinned.clear();
@@ -327,8 +364,8 @@ static void release_dead(pw_ctx *ctx) {
 
 namespace {
 
-void free_buf(DevBuf &b) {
-    if (b.p) (void)hipFree(b.p);
+void free_buf(pw_ctx *ctx, DevBuf &b) {
+    dev_release(ctx, b.p, b.cap);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -339,10 +376,10 @@ void free_pinned(PinBuf &b) {
     b.cap = 0;
 }
 
-void free_workspace(Workspace &W) {
+void free_workspace(pw_ctx *ctx, Workspace &W) {
     DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
                       &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks, &W.dprog};
-    for (DevBuf *b : bufs) free_buf(*b);
+    for (DevBuf *b : bufs) free_buf(ctx, *b);
     PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_sps, &W.h_hits, &W.h_ops};
     for (PinBuf *b : pins) free_pinned(*b);
 }
@@ -429,9 +466,10 @@ extern "C" int pw_destroy(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
     (void)hipSetDevice(ctx->device);
     destroy_lane(ctx->lane);
-    free_workspace(ctx->ws);
+    free_workspace(ctx, ctx->ws);
     free_pinned(ctx->h_stage);
     release_dead(ctx);
+    dev_cache_flush(ctx);
     for (HostResult *R : ctx->result_pool) delete R;
     for (auto &pb : ctx->pinned_free) (void)hipHostFree(pb.first);
     delete ctx;
@@ -678,19 +716,24 @@ int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, con
     return PW_OK;
 }
 
-void free_plan_buffers(pw_plan *P) {
+void free_plan_buffers(pw_ctx *ctx, pw_plan *P) {
     DevBuf *bufs[] = {&P->blob, &P->d_links, &P->d_prog};
-    for (DevBuf *b : bufs) free_buf(*b);
+    for (DevBuf *b : bufs) free_buf(ctx, *b);
     free_pinned(P->h_blob);
-    free_workspace(P->own);
+    free_workspace(ctx, P->own);
     if (P->graph) (void)hipGraphExecDestroy(P->graph);
     P->graph = nullptr;
     P->graph_sig.clear();
 }
 
-void free_plan(pw_plan *P) {
+// The plan's buffers are reused at once: work that an error left queued on the
+// context's lane must end first (a server's lanes are destroyed before their
+// plans are freed).
+void free_plan(pw_ctx *ctx, pw_plan *P) {
     if (!P) return;
-    free_plan_buffers(P);
+    (void)hipStreamSynchronize(ctx->lane.s);
+    if (ctx->lane.s2 != ctx->lane.s) (void)hipStreamSynchronize(ctx->lane.s2);
+    free_plan_buffers(ctx, P);
     delete P;
 }
 
@@ -1860,7 +1903,7 @@ int time_synthetic(pw_ctx *ctx, const PlanOpts &opt, int nq, int A, int nt, int 
         }
         if (!rc && rep > 0) ms = std::min(ms, elapsed(ctx->lane.ev[0], ctx->lane.ev[1]));
     }
-    free_plan_buffers(&P);
+    free_plan_buffers(ctx, &P);
     return rc;
 }
 
@@ -1920,6 +1963,8 @@ int autotune(pw_ctx *ctx, double *seconds) {
     }
     tu.valid = true;
     ctx->tune[ctx->sc.affine ? 1 : 0] = tu;
+    // the synthetic jobs' upload staging is large; give it back
+    free_pinned(ctx->h_stage);
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PW_OK;
 }
@@ -1952,7 +1997,14 @@ int tune_plan(pw_ctx *ctx, pw_plan *P, double *seconds, double *best_ms, int32_t
         o.tune = c.table ? tu : nullptr;
         return o;
     };
-    double best = 1e300;
+    // the default choices are timed first and kept unless another candidate
+    // is clearly faster (differences within a few per cent are noise)
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) {
+        const bool da = a.share == 0.25 && a.nb == 64 && !a.table, db = b.share == 0.25 && b.nb == 64 && !b.table;
+        return da && !db;
+    });
+    constexpr double PLAN_TUNE_MARGIN = 0.97;
+    double best = 1e300, base = 1e300;
     size_t bi = 0;
     Lane &L = ctx->lane;
     for (size_t i = 0; i < cands.size(); ++i) {
@@ -1961,7 +2013,7 @@ int tune_plan(pw_ctx *ctx, pw_plan *P, double *seconds, double *best_ms, int32_t
         const Geometry g = geometry(ctx, P);
         if (!rc) rc = size_workspace(ctx, P, P->own, g);
         double ms = 1e300;
-        for (int rep = 0; rep < 3 && !rc; ++rep) {
+        for (int rep = 0; rep < 4 && !rc; ++rep) {
             rc = launch_clears(ctx, P, P->own, L);
             if (!rc) rc = launch_score(ctx, P, P->own, L, g, true, nullptr);
             if (!rc) {
@@ -1974,7 +2026,9 @@ int tune_plan(pw_ctx *ctx, pw_plan *P, double *seconds, double *best_ms, int32_t
             if (!rc && rep > 0) ms = std::min(ms, elapsed(L.ev[0], L.ev[1]));
         }
         if (rc) return rc;
-        if (ms < best) {
+        if (i == 0) {
+            base = best = ms;
+        } else if (ms < best && ms < PLAN_TUNE_MARGIN * base) {
             best = ms;
             bi = i;
         }
@@ -2068,7 +2122,7 @@ extern "C" int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff,
                    min_start_score, same_id, mode);
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     if (rc) {
-        free_plan(P);
+        free_plan(ctx, P);
         return rc;
     }
     *out = P;
@@ -2108,7 +2162,7 @@ extern "C" int pw_release_plan(pw_ctx *ctx, pw_plan *plan) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->lane.s);
     (void)hipStreamSynchronize(ctx->lane.s2);
-    free_plan(plan);
+    free_plan(ctx, plan);
     return PW_OK;
 }
 
@@ -2152,7 +2206,7 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         }
     }
     if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
-    free_plan(P);
+    free_plan(ctx, P);
     return rc;
 }
 
@@ -2178,7 +2232,7 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         if (rc) pw_free_result(ctx, &R->pub);
         else *out = &R->pub;
     }
-    free_plan(P);
+    free_plan(ctx, P);
     return rc;
 }
 

@@ -372,9 +372,9 @@ extern "C" int pw_server_create(pw_ctx *ctx, int32_t lanes, int64_t direction_by
             for (ServeLane &l2 : S->lanes) {
                 destroy_lane(l2.L);
                 if (l2.done_ev) (void)hipEventDestroy(l2.done_ev);
-                if (l2.P) free_plan(l2.P);
+                if (l2.P) free_plan(ctx, l2.P);
             }
-            free_buf(S->arena);
+            free_buf(ctx, S->arena);
             ctx->defer_free = false;
             release_dead(ctx);
             delete S;
@@ -522,7 +522,7 @@ extern "C" int pw_server_destroy(pw_server *S) {
     for (ServeLane &ln : S->lanes) {
         destroy_lane(ln.L);
         if (ln.done_ev) (void)hipEventDestroy(ln.done_ev);
-        if (ln.P) free_plan(ln.P);
+        if (ln.P) free_plan(ctx, ln.P);
         free_pinned(ln.h_max);
     }
     for (auto &kv : S->jobs) {
@@ -530,7 +530,7 @@ extern "C" int pw_server_destroy(pw_server *S) {
         delete kv.second;
     }
     // queued jobs are in the map too
-    free_buf(S->arena);
+    free_buf(ctx, S->arena);
     ctx->defer_free = false;
     release_dead(ctx);
     delete S;
