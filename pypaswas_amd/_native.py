@@ -142,7 +142,7 @@ class AlignResult:
     maxima: np.ndarray          # [n_pairs] f32
     starting_points: np.ndarray  # structured, 36-byte records
     hits: np.ndarray             # structured
-    ops: bytes
+    ops: memoryview             # one byte per alignment column ('D', 'L', 'U'), bytes-like
     n_traceback_pairs: int
     sp_truncated: int
     cells: float
@@ -172,6 +172,20 @@ def _release_result(dev_ref, lib, out):
     dev = dev_ref()
     ctx = dev.ctx if dev is not None and dev.ctx else None
     lib.pw_free_result(ctx, out)
+
+
+class _ResultHold:
+    """Keeps a library result alive while any view of its buffers lives."""
+    __slots__ = ("__weakref__",)
+
+
+def _view(ptr, count: int, dtype: np.dtype, hold: _ResultHold) -> np.ndarray:
+    """count records of dtype at a library pointer, as an array that keeps hold alive."""
+    if count <= 0:
+        return np.zeros(0, dtype)
+    buf = (ctypes.c_char * (count * dtype.itemsize)).from_address(ctypes.cast(ptr, ctypes.c_void_p).value)
+    buf._hold = hold
+    return np.frombuffer(buf, dtype=dtype)
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -243,33 +257,31 @@ class Device:
         return self._take_result(out)
 
     def _take_result(self, out) -> AlignResult:
-        free_now = True
-        try:
-            r = out.contents
-            n = int(r.n_pairs)
-            if n:
-                # a view of the library's (pinned) maxima buffer, not a copy: the result
-                # goes back to the library when the array is collected
-                maxima = np.ctypeslib.as_array(r.maxima, shape=(n,))
-                fin = weakref.finalize(maxima, _release_result, weakref.ref(self), self.lib, out)
-                fin.atexit = False
-                free_now = False
-            else:
-                maxima = np.zeros(0, np.float32)
-            nsp = int(r.n_starting_points)
-            sps = (np.frombuffer(ctypes.string_at(r.starting_points, nsp * SP_DTYPE.itemsize), dtype=SP_DTYPE).copy()
-                   if nsp else np.zeros(0, SP_DTYPE))
-            nh = int(r.n_hits)
-            hits = (np.frombuffer(ctypes.string_at(r.hits, nh * HIT_DTYPE.itemsize), dtype=HIT_DTYPE).copy()
-                    if nh else np.zeros(0, HIT_DTYPE))
-            ops = ctypes.string_at(r.ops, int(r.n_ops)) if r.n_ops else b""
-            return AlignResult(maxima, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
-                               float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
-                               float(r.ms_walk), float(r.ms_total), int(r.score_launches), float(r.ms_readback),
-                               float(r.ms_tail), float(r.ms_assemble))
-        finally:
-            if free_now:
-                self.lib.pw_free_result(self.ctx, out)
+        """Wraps a library result without copying: maxima, starting points,
+        hits and ops are views of the library's buffers, which go back to the
+        library once the last of these views is collected."""
+        r = out.contents
+        hold = _ResultHold()
+        fin = weakref.finalize(hold, _release_result, weakref.ref(self), self.lib, out)
+        fin.atexit = False
+        n = int(r.n_pairs)
+        maxima = _view(r.maxima, n, np.dtype(np.float32), hold)
+        sps = _view(r.starting_points, int(r.n_starting_points), SP_DTYPE, hold)
+        hits = _view(r.hits, int(r.n_hits), HIT_DTYPE, hold)
+        n_ops = int(r.n_ops)
+        if n_ops:
+            buf = (ctypes.c_char * n_ops).from_address(ctypes.cast(r.ops, ctypes.c_void_p).value)
+            buf._hold = hold
+            ops = memoryview(buf).cast("B")
+        else:
+            ops = memoryview(b"")
+        res = AlignResult(maxima, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
+                          float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
+                          float(r.ms_walk), float(r.ms_total), int(r.score_launches), float(r.ms_readback),
+                          float(r.ms_tail), float(r.ms_assemble))
+        # with no view left holding it (an empty result), the record goes back here
+        del hold
+        return res
 
     def prepare(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                 min_start_score: Optional[np.ndarray], same_id: Optional[np.ndarray], mode: int = 1) -> "Plan":

@@ -445,20 +445,23 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                 lb_next.y = load_scr(&scr[(r + 1) * WAVE + lane].y);
             }
             const float *prow = prof + a * NBP;
-            float diag = dl;   // T[r-1][c-1] for the first column of the block
             float ic = lb.y;   // affine: I candidate entering the column; linear: H[r][c-1]
             // the profile of the next 4-column block is read one block ahead,
             // so that the LDS latency overlaps the current block's arithmetic
             float4 s4n = *reinterpret_cast<const float4 *>(prow);
+            // diagonal sum of the block's first column: T[r-1][c-1] + s; for
+            // the next block it is formed before this block's last column
+            // overwrites T[r-1][c-1], so each column's T keeps one register
+            float dgc = dl + s4n.x;
 #pragma unroll
             for (int b = 0; b < NB; b += 4) {
                 const float4 s4 = s4n;
                 if (b + 4 < NB) s4n = *reinterpret_cast<const float4 *>(prow + b + 4);
-                const float dg0 = diag + s4.x;
+                const float dg0 = dgc;
                 const float dg1 = Tp[b] + s4.y;
                 const float dg2 = Tp[b + 1] + s4.z;
                 const float dg3 = Tp[b + 2] + s4.w;
-                diag = Tp[b + 3];
+                if (b + 4 < NB) dgc = Tp[b + 3] + s4n.x;
                 const float dg[4] = {dg0, dg1, dg2, dg3};
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -486,7 +489,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                     if (AFF) asm volatile("" : "+v"(Tp[b + u]), "+v"(Jc[b + u]));
                     else asm volatile("" : "+v"(Tp[b + u]));
                 }
-                asm volatile("" : "+v"(rmax), "+v"(ic), "+v"(diag));
+                asm volatile("" : "+v"(rmax), "+v"(ic));
             }
             if (right_out) {
                 store_scr(&scr[r * WAVE + lane].x, Tp[NB - 1]);
