@@ -414,11 +414,15 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
     for (int k = 0; k < nchunks; ++k) {
         const int c0 = k * NB;
         __syncthreads();
-        for (int idx = lane; idx < kp * NB; idx += WAVE) {
-            const int a = idx / NB, c = idx - a * NB;
-            const int col = c0 + c;
+        // profile row of residue a, shifted by one column: entry p < NB holds
+        // the score of column c0+p+1, entry NB the score of column c0; so the
+        // 4-column read of block b supplies columns b+1..b+4, i.e. three of
+        // the block's own diagonal sums and the first one of the next block
+        for (int idx = lane; idx < kp * (NB + 1); idx += WAVE) {
+            const int a = idx / (NB + 1), p = idx - a * (NB + 1);
+            const int col = c0 + (p < NB ? p + 1 : 0);
             const int b = col < C.len ? (int)res[C.off + col] : fill;
-            prof[a * NBP + c] = tab[b * kp + a];
+            prof[a * NBP + p] = tab[b * kp + a];
         }
         __syncthreads();
         const bool left_in = k > 0, right_out = k + 1 < nchunks;
@@ -435,33 +439,42 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
             lb_next.x = load_scr(&scr[lane].x);
             lb_next.y = load_scr(&scr[lane].y);
         }
+        // profile values of the first row: columns 1..4 and column 0
+        const float *prow = prof + a_next * NBP;
+        float4 s4n = *reinterpret_cast<const float4 *>(prow);
+        float s0n = prow[NB];
 #pragma unroll 1
         for (int r = 0; r < it.rows; ++r) {
-            const int a = a_next;
             const float2 lb = lb_next;
             a_next = (active && r + 1 < Rw.len) ? (int)res[Rw.off + r + 1] : fill;
             if (left_in && r + 1 < it.rows) {
                 lb_next.x = load_scr(&scr[(r + 1) * WAVE + lane].x);
                 lb_next.y = load_scr(&scr[(r + 1) * WAVE + lane].y);
             }
-            const float *prow = prof + a * NBP;
-            float ic = lb.y;   // affine: I candidate entering the column; linear: H[r][c-1]
-            // the profile of the next 4-column block is read one block ahead,
-            // so that the LDS latency overlaps the current block's arithmetic
-            float4 s4n = *reinterpret_cast<const float4 *>(prow);
-            // diagonal sum of the block's first column: T[r-1][c-1] + s; for
-            // the next block it is formed before this block's last column
-            // overwrites T[r-1][c-1], so each column's T keeps one register
-            float dgc = dl + s4n.x;
+            float ic = lb.y;  // affine: I candidate entering the column; linear: H[r][c-1]
+            // diagonal sum of the block's first column, T[r-1][c-1] + s; each
+            // block forms the next block's from the profile values it already
+            // holds, before its last column overwrites T[r-1][c-1]
+            float dgc = dl + s0n;
 #pragma unroll
             for (int b = 0; b < NB; b += 4) {
+                // the next block's profile values (the next row's first block
+                // after the last block) are read one block ahead, so that the
+                // LDS latency overlaps this block's arithmetic
                 const float4 s4 = s4n;
-                if (b + 4 < NB) s4n = *reinterpret_cast<const float4 *>(prow + b + 4);
+                if (b + 4 < NB) {
+                    s4n = *reinterpret_cast<const float4 *>(prow + b + 4);
+                } else {
+                    // the next row's first block and column 0
+                    prow = prof + a_next * NBP;
+                    s4n = *reinterpret_cast<const float4 *>(prow);
+                    s0n = prow[NB];
+                }
                 const float dg0 = dgc;
-                const float dg1 = Tp[b] + s4.y;
-                const float dg2 = Tp[b + 1] + s4.z;
-                const float dg3 = Tp[b + 2] + s4.w;
-                if (b + 4 < NB) dgc = Tp[b + 3] + s4n.x;
+                const float dg1 = Tp[b] + s4.x;
+                const float dg2 = Tp[b + 1] + s4.y;
+                const float dg3 = Tp[b + 2] + s4.z;
+                if (b + 4 < NB) dgc = Tp[b + 3] + s4.w;
                 const float dg[4] = {dg0, dg1, dg2, dg3};
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
