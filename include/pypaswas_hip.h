@@ -119,6 +119,15 @@ int pw_prepare(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int3
 int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out);
 int pw_release_plan(pw_ctx *ctx, pw_plan *plan);
 
+/* Routing of a prepared plan, for diagnostics and tests (no counterpart in
+ * pyPaSWAS, whose kernels have one launch shape): info[0..n) receives, in this
+ * order, pairs, lane-per-pair jobs, wave-per-pair items, the orientation of
+ * the lane-per-pair jobs (0: the target is the shared column sequence, 1: the
+ * query), cells scored lane-per-pair, cells scored wave-per-pair, the widest
+ * chunk of a lane-per-pair job and the direction pass's largest residues per
+ * lane (at most 8 values). */
+int pw_plan_info(const pw_plan *plan, int64_t *info, int32_t n);
+
 /* pw_execute with the device work of the execution (scoring, gate, direction
  * pass, start selection, walk and the control-block read-back) captured once
  * in a HIP graph and replayed on later calls; the graph is captured again
@@ -145,6 +154,15 @@ int pw_execute_graph(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap,
  * prepared afterwards and to the one-shot calls. */
 int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t candidates_per_pair, int64_t ops_bytes,
                   int32_t grow);
+
+/* Routing rules of the batched scoring pass for plans prepared afterwards (no
+ * counterpart in pyPaSWAS, whose kernels have one launch shape; results never
+ * depend on them).  orient: shared column sequence of lane-per-pair jobs, 0 the
+ * target, 1 the query, -1 whichever the cost estimate prefers (default).
+ * group_max_share: longest lane-per-pair job as a share of the plan's cells
+ * per device lane (default 0.25; <= 0 restores it; a large value keeps every
+ * job whose lanes are busy enough, e.g. for small plans in tests). */
+int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share);
 
 /* Score pass only: pair maxima, the quantity the traceback gate of
  * default_main_cpu.cl:408-410 reads.  maxima_out is [n_pairs]. */

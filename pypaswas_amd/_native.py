@@ -26,7 +26,7 @@ PW_EDEVICE = -3
 
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
-           "pw_execute", "pw_release_plan", "pw_execute_graph", "pw_set_limits", "pw_score_pairs",
+           "pw_execute", "pw_release_plan", "pw_plan_info", "pw_execute_graph", "pw_set_limits", "pw_set_routing", "pw_score_pairs",
            "pw_autotune", "pw_get_tuning", "pw_set_tuning", "pw_tune_plan", "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
            "pw_server_stats", "pw_server_destroy", "pw_free_result", "pw_last_error")
 
@@ -89,9 +89,11 @@ def load() -> ctypes.CDLL:
                                  vp, vp, ctypes.c_int32, ctypes.POINTER(vp)]
         L.pw_execute.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(ResultRec))]
         L.pw_release_plan.argtypes = [vp, vp]
+        L.pw_plan_info.argtypes = [vp, vp, ctypes.c_int32]
         L.pw_execute_graph.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.POINTER(ResultRec))]
         L.pw_set_limits.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
+        L.pw_set_routing.argtypes = [vp, ctypes.c_int32, ctypes.c_double]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
@@ -344,6 +346,10 @@ class Device:
         self._check(self.lib.pw_set_limits(self.ctx, int(max_direction_bytes), int(candidates_per_pair),
                                            int(ops_bytes), 1 if grow else 0), "pw_set_limits")
 
+    def set_routing(self, orient: int = -1, group_max_share: float = 0.0):
+        """Routing rules of batched plans prepared afterwards (pw_set_routing); defaults restore them."""
+        self._check(self.lib.pw_set_routing(self.ctx, int(orient), float(group_max_share)), "pw_set_routing")
+
     def score_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                     mode: int = 1):
         pq = np.ascontiguousarray(pq, dtype=np.int32)
@@ -364,6 +370,14 @@ class Plan:
     def __init__(self, dev: Device, handle):
         self.dev = dev
         self.handle = handle
+
+    def info(self) -> dict:
+        """Routing of the plan (pw_plan_info)."""
+        v = np.zeros(8, dtype=np.int64)
+        self.dev._check(self.dev.lib.pw_plan_info(self.handle, v.ctypes.data, 8), "pw_plan_info")
+        keys = ("pairs", "lane_per_pair_jobs", "wave_per_pair_items", "orientation", "lane_per_pair_cells",
+                "wave_per_pair_cells", "max_chunk_width", "direction_residues_per_lane")
+        return {k: int(x) for k, x in zip(keys, v)}
 
     def release(self):
         if self.handle:

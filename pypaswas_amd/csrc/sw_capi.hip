@@ -119,6 +119,7 @@ struct PlanOpts {
     bool force_group = false; // every pair to lane-per-pair jobs (no fill or length rule)
     int force_nb = 0;       // > 0: chunk width of multi-chunk lane-per-pair jobs
     double group_max_share = 0.25;  // longest lane-per-pair job, as a share of the cells per device lane
+    int orient = -1;  // column sequence of lane-per-pair jobs: 0 target, 1 query, -1 the cheaper one
 };
 
 // Bucket k holds lengths up to 4/3 of its representative length rep[k]
@@ -168,6 +169,8 @@ struct pw_ctx {
     std::map<std::pair<const void *, size_t>, int> occupancy;
     PinBuf h_stage;  // upload staging of the plans of pw_prepare and the one-shot calls
     Tuning tune[2];  // linear, affine
+    int route_orient = -1;        // pw_set_routing: orientation of lane-per-pair jobs (-1: the cheaper)
+    double route_share = 0.25;    // pw_set_routing: longest lane-per-pair job (share of cells per lane)
     std::vector<HostResult *> result_pool;  // freed results kept for reuse
 };
 
@@ -227,6 +230,8 @@ struct pw_plan {
     int64_t dprog_worst = 0;   // and its progress counters
     int32_t dir_rmax = 8;      // largest residues per lane of the direction pass
     int64_t n_groups = 0;      // lane-per-pair jobs
+    int32_t orient = 0;        // column sequence of the lane-per-pair jobs: 0 target, 1 query
+    double group_cells = 0;    // cells scored lane-per-pair
     int64_t group_rows = 0;    // longest row sequence of a multi-chunk job (scratch sizing)
     int32_t max_nb = 16;
     int64_t nb_begin[NB_KINDS] = {}, nb_end[NB_KINDS] = {};  // job ranges per chunk width
@@ -552,6 +557,17 @@ extern "C" int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t c
     return PW_OK;
 }
 
+extern "C" int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share) {
+    if (!ctx) return PW_EINVAL;
+    if (orient < -1 || orient > 1) {
+        ctx->err = "pw_set_routing: orient must be -1, 0 or 1";
+        return PW_EINVAL;
+    }
+    ctx->route_orient = orient;
+    ctx->route_share = group_max_share > 0 ? group_max_share : 0.25;
+    return PW_OK;
+}
+
 namespace {
 
 int encode(std::string &err, const Scoring &sc, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen,
@@ -691,6 +707,9 @@ void choose_shape_model(int qlen, int tlen, int &trans, int &R) {
 // a 64-lane job is used when its lanes are busy for at least this fraction
 // of its rows; the longest row sequence such a job takes
 constexpr double GROUP_MIN_FILL = 0.6;
+// cost of a cell in the wave-per-pair tile kernel relative to a lane-cell of a
+// lane-per-pair job (measured on MI355X: ~2.5x)
+constexpr double TILE_COST_FACTOR = 2.5;
 constexpr int GROUP_MAX_ROWS = 1 << 15;
 
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
@@ -822,6 +841,8 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
     P->gitems.clear();
     P->gents.clear();
     P->n_groups = 0;
+    P->orient = 0;
+    P->group_cells = 0;
     P->group_rows = 0;
     P->max_nb = 16;
     for (int w = 0; w < NB_KINDS; ++w) P->nb_begin[w] = P->nb_end[w] = 0;
@@ -861,61 +882,88 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
         }
         if (n_pairs > begin) close(n_pairs);
     } else {
-        // lane-per-pair jobs for pairs whose target is shared by enough pairs;
-        // the rest (long or unshared pairs) go to the wave-per-pair tile kernel
+        // lane-per-pair jobs for pairs whose column sequence is shared by
+        // enough pairs; the rest (long or unshared pairs) go to the
+        // wave-per-pair tile kernel.  The column sequence of a job is the
+        // target (orient 0, rows = queries) or the query (orient 1, rows =
+        // targets); the orientation with the lower estimated cost is used.
         const double per_lane = P->cells / ((double)std::max(1, num_cus) * 4.0 * WAVE);
-        std::vector<std::vector<int64_t>> by_target((size_t)nt);
-        for (int64_t i = 0; i < n_pairs; ++i) by_target[pt[i]].push_back(i);
         std::vector<int64_t> tile_pairs;
         std::vector<GroupItem> gitems;
         std::vector<GroupEntry> &gents = P->gents;
         std::vector<double> gcost;
-        for (int32_t t = 0; t < nt; ++t) {
-            std::vector<int64_t> &L = by_target[t];
-            if (L.empty()) continue;
-            std::vector<int64_t> cand;
-            for (int64_t i : L)
-                if (!opt.force_tile && qlen[pq[i]] <= GROUP_MAX_ROWS) cand.push_back(i);
-                else tile_pairs.push_back(i);
-            std::stable_sort(cand.begin(), cand.end(),
-                             [&](int64_t x, int64_t y) { return qlen[pq[x]] > qlen[pq[y]]; });
-            // keep a 64-lane job only if its lanes are busy for most of its
-            // rows (sum of row lengths vs 64 x the longest) and its cells per
-            // lane stay below a quarter of the plan's cells per lane of the
-            // device: the job runs as one wave, which alone on a SIMD needs as
-            // long per cell as the whole SIMD needs per lane-cell, so a longer
-            // job would set the makespan (split tiles finish it sooner);
-            // otherwise its pairs go to the tile kernel
-            const int tl = tlen[t];
-            std::vector<int64_t> grp;
-            for (size_t g0 = 0; g0 < cand.size(); g0 += 64) {
-                const size_t g1 = std::min(cand.size(), g0 + 64);
-                double sum = 0;
-                for (size_t u = g0; u < g1; ++u) sum += qlen[pq[cand[u]]];
-                const double fill = sum / (64.0 * std::max(1, qlen[pq[cand[g0]]]));
-                const double job_steps = (double)qlen[pq[cand[g0]]] * tl;
-                const bool keep =
-                    opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
-                for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
+        auto form_groups = [&](int o, bool commit) -> double {
+            const int32_t ncol = o ? nq : nt;
+            auto col_of = [&](int64_t i) { return o ? pq[i] : pt[i]; };
+            auto row_len = [&](int64_t i) { return o ? tlen[pt[i]] : qlen[pq[i]]; };
+            std::vector<std::vector<int64_t>> by_col((size_t)ncol);
+            for (int64_t i = 0; i < n_pairs; ++i) by_col[col_of(i)].push_back(i);
+            // lane slots of the jobs (64 x longest row x columns) plus the tile
+            // kernel's cells at its lower rate per cell
+            double cost = 0;
+            for (int32_t cs = 0; cs < ncol; ++cs) {
+                std::vector<int64_t> &L = by_col[cs];
+                if (L.empty()) continue;
+                const int cl = o ? qlen[cs] : tlen[cs];
+                std::vector<int64_t> cand;
+                for (int64_t i : L)
+                    if (!opt.force_tile && row_len(i) <= GROUP_MAX_ROWS) cand.push_back(i);
+                    else {
+                        cost += TILE_COST_FACTOR * (double)row_len(i) * cl;
+                        if (commit) tile_pairs.push_back(i);
+                    }
+                std::stable_sort(cand.begin(), cand.end(),
+                                 [&](int64_t x, int64_t y) { return row_len(x) > row_len(y); });
+                // keep a 64-lane job only if its lanes are busy for most of its
+                // rows (sum of row lengths vs 64 x the longest) and its cells per
+                // lane stay below a share of the plan's cells per lane of the
+                // device: the job runs as one wave, which alone on a SIMD needs as
+                // long per cell as the whole SIMD needs per lane-cell, so a longer
+                // job would set the makespan (split tiles finish it sooner);
+                // otherwise its pairs go to the tile kernel
+                std::vector<int64_t> grp;
+                for (size_t g0 = 0; g0 < cand.size(); g0 += 64) {
+                    const size_t g1 = std::min(cand.size(), g0 + 64);
+                    double sum = 0;
+                    for (size_t u = g0; u < g1; ++u) sum += row_len(cand[u]);
+                    const int top = std::max(1, row_len(cand[g0]));
+                    const double fill = sum / (64.0 * top);
+                    const double job_steps = (double)top * cl;
+                    const bool keep =
+                        opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
+                    cost += keep ? 64.0 * job_steps : TILE_COST_FACTOR * sum * cl;
+                    if (commit)
+                        for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
+                }
+                if (!commit) continue;
+                const int nb = (opt.force_nb > 0 && cl > 96) ? opt.force_nb : group_nb(cl, opt.tune);
+                const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
+                for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
+                    const int cnt = (int)std::min<size_t>(64, grp.size() - g0);
+                    GroupItem gi;
+                    gi.c_seq = o ? cs : nq + cs;
+                    gi.entry_off = (int32_t)gents.size();
+                    gi.count = cnt;
+                    gi.rows = row_len(grp[g0]);
+                    gi.nb = nb;
+                    gi.trans = o;
+                    for (int u = 0; u < cnt; ++u) {
+                        const int64_t i = grp[g0 + u];
+                        gents.push_back(GroupEntry{o ? nq + pt[i] : pq[i], (int32_t)i});
+                    }
+                    gitems.push_back(gi);
+                    gcost.push_back((double)gi.rows * nch * nb);
+                    for (int u = 0; u < cnt; ++u) P->group_cells += (double)row_len(grp[g0 + u]) * cl;
+                    if (nch > 1) P->group_rows = std::max<int64_t>(P->group_rows, gi.rows);
+                    P->max_nb = std::max(P->max_nb, nb);
+                }
             }
-            const int nb = (opt.force_nb > 0 && tl > 96) ? opt.force_nb : group_nb(tl, opt.tune);
-            const int nch = tl > nb ? (tl + nb - 1) / nb : 1;
-            for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
-                const int cnt = (int)std::min<size_t>(64, grp.size() - g0);
-                GroupItem gi;
-                gi.c_seq = nq + t;
-                gi.entry_off = (int32_t)gents.size();
-                gi.count = cnt;
-                gi.rows = qlen[pq[grp[g0]]];
-                gi.nb = nb;
-                gi.trans = 0;
-                for (int u = 0; u < cnt; ++u) gents.push_back(GroupEntry{pq[grp[g0 + u]], (int32_t)grp[g0 + u]});
-                gitems.push_back(gi);
-                gcost.push_back((double)gi.rows * nch * nb);
-                if (nch > 1) P->group_rows = std::max<int64_t>(P->group_rows, gi.rows);
-                P->max_nb = std::max(P->max_nb, nb);
-            }
-        }
+            return cost;
+        };
+        int orient = opt.orient;
+        if (orient < 0) orient = form_groups(1, false) < form_groups(0, false) ? 1 : 0;
+        form_groups(orient, true);
+        P->orient = orient;
         // longest jobs first for the persistent workers
         std::vector<int64_t> gorder(gitems.size());
         std::iota(gorder.begin(), gorder.end(), 0);
@@ -1827,6 +1875,8 @@ PlanOpts plan_opts(pw_ctx *ctx, const Scoring &sc) {
     o.total_mem = ctx->total_mem;
     o.num_cus = ctx->num_cus;
     o.tune = &ctx->tune[sc.affine ? 1 : 0];
+    o.orient = ctx->route_orient;
+    o.group_max_share = ctx->route_share;
     return o;
 }
 
@@ -2155,6 +2205,20 @@ extern "C" int pw_execute(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp
 
 extern "C" int pw_execute_graph(pw_ctx *ctx, pw_plan *plan, double minimum, int32_t sp_cap, pw_result **out) {
     return execute_entry(ctx, plan, minimum, sp_cap, true, out, "pw_execute_graph");
+}
+
+extern "C" int pw_plan_info(const pw_plan *plan, int64_t *info, int32_t n) {
+    if (!plan || !info || n < 0) return PW_EINVAL;
+    const int64_t v[8] = {plan->n_pairs,
+                          plan->n_groups,
+                          plan->n_tile,
+                          plan->orient,
+                          (int64_t)plan->group_cells,
+                          (int64_t)(plan->cells - plan->group_cells),
+                          plan->mode == 1 ? plan->max_nb : 0,
+                          plan->dir_rmax};
+    for (int32_t i = 0; i < n && i < 8; ++i) info[i] = v[i];
+    return PW_OK;
 }
 
 extern "C" int pw_release_plan(pw_ctx *ctx, pw_plan *plan) {

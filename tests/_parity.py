@@ -53,6 +53,22 @@ def hip_all_vs_all(dev, O, queries, targets, p, mode=1, same_id=None):
                            mode=mode)
 
 
+def plan_info(dev, O, queries, targets, p, mode=1):
+    """Routing of the all-vs-all plan hip_all_vs_all runs (pw_plan_info)."""
+    from pypaswas_amd._native import PackedSeqs
+    N, T = len(queries), len(targets)
+    pq = np.repeat(np.arange(N, dtype=np.int32), T)
+    pt = np.tile(np.arange(T, dtype=np.int32), N)
+    mss = np.zeros(len(pq), dtype=np.float32)
+    dev.set_scoring(p.subst_f32(), p.char_offset, p.gap, p.gap_ext if p.gap_ext is not None else 0.0,
+                    p.lower_limit, p.minimum)
+    plan = dev.prepare(PackedSeqs.pack(queries), PackedSeqs.pack(targets), pq, pt, mss, None, mode=mode)
+    try:
+        return plan.info()
+    finally:
+        plan.release()
+
+
 def sp_tuple(s):
     return (int(s["sequence"]) if not hasattr(s, "sequence") else int(s.sequence),)
 

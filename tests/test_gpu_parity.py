@@ -7,7 +7,7 @@ import zlib
 import numpy as np
 import pytest
 
-from tests._parity import AMINO, DNA, compare, hip_all_vs_all, planted_set
+from tests._parity import AMINO, DNA, compare, hip_all_vs_all, plan_info, planted_set
 
 pytestmark = pytest.mark.gpu
 
@@ -16,6 +16,18 @@ pytestmark = pytest.mark.gpu
 def dev():
     from pypaswas_amd._native import Device
     d = Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def gdev():
+    """A context that keeps every lane-per-pair job whose lanes are busy enough:
+    the default length rule sends the pairs of plans this small to the
+    wave-per-pair kernel."""
+    from pypaswas_amd._native import Device
+    d = Device(0)
+    d.set_routing(-1, 1e9)
     yield d
     d.close()
 
@@ -107,23 +119,49 @@ def overwrite_set(rng, nq, nt, ql, tl, alphabet, plant=0.6, sub=0.05):
 
 
 @pytest.mark.parametrize("case", GROUP_CASES, ids=[c[0] for c in GROUP_CASES])
-def test_group_kernel_parity(dev, oracle, case):
+def test_group_kernel_parity(gdev, oracle, case):
     name, alpha, nq, nt, ql, tl, affine = case
     print("case", name, flush=True)
     rng = np.random.default_rng(zlib.crc32(name.encode()))
     qs, ts = overwrite_set(rng, nq, nt, ql, tl, alpha)
     p = dna_params(oracle, affine, minimum=8.0)
-    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    info = plan_info(gdev, oracle, qs, ts, p)
+    assert info["orientation"] == 0 and info["lane_per_pair_cells"] > 0, info
+    res = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=1)
     compare(oracle, qs, ts, p, res)
     assert res.n_traceback_pairs > 0
 
 
-def test_group_kernel_protein(dev, oracle):
+def test_group_kernel_protein(gdev, oracle):
     rng = np.random.default_rng(99)
     qs, ts = overwrite_set(rng, 70, 3, (200, 260), (50, 700), AMINO, plant=0.8, sub=0.4)
     p = blosum_params(oracle, minimum=25.0)
-    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    info = plan_info(gdev, oracle, qs, ts, p)
+    assert info["orientation"] == 0 and info["lane_per_pair_cells"] > 0, info
+    res = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=1)
     compare(oracle, qs, ts, p, res)
-    res0 = hip_all_vs_all(dev, oracle, qs, ts, p, mode=0)
+    res0 = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=0)
+    assert np.array_equal(res0.maxima, res.maxima)
+    assert np.array_equal(res0.hits, res.hits)
+
+
+@pytest.mark.parametrize("kind", ["dna-linear", "dna-affine", "protein"])
+def test_group_kernel_query_columns(gdev, oracle, kind):
+    """Few queries against many targets of similar length: the lane-per-pair jobs
+    take the query as their shared column sequence (single- and multi-chunk)."""
+    print("case", kind, flush=True)
+    rng = np.random.default_rng(zlib.crc32(kind.encode()))
+    alpha = AMINO if kind == "protein" else DNA
+    rows, cols = overwrite_set(rng, 70, 3, (200, 260), (40, 300), alpha, plant=0.8,
+                               sub=0.4 if kind == "protein" else 0.05)
+    qs, ts = cols, rows  # 3 queries x 70 targets
+    p = blosum_params(oracle, minimum=25.0) if kind == "protein" else dna_params(oracle, kind == "dna-affine",
+                                                                                  minimum=8.0)
+    info = plan_info(gdev, oracle, qs, ts, p)
+    assert info["orientation"] == 1 and info["lane_per_pair_cells"] > 0, info
+    res = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs > 0
+    res0 = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=0)
     assert np.array_equal(res0.maxima, res.maxima)
     assert np.array_equal(res0.hits, res.hits)
