@@ -161,8 +161,10 @@ int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t candidates_p
  * target, 1 the query, -1 whichever the cost estimate prefers (default).
  * group_max_share: longest lane-per-pair job as a share of the plan's cells
  * per device lane (default 0.25; <= 0 restores it; a large value keeps every
- * job whose lanes are busy enough, e.g. for small plans in tests). */
-int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share);
+ * job whose lanes are busy enough, e.g. for small plans in tests).
+ * dir_residues: largest query residues per lane of the direction pass, 2, 4
+ * or 8 (0: chosen per plan so that the pairs give enough work units). */
+int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share, int32_t dir_residues);
 
 /* Score pass only: pair maxima, the quantity the traceback gate of
  * default_main_cpu.cl:408-410 reads.  maxima_out is [n_pairs]. */

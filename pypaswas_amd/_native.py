@@ -93,7 +93,7 @@ def load() -> ctypes.CDLL:
         L.pw_execute_graph.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.POINTER(ResultRec))]
         L.pw_set_limits.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
-        L.pw_set_routing.argtypes = [vp, ctypes.c_int32, ctypes.c_double]
+        L.pw_set_routing.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
@@ -346,9 +346,10 @@ class Device:
         self._check(self.lib.pw_set_limits(self.ctx, int(max_direction_bytes), int(candidates_per_pair),
                                            int(ops_bytes), 1 if grow else 0), "pw_set_limits")
 
-    def set_routing(self, orient: int = -1, group_max_share: float = 0.0):
-        """Routing rules of batched plans prepared afterwards (pw_set_routing); defaults restore them."""
-        self._check(self.lib.pw_set_routing(self.ctx, int(orient), float(group_max_share)), "pw_set_routing")
+    def set_routing(self, orient: int = -1, group_max_share: float = 0.0, dir_residues: int = 0):
+        """Routing rules of plans prepared afterwards (pw_set_routing); defaults restore them."""
+        self._check(self.lib.pw_set_routing(self.ctx, int(orient), float(group_max_share), int(dir_residues)),
+                    "pw_set_routing")
 
     def score_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                     mode: int = 1):

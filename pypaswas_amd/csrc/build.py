@@ -25,6 +25,9 @@ FLAGS = [
     # strict IEEE single precision: no contraction, no fast math
     "-ffp-contract=off",
     "-fno-fast-math",
+    # every value is finite (pw_set_scoring rejects non-finite or huge
+    # constants), so a maximum needs no signalling-NaN quieting of its inputs
+    "-fno-honor-nans",
     "-Wall",
 ]
 

@@ -120,6 +120,7 @@ struct PlanOpts {
     int force_nb = 0;       // > 0: chunk width of multi-chunk lane-per-pair jobs
     double group_max_share = 0.25;  // longest lane-per-pair job, as a share of the cells per device lane
     int orient = -1;  // column sequence of lane-per-pair jobs: 0 target, 1 query, -1 the cheaper one
+    int dir_rmax = 0;  // > 0: largest residues per lane of the direction pass (2, 4 or 8)
 };
 
 // Bucket k holds lengths up to 4/3 of its representative length rep[k]
@@ -171,6 +172,7 @@ struct pw_ctx {
     Tuning tune[2];  // linear, affine
     int route_orient = -1;        // pw_set_routing: orientation of lane-per-pair jobs (-1: the cheaper)
     double route_share = 0.25;    // pw_set_routing: longest lane-per-pair job (share of cells per lane)
+    int route_dir_rmax = 0;       // pw_set_routing: residues per lane of the direction pass (0: by plan)
     std::vector<HostResult *> result_pool;  // freed results kept for reuse
 };
 
@@ -504,7 +506,7 @@ int make_scoring(std::string &err, const float *subst, int dim, int char_offset,
         err = "scoring: gap and gap extension scores must be <= 0";
         return PW_EINVAL;
     }
-    if (!(gap > GAP_INIT) || !std::isfinite(lower_limit) || !std::isfinite(minimum_score)) {
+    if (!(gap > GAP_INIT) || !(gap_ext > GAP_INIT) || !std::isfinite(lower_limit) || !std::isfinite(minimum_score)) {
         err = "scoring: scoring constants out of range";
         return PW_EINVAL;
     }
@@ -557,14 +559,19 @@ extern "C" int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t c
     return PW_OK;
 }
 
-extern "C" int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share) {
+extern "C" int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share, int32_t dir_residues) {
     if (!ctx) return PW_EINVAL;
     if (orient < -1 || orient > 1) {
         ctx->err = "pw_set_routing: orient must be -1, 0 or 1";
         return PW_EINVAL;
     }
+    if (dir_residues != 0 && dir_residues != 2 && dir_residues != 4 && dir_residues != 8) {
+        ctx->err = "pw_set_routing: dir_residues must be 0, 2, 4 or 8";
+        return PW_EINVAL;
+    }
     ctx->route_orient = orient;
     ctx->route_share = group_max_share > 0 ? group_max_share : 0.25;
+    ctx->route_dir_rmax = dir_residues;
     return PW_OK;
 }
 
@@ -720,7 +727,13 @@ int group_nb(int clen, const Tuning *tu) {
 }
 
 
-size_t lds_bytes(int KP) { return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE; }
+// two transposed substitution tables, the swept-residue window, the staging
+// of the stripe boundary rows (64 columns in, 64 out) and, for the direction
+// pass, the candidate buffer
+size_t lds_bytes(int KP, bool dirs = false) {
+    return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE + 2 * WAVE * sizeof(float4) +
+           (dirs ? CAND_BUF * sizeof(Cand) : 0);
+}
 
 int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
     if (n_pairs < 0 || n_pairs > (int64_t)INT32_MAX) {
@@ -794,8 +807,9 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
     // residues per lane of the direction pass: the largest of 8, 4, 2 for
     // which the pairs, all gated, would give at least 4 work units per SIMD
     // (more, shorter stripes for few long pairs: their waves run latency-bound)
-    P->dir_rmax = 2;
+    P->dir_rmax = opt.dir_rmax > 0 ? opt.dir_rmax : 2;
     for (int r : {8, 4}) {
+        if (opt.dir_rmax > 0) break;
         int64_t units = 0;
         for (int64_t i = 0; i < n_pairs; ++i) {
             const int ql = qlen[pq[i]], tl = tlen[pt[i]];
@@ -1141,7 +1155,7 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
     // passing the gate could use (one per stripe of a split pair), up to the
     // resident limit; split stripes need their producer stripes running
     g.dir_grid = persistent_grid(ctx, (const void *)(aff ? dir_kernel<true> : dir_kernel<false>),
-                                 lds_bytes(P->E.KP), std::max<int64_t>(1, P->units_worst));
+                                 lds_bytes(P->E.KP, true), std::max<int64_t>(1, P->units_worst));
     g.dscr_stride = 3 * (P->max_t + 64);
     g.task_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * 4));
     return g;
@@ -1283,7 +1297,7 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
         return PW_EDEVICE;
     }
     auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
-    hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks,
+    hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP, true), L.s, res, seqs, tasks,
                        (const DirUnit *)W.units.p, ctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
                        P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
                        (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
@@ -1529,7 +1543,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         }
         HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
         auto dk = aff ? dir_kernel<true> : dir_kernel<false>;
-        hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP), L.s, res, seqs, tasks,
+        hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP, true), L.s, res, seqs, tasks,
                            (const DirUnit *)W.units.p, dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
                            P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
                            (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
@@ -1877,6 +1891,7 @@ PlanOpts plan_opts(pw_ctx *ctx, const Scoring &sc) {
     o.tune = &ctx->tune[sc.affine ? 1 : 0];
     o.orient = ctx->route_orient;
     o.group_max_share = ctx->route_share;
+    o.dir_rmax = ctx->route_dir_rmax;
     return o;
 }
 

@@ -102,8 +102,8 @@ __device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need) 
 // ---------------------------------------------------------------------------
 template <bool AFF, int R, bool DIRS>
 __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int alen_it, int blen_it,
-                      const float *subT, int lkp, uint8_t *bwin, float *__restrict__ scr, float G, float E,
-                      int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
+                      const float *subT, int lkp, uint8_t *bwin, float4 *stg, float *__restrict__ scr, float G,
+                      float E, int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
                       Cand *__restrict__ cands, int cand_cap, int s_first, int s_last, float *__restrict__ links,
                       unsigned *__restrict__ prog) {
     const int lane = threadIdx.x & 63;
@@ -117,6 +117,47 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     if (s_last > nstripes) s_last = nstripes;
     // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
     const float thr = __builtin_fmaxf(t1, mp);
+    // LDS staging of the stripe boundary rows, 64 columns at a time: the row
+    // above (read by lane 0, one column per step) and this stripe's bottom row
+    // (written by lane 63, one column per step, stored as one row of 64)
+    float4 *stg_in = stg, *stg_out = stg + WAVE;
+
+    // window of the swept residues for steps [t0, t0 + BWIN): every load is
+    // issued before the first LDS store, so the wave waits for memory once
+    auto load_window = [&](int t0) {
+        constexpr int PER_LANE = (BWIN + WAVE - 1 + WAVE - 1) / WAVE;
+        uint8_t v[PER_LANE];
+#pragma unroll
+        for (int k = 0; k < PER_LANE; ++k) {
+            const int i = lane + k * WAVE;
+            const int c = t0 - (WAVE - 1) + i;
+            v[k] = (i < BWIN + WAVE - 1 && c >= 0 && c < B.len && c < blen_it) ? res[B.off + c] : (uint8_t)fill;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER_LANE; ++k) {
+            const int i = lane + k * WAVE;
+            if (i < BWIN + WAVE - 1) bwin[i] = v[k];
+        }
+        __syncthreads();
+    };
+    // Candidate starts are gathered in LDS and appended to the pair's list
+    // in blocks, one atomic per block instead of one per candidate cell
+    Cand *cbuf = reinterpret_cast<Cand *>(stg + 2 * WAVE);
+    int nbuf = 0;  // wave-uniform
+    auto flush_cands = [&]() {
+        if (nbuf == 0) return;
+        __syncthreads();
+        unsigned base = 0;
+        if (lane == 0) base = (unsigned)atomicAdd(cand_count, nbuf);
+        base = __builtin_amdgcn_readfirstlane(base);
+        for (int k = lane; k < nbuf; k += WAVE) {
+            const int idx = (int)base + k;
+            if (idx < cand_cap) cands[idx] = cbuf[k];
+        }
+        __syncthreads();
+        nbuf = 0;
+    };
 
     for (int s = s_first; s < s_last; ++s) {
         int a4[R];
@@ -125,61 +166,116 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             const int i = s * rows + lane * R + r;
             a4[r] = ((i < A.len) ? (int)res[A.off + i] : fill) * 4;
         }
+        // state of this lane's rows at the previous column: T, I and, for
+        // affine gaps, M (the direction pass) or M + G (the scoring pass)
         float Tp[R], Mp[R], Ip[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Tp[r] = 0.0f;
-            Mp[r] = 0.0f;
+            Mp[r] = DIRS ? 0.0f : 0.0f + G;  // M + G of the virtual column left of the matrix
             Ip[r] = GAP_INIT;
         }
         const bool above = s > 0;
         const bool below = s + 1 < nstripes;
         const float *sin = links ? links + (int64_t)(s - 1) * row_floats : scr;  // row above (when above)
         float *sout = links ? links + (int64_t)s * row_floats : scr;             // bottom row (when below)
-        float inT = 0.0f, inJ = bJ, inM = 0.0f;  // staged previous-stripe row, lane l = column c0+l
-        float oT = 0.0f, oJ = 0.0f, oM = 0.0f;   // staged bottom row to hand down
-        if (links && above && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(WAVE, blen_it));
-        if (above && lane < blen_it) {
-            inT = load_scr(&sin[3 * lane]);
-            if (AFF) inJ = load_scr(&sin[3 * lane + 1]);
-            if (DIRS && AFF) inM = load_scr(&sin[3 * lane + 2]);
+        // columns [c0, c0 + 64) of the row above into LDS (one column per lane)
+        auto stage_in = [&](int c0) {
+            float4 v = make_float4(0.0f, bJ, 0.0f, 0.0f);
+            const int c = c0 + lane;
+            if (c < blen_it) {
+                v.x = load_scr(&sin[3 * c]);
+                if (AFF) v.y = load_scr(&sin[3 * c + 1]);
+                if (DIRS && AFF) v.z = load_scr(&sin[3 * c + 2]);
+            }
+            __syncthreads();
+            stg_in[lane] = v;
+            __syncthreads();
+        };
+        if (above) {
+            if (links && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(WAVE, blen_it));
+            stage_in(0);
         }
         // inputs of step 0: lane 0 reads column 0 of the row above; the other
         // lanes start in the virtual columns left of the matrix
-        float upT = above ? rdlane(inT, 0) : 0.0f;
-        float upJ = above ? rdlane(inJ, 0) : bJ;
-        float upM = above ? rdlane(inM, 0) : 0.0f;
-        if (lane != 0) {
-            upT = 0.0f;
-            upJ = bJ;
-            upM = 0.0f;
+        float upT = 0.0f, upJ = bJ, upM = 0.0f;
+        if (above && lane == 0) {
+            const float4 v = stg_in[0];
+            upT = v.x;
+            upJ = v.y;
+            upM = v.z;
         }
         float dgT = 0.0f, dgM = 0.0f;
 
-        for (int t = 0; t < nsteps; ++t) {
-            const int tw = t & (BWIN - 1);
-            if (tw == 0) {
-                __syncthreads();
-                for (int i = lane; i < BWIN + WAVE - 1; i += WAVE) {
-                    const int c = t - (WAVE - 1) + i;
-                    bwin[i] = (c >= 0 && c < B.len && c < blen_it) ? res[B.off + c] : (uint8_t)fill;
-                }
-                __syncthreads();
-            }
-            const int j = t - lane;  // this lane's column
-            const int boff = ((int)bwin[tw + (WAVE - 1) - lane]) << lkp;
-
-            float td = dgT, md = dgM, tup = upT, jin = upJ;
-            uint32_t bits[R];
+        // substitution scores of step 0; each step reads those of the next
+        // one, so that the LDS latency overlaps a step of arithmetic
+        load_window(0);
+        float scn[R];
+        {
+            const int boff = ((int)bwin[(WAVE - 1) - lane]) << lkp;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float sc = *reinterpret_cast<const float *>(subB + boff + a4[r]);
-                if (AFF) {
-                    const float M = __builtin_fmaxf(td + sc, 0.0f);
-                    const float I = __builtin_fmaxf(Ip[r] + E, Mp[r] + G);
-                    const float T = fmax3(M, I, jin);
-                    rmax = __builtin_fmaxf(rmax, T);
-                    if (DIRS) {
+            for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+        }
+
+        for (int t = 0; t < nsteps; ++t) {
+            const int j = t - lane;  // this lane's column
+            float sc[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) sc[r] = scn[r];
+            const int tn = t + 1;
+            if (tn < nsteps) {
+                if ((tn & (BWIN - 1)) == 0) load_window(tn);
+                const int boff = ((int)bwin[(tn & (BWIN - 1)) + (WAVE - 1) - lane]) << lkp;
+#pragma unroll
+                for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+            }
+            // input of step t+1 for lane 0: column tn of the row above
+            float4 inj = make_float4(0.0f, bJ, 0.0f, 0.0f);
+            if (above && tn < nsteps) {
+                if ((tn & (WAVE - 1)) == 0) {
+                    if (links && tn < blen_it) wait_progress(prog + (s - 1), (unsigned)min(tn + WAVE, blen_it));
+                    stage_in(tn);
+                }
+                inj = stg_in[tn & (WAVE - 1)];
+            }
+
+            float tup = upT, jin = upJ;
+            uint32_t bits[R];
+            if (!DIRS) {
+                // diagonal sums T[i-1][j-1] + s of all of this step's cells
+                // first, so that each row's T is replaced in place
+                float dg[R];
+                dg[0] = dgT + sc[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) dg[r] = Tp[r - 1] + sc[r];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (AFF) {
+                        const float M = __builtin_fmaxf(dg[r], 0.0f);
+                        const float I = __builtin_fmaxf(Ip[r] + E, Mp[r]);  // Mp holds M + G
+                        const float T = fmax3(M, I, jin);
+                        rmax = __builtin_fmaxf(rmax, T);
+                        const float MG = M + G;
+                        jin = __builtin_fmaxf(jin + E, MG);
+                        Tp[r] = T;
+                        Mp[r] = MG;
+                        Ip[r] = I;
+                    } else {
+                        const float h = fmax3(dg[r], __builtin_fmaxf(tup, Tp[r]) + G, 0.0f);
+                        rmax = __builtin_fmaxf(rmax, h);
+                        Tp[r] = h;
+                        tup = h;
+                    }
+                }
+            } else {
+                float td = dgT, md = dgM;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (AFF) {
+                        const float M = __builtin_fmaxf(td + sc[r], 0.0f);
+                        const float I = __builtin_fmaxf(Ip[r] + E, Mp[r] + G);
+                        const float T = fmax3(M, I, jin);
+                        rmax = __builtin_fmaxf(rmax, T);
                         // default_main_cpu.cl:283-345: source = M, I, J in that
                         // priority among equal positive values, none when 0; a
                         // main-matrix cell stops on a zero diagonal predecessor
@@ -188,24 +284,16 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         const uint32_t main = DB_MAIN | (md == 0.0f ? DB_STOP : 0u) | (M >= thr ? DB_CAND : 0u) |
                                               (M <= -thr ? DB_CANDM : 0u);
                         bits[r] = T > 0.0f ? (T == M ? main : (T == I ? 2u : 3u)) : 0u;
-                    }
-                    const float jout = __builtin_fmaxf(jin + E, M + G);
-                    td = Tp[r];
-                    md = Mp[r];
-                    Tp[r] = T;
-                    Mp[r] = M;
-                    Ip[r] = I;
-                    jin = jout;
-                } else {
-                    const float hd = td;
-                    const float d = hd + sc;
-                    if (!DIRS) {
-                        const float h = fmax3(d, __builtin_fmaxf(tup, Tp[r]) + G, 0.0f);
-                        rmax = __builtin_fmaxf(rmax, h);
+                        const float jout = __builtin_fmaxf(jin + E, M + G);
                         td = Tp[r];
-                        Tp[r] = h;
-                        tup = h;
+                        md = Mp[r];
+                        Tp[r] = T;
+                        Mp[r] = M;
+                        Ip[r] = I;
+                        jin = jout;
                     } else {
+                        const float hd = td;
+                        const float d = hd + sc[r];
                         // default_main_cpu.cl:162-177: LEFT (x-1), UP (y-1), UL,
                         // first strict improvement over 0 wins: the value is
                         // max(0, l, u, d) and the code that of the first of
@@ -244,20 +332,24 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                 bool any = false;
 #pragma unroll
                 for (int r = 0; r < R; ++r) any |= (bits[r] & (DB_CAND | DB_CANDM)) != 0;
-                if (any && j >= 0 && j < B.len) {
+                if (__builtin_amdgcn_ballot_w64(any && j >= 0 && j < B.len) != 0) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int x = s * rows + lane * R + r;
-                        if ((bits[r] & (DB_CAND | DB_CANDM)) && x < A.len) {
-                            const int idx = atomicAdd(cand_count, 1);
-                            if (idx < cand_cap) {
-                                Cand c;
-                                c.x = x;
-                                c.y = j;
-                                c.v = Tp[r];
-                                cands[idx] = c;
-                            }
+                        const bool c = (bits[r] & (DB_CAND | DB_CANDM)) && x < A.len && j >= 0 && j < B.len;
+                        const uint64_t mask = __builtin_amdgcn_ballot_w64(c);
+                        if (mask == 0) continue;
+                        if (nbuf > CAND_BUF - WAVE) flush_cands();
+                        if (c) {
+                            const int pos = nbuf + (int)__builtin_amdgcn_mbcnt_hi(
+                                                       (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                            Cand cd;
+                            cd.x = x;
+                            cd.y = j;
+                            cd.v = Tp[r];
+                            cbuf[pos] = cd;
                         }
+                        nbuf += __builtin_popcountll(mask);
                     }
                 }
             }
@@ -265,51 +357,32 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             // bottom row of this lane for column j
             const float botT = Tp[R - 1];
             const float botJ = AFF ? jin : 0.0f;
-            const float botM = Mp[R - 1];
+            const float botM = DIRS ? Mp[R - 1] : 0.0f;
             if (below) {
                 const int c63 = t - (WAVE - 1);
                 if (c63 >= 0 && c63 < blen_it) {
-                    const int sl = c63 & (WAVE - 1);
-                    oT = wrlane(rdlane(botT, WAVE - 1), sl, oT);
-                    if (AFF) oJ = wrlane(rdlane(botJ, WAVE - 1), sl, oJ);
-                    if (DIRS && AFF) oM = wrlane(rdlane(botM, WAVE - 1), sl, oM);
-                    if (sl == WAVE - 1 || c63 == blen_it - 1) {
+                    if (lane == WAVE - 1) stg_out[c63 & (WAVE - 1)] = make_float4(botT, botJ, botM, 0.0f);
+                    if ((c63 & (WAVE - 1)) == WAVE - 1 || c63 == blen_it - 1) {
+                        __syncthreads();
+                        const float4 v = stg_out[lane];
                         const int c = (c63 & ~(WAVE - 1)) + lane;
                         if (c <= c63) {
-                            store_scr(&sout[3 * c], oT);
-                            if (AFF) store_scr(&sout[3 * c + 1], oJ);
-                            if (DIRS && AFF) store_scr(&sout[3 * c + 2], oM);
+                            store_scr(&sout[3 * c], v.x);
+                            if (AFF) store_scr(&sout[3 * c + 1], v.y);
+                            if (DIRS && AFF) store_scr(&sout[3 * c + 2], v.z);
                         }
                         if (links) publish_progress(prog + s, (unsigned)(c63 + 1));
                     }
                 }
             }
             // inputs of step t+1
-            const int tn = t + 1;
-            float injT = 0.0f, injJ = bJ, injM = 0.0f;
-            if (above) {
-                if ((tn & (WAVE - 1)) == 0) {
-                    const int c = tn + lane;
-                    inT = 0.0f;
-                    inJ = bJ;
-                    inM = 0.0f;
-                    if (links && tn < blen_it) wait_progress(prog + (s - 1), (unsigned)min(tn + WAVE, blen_it));
-                    if (c < blen_it) {
-                        inT = load_scr(&sin[3 * c]);
-                        if (AFF) inJ = load_scr(&sin[3 * c + 1]);
-                        if (DIRS && AFF) inM = load_scr(&sin[3 * c + 2]);
-                    }
-                }
-                injT = rdlane(inT, tn & (WAVE - 1));
-                if (AFF) injJ = rdlane(inJ, tn & (WAVE - 1));
-                if (DIRS && AFF) injM = rdlane(inM, tn & (WAVE - 1));
-            }
             dgT = upT;
             dgM = upM;
-            upT = shr1(injT, botT);
-            if (AFF) upJ = shr1(injJ, botJ);
-            if (DIRS && AFF) upM = shr1(injM, botM);
+            upT = shr1(inj.x, botT);
+            if (AFF) upJ = shr1(inj.y, botJ);
+            if (DIRS && AFF) upM = shr1(inj.z, botM);
         }
+        if (DIRS) flush_cands();
     }
     return rmax;
 }
@@ -322,12 +395,13 @@ __device__ __forceinline__ void load_tables(float *lds, const float *__restrict_
 
 template <bool AFF, int R>
 __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *seqs, const ScoreItem &it,
-                                            const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float *scr,
+                                            const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float4 *stg,
+                                            float *scr,
                                             float G, float E, int fill, int trans, float *links, unsigned *prog) {
     const float *tab = lds_tab + (trans ? kp2 : 0);
     float *lk = it.rows_off >= 0 ? links + it.rows_off : nullptr;
     unsigned *pg = it.rows_off >= 0 ? prog + it.prog_off : nullptr;
-    return tile<AFF, R, false>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, scr, G, E,
+    return tile<AFF, R, false>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, stg, scr, G, E,
                                fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg);
 }
 
@@ -350,6 +424,7 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
     uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
+    float4 *stg = reinterpret_cast<float4 *>(bwin + BWIN + WAVE);
     float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
     int k = 0;
@@ -368,10 +443,10 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
         const int tr = trans_flags ? trans_flags[item] : 0;
         float m;
         switch (it.R) {
-        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
-        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
-        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
-        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, scr, G, E, fill, tr, links, prog); break;
+        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
+        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
+        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
+        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
         }
         m = wave_max(m);
         if (threadIdx.x == 0) {
@@ -565,6 +640,7 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
     uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
+    float4 *stg = reinterpret_cast<float4 *>(bwin + BWIN + WAVE);
     float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
     const int n = (int)min((int64_t)__hip_atomic_load(&ctl->units_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
@@ -586,10 +662,10 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
         float *lk = du.stripe < 0 ? nullptr : links + it.rows_off;
         unsigned *pg = du.stripe < 0 ? nullptr : prog + it.prog_off;
         switch (it.R) {
-        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
         }
     }
 }

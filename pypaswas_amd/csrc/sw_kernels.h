@@ -15,6 +15,7 @@ constexpr float GAP_INIT = -1e10f;  // AFFINE_GAP_INIT, default_main_cpu.cl:24
 
 constexpr int WAVE = 64;
 constexpr int BWIN = 1024;  // swept residues staged in LDS per refill
+constexpr int CAND_BUF = 128;  // candidate starts gathered in LDS per wave before an append
 
 struct SeqRef {
     int64_t off;
