@@ -1228,10 +1228,27 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
     const SeqRef *seqs = (const SeqRef *)P->seqs.p;
     const float *tabs = (const float *)P->tables.p;
     float *mx = (float *)W.maxima.p;
+    // the lane-per-pair kernels of the chunk widths present alternate between
+    // the lane's two streams, so that one kernel's tail overlaps the next
+    // kernel's work (only the multi-chunk width uses the per-block scratch)
+    bool forked = false;
+    auto fork = [&]() -> int {
+        if (forked || L.s2 == L.s) return PW_OK;
+        HIPCHK(ctx, hipEventRecord(L.fork, L.s));
+        HIPCHK(ctx, hipStreamWaitEvent(L.s2, L.fork, 0));
+        forked = true;
+        return PW_OK;
+    };
     if (P->n_groups > 0) {
+        int nk = 0;
         for (int w = 0; w < NB_KINDS; ++w) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
             if (n <= 0) continue;
+            hipStream_t gs = L.s;
+            if (P->mode == 1 && (nk++ & 1) && L.s2 != L.s) {
+                if (fork()) return PW_EDEVICE;
+                gs = L.s2;
+            }
             const void *kern = group_kernel_for(aff, NB_WIDTHS[w]);
             const GroupItem *items = (const GroupItem *)P->d_gitems.p + P->nb_begin[w];
             const GroupEntry *ents = (const GroupEntry *)P->d_gents.p;
@@ -1241,7 +1258,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             int64_t ss = g.gscr_stride;
             unsigned *q = ctl->queue + 16 * (1 + w);
             void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &gp, &e, &scr, &ss, &q, &mx};
-            HIPCHK(ctx, hipLaunchKernel(kern, dim3(g.group_grid[w]), dim3(64), args, group_lds(P, w), L.s));
+            HIPCHK(ctx, hipLaunchKernel(kern, dim3(g.group_grid[w]), dim3(64), args, group_lds(P, w), gs));
             if (launches) *launches += 1;
         }
     }
@@ -1265,20 +1282,17 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
         } else {
             // persistent workers on the second stream, overlapping the lane-per-pair kernels
             hipStream_t st = P->n_groups > 0 ? L.s2 : L.s;
-            if (st != L.s) {
-                HIPCHK(ctx, hipEventRecord(L.fork, L.s));
-                HIPCHK(ctx, hipStreamWaitEvent(st, L.fork, 0));
-            }
+            if (st != L.s && fork()) return PW_EDEVICE;
             hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
                                tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
                                ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p);
             HIPCHK(ctx, hipGetLastError());
             if (launches) *launches += 1;
-            if (st != L.s) {
-                HIPCHK(ctx, hipEventRecord(L.join, st));
-                HIPCHK(ctx, hipStreamWaitEvent(L.s, L.join, 0));
-            }
         }
+    }
+    if (forked) {
+        HIPCHK(ctx, hipEventRecord(L.join, L.s2));
+        HIPCHK(ctx, hipStreamWaitEvent(L.s, L.join, 0));
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[1], L.s));
     return PW_OK;
