@@ -117,6 +117,7 @@ struct PlanOpts {
     int force_R = 0;        // > 0: every tile item uses these residues per lane, untransposed
     bool force_tile = false;  // every pair to the tile kernel
     bool force_group = false; // every pair to lane-per-pair jobs (no fill or length rule)
+    bool no_group_split = false;  // no lane-per-pair job split into chunk items
     int force_nb = 0;       // > 0: chunk width of multi-chunk lane-per-pair jobs
     double group_max_share = 0.25;  // longest lane-per-pair job, as a share of the cells per device lane
     int orient = -1;  // column sequence of lane-per-pair jobs: 0 target, 1 query, -1 the cheaper one
@@ -231,7 +232,8 @@ struct pw_plan {
     int64_t dlinks_worst = 0;  // boundary-row floats of the direction pass in that case
     int64_t dprog_worst = 0;   // and its progress counters
     int32_t dir_rmax = 8;      // largest residues per lane of the direction pass
-    int64_t n_groups = 0;      // lane-per-pair jobs
+    int64_t n_groups = 0;      // lane-per-pair items (a split job counts one per chunk)
+    int64_t group_split_items = 0;  // of which items of split jobs
     int32_t orient = 0;        // column sequence of the lane-per-pair jobs: 0 target, 1 query
     double group_cells = 0;    // cells scored lane-per-pair
     int64_t group_rows = 0;    // longest row sequence of a multi-chunk job (scratch sizing)
@@ -717,6 +719,10 @@ constexpr double GROUP_MIN_FILL = 0.6;
 // cost of a cell in the wave-per-pair tile kernel relative to a lane-cell of a
 // lane-per-pair job (measured on MI355X: ~2.5x)
 constexpr double TILE_COST_FACTOR = 2.5;
+// a multi-chunk job longer than the length rule allows runs as one item per
+// chunk in a row pipeline if its waves would wait at most this share of the
+// job's work at the start (GROUP_PUB rows per chunk of the pipeline)
+constexpr double GROUP_SPLIT_MAX_WAIT = 0.2;
 constexpr int GROUP_MAX_ROWS = 1 << 15;
 
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
@@ -855,6 +861,7 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
     P->gitems.clear();
     P->gents.clear();
     P->n_groups = 0;
+    P->group_split_items = 0;
     P->orient = 0;
     P->group_cells = 0;
     P->group_rows = 0;
@@ -935,7 +942,10 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                 // long per cell as the whole SIMD needs per lane-cell, so a longer
                 // job would set the makespan (split tiles finish it sooner);
                 // otherwise its pairs go to the tile kernel
+                const int nb = (opt.force_nb > 0 && cl > 96) ? opt.force_nb : group_nb(cl, opt.tune);
+                const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
                 std::vector<int64_t> grp;
+                std::vector<uint8_t> grp_split;  // per kept job
                 for (size_t g0 = 0; g0 < cand.size(); g0 += 64) {
                     const size_t g1 = std::min(cand.size(), g0 + 64);
                     double sum = 0;
@@ -943,15 +953,21 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                     const int top = std::max(1, row_len(cand[g0]));
                     const double fill = sum / (64.0 * top);
                     const double job_steps = (double)top * cl;
-                    const bool keep =
+                    bool keep =
                         opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
-                    cost += keep ? 64.0 * job_steps : TILE_COST_FACTOR * sum * cl;
-                    if (commit)
+                    // too long for one wave: one item per chunk, pipelined over rows
+                    const bool split = !keep && !opt.no_group_split && fill >= GROUP_MIN_FILL && nch > 1 &&
+                                       (double)top * nb <= opt.group_max_share * per_lane &&
+                                       (double)GROUP_PUB * (nch - 1) <= 2.0 * GROUP_SPLIT_MAX_WAIT * top;
+                    keep = keep || split;
+                    const double wait = split ? (double)GROUP_PUB * (nch - 1) / (2.0 * top) : 0.0;
+                    cost += keep ? 64.0 * job_steps * (1.0 + wait) : TILE_COST_FACTOR * sum * cl;
+                    if (commit) {
                         for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
+                        if (keep) grp_split.push_back(split ? 1 : 0);
+                    }
                 }
                 if (!commit) continue;
-                const int nb = (opt.force_nb > 0 && cl > 96) ? opt.force_nb : group_nb(cl, opt.tune);
-                const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
                 for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
                     const int cnt = (int)std::min<size_t>(64, grp.size() - g0);
                     GroupItem gi;
@@ -961,14 +977,21 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                     gi.rows = row_len(grp[g0]);
                     gi.nb = nb;
                     gi.trans = o;
+                    gi.k0 = 0;
+                    gi.k1 = nch;
+                    gi.link_in = gi.link_out = -1;
+                    gi.prog_in = gi.prog_out = -1;
+                    gi.split = grp_split[g0 / 64];
+                    gi.pad = 0;
                     for (int u = 0; u < cnt; ++u) {
                         const int64_t i = grp[g0 + u];
                         gents.push_back(GroupEntry{o ? nq + pt[i] : pq[i], (int32_t)i});
                     }
                     gitems.push_back(gi);
-                    gcost.push_back((double)gi.rows * nch * nb);
+                    // critical path: the whole job in one wave, or the row pipeline of a split one
+                    gcost.push_back(gi.split ? (double)(gi.rows + GROUP_PUB * (nch - 1)) * nb : (double)gi.rows * nch * nb);
                     for (int u = 0; u < cnt; ++u) P->group_cells += (double)row_len(grp[g0 + u]) * cl;
-                    if (nch > 1) P->group_rows = std::max<int64_t>(P->group_rows, gi.rows);
+                    if (nch > 1 && !gi.split) P->group_rows = std::max<int64_t>(P->group_rows, gi.rows);
                     P->max_nb = std::max(P->max_nb, nb);
                 }
             }
@@ -985,16 +1008,10 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
             if (gitems[x].nb != gitems[y].nb) return gitems[x].nb < gitems[y].nb;
             return gcost[x] > gcost[y];
         });
-        P->gitems.resize(gitems.size());
-        for (size_t i = 0; i < gorder.size(); ++i) P->gitems[i] = gitems[gorder[i]];
-        for (int w = 0; w < NB_KINDS; ++w) {
-            for (size_t i = 0; i < P->gitems.size(); ++i)
-                if (P->gitems[i].nb == NB_WIDTHS[w]) {
-                    if (P->nb_end[w] == 0) P->nb_begin[w] = (int64_t)i;
-                    P->nb_end[w] = (int64_t)i + 1;
-                }
-        }
-        P->n_groups = (int64_t)P->gitems.size();
+        std::vector<GroupItem> gjobs(gitems.size());
+        for (size_t i = 0; i < gorder.size(); ++i) gjobs[i] = gitems[gorder[i]];
+        // split jobs are expanded into their chunk items (below) after the
+        // tile items have taken their link rows
 
         // pairs with several stripes and long rows are split into one item
         // per stripe (consecutive, in stripe order), run by as many waves in a
@@ -1053,6 +1070,43 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                 trans.push_back(ttrans[o]);
             }
         }
+
+        // lane-per-pair jobs in order; a split job becomes one item per chunk,
+        // consecutive in chunk order (the persistent workers dequeue in order,
+        // so the item a wave waits for is held by a running wave), with a link
+        // column of rows x 64 (T, I) pairs and a progress counter per boundary
+        P->score_links = (P->score_links + 1) / 2 * 2;  // float2 alignment
+        P->gitems.clear();
+        for (const GroupItem &j : gjobs) {
+            if (!j.split) {
+                P->gitems.push_back(j);
+                continue;
+            }
+            const int64_t col = (int64_t)j.rows * WAVE;  // float2 per link column
+            const int64_t link0 = P->score_links / 2;
+            const int32_t prog0 = (int32_t)P->score_prog;
+            for (int k = j.k0; k < j.k1; ++k) {
+                GroupItem it = j;
+                it.k0 = k;
+                it.k1 = k + 1;
+                it.link_in = k > j.k0 ? link0 + (int64_t)(k - 1 - j.k0) * col : -1;
+                it.prog_in = k > j.k0 ? prog0 + (k - 1 - j.k0) : -1;
+                it.link_out = k + 1 < j.k1 ? link0 + (int64_t)(k - j.k0) * col : -1;
+                it.prog_out = k + 1 < j.k1 ? prog0 + (k - j.k0) : -1;
+                P->gitems.push_back(it);
+            }
+            P->score_links += 2 * (int64_t)(j.k1 - j.k0 - 1) * col;
+            P->score_prog += j.k1 - j.k0 - 1;
+            P->group_split_items += j.k1 - j.k0;
+        }
+        for (int w = 0; w < NB_KINDS; ++w) {
+            for (size_t i = 0; i < P->gitems.size(); ++i)
+                if (P->gitems[i].nb == NB_WIDTHS[w]) {
+                    if (P->nb_end[w] == 0) P->nb_begin[w] = (int64_t)i;
+                    P->nb_end[w] = (int64_t)i + 1;
+                }
+        }
+        P->n_groups = (int64_t)P->gitems.size();
     }
     P->max_b = max_b;
     P->n_tile = (int64_t)items.size();
@@ -1257,7 +1311,9 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             float2 *scr = (float2 *)W.gscratch.p;
             int64_t ss = g.gscr_stride;
             unsigned *q = ctl->queue + 16 * (1 + w);
-            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &gp, &e, &scr, &ss, &q, &mx};
+            float2 *lk = (float2 *)P->d_links.p;
+            unsigned *pg = (unsigned *)P->d_prog.p;
+            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &gp, &e, &scr, &ss, &q, &mx, &lk, &pg};
             HIPCHK(ctx, hipLaunchKernel(kern, dim3(g.group_grid[w]), dim3(64), args, group_lds(P, w), gs));
             if (launches) *launches += 1;
         }
@@ -2238,15 +2294,16 @@ extern "C" int pw_execute_graph(pw_ctx *ctx, pw_plan *plan, double minimum, int3
 
 extern "C" int pw_plan_info(const pw_plan *plan, int64_t *info, int32_t n) {
     if (!plan || !info || n < 0) return PW_EINVAL;
-    const int64_t v[8] = {plan->n_pairs,
+    const int64_t v[9] = {plan->n_pairs,
                           plan->n_groups,
                           plan->n_tile,
                           plan->orient,
                           (int64_t)plan->group_cells,
                           (int64_t)(plan->cells - plan->group_cells),
                           plan->mode == 1 ? plan->max_nb : 0,
-                          plan->dir_rmax};
-    for (int32_t i = 0; i < n && i < 8; ++i) info[i] = v[i];
+                          plan->dir_rmax,
+                          plan->group_split_items};
+    for (int32_t i = 0; i < n && i < 9; ++i) info[i] = v[i];
     return PW_OK;
 }
 

@@ -77,6 +77,19 @@ __device__ __forceinline__ void publish_progress(unsigned *p, unsigned v) {
     if ((threadIdx.x & 63) == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// As wait_progress, returning the count it saw (at least need).
+__device__ __forceinline__ unsigned wait_rows(const unsigned *p, unsigned need) {
+    for (;;) {
+        const unsigned v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (v >= need) {
+            __builtin_amdgcn_s_waitcnt(0);
+            return v;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
 __device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need) {
     for (;;) {
         const unsigned v = __builtin_amdgcn_readfirstlane(
@@ -473,7 +486,8 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
 template <bool AFF, int NB>
 __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs, const GroupItem &it,
                             const GroupEntry *__restrict__ ents, const float *__restrict__ gtab, int kp,
-                            float *prof, float2 *__restrict__ scr, float G, float E, int fill) {
+                            float *prof, float2 *__restrict__ scr, float2 *__restrict__ links,
+                            unsigned *__restrict__ prog, float G, float E, int fill) {
     constexpr int NBP = NB + 4;
     const int lane = threadIdx.x & 63;
     const SeqRef C = seqs[it.c_seq];
@@ -486,8 +500,17 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
     const float *tab = gtab + (it.trans ? kp * kp : 0);
     const float bJ = AFF ? G : 0.0f;
     float rmax = 0.0f;
-    for (int k = 0; k < nchunks; ++k) {
+    const int k_end = min(it.k1, nchunks);
+    for (int k = it.k0; k < k_end; ++k) {
         const int c0 = k * NB;
+        // the left edge comes from the previous item's link column (published
+        // row by row) for the first chunk of a split item, else from the
+        // wave's own scratch column; likewise the right edge goes out
+        const bool from_link = k > 0 && k == it.k0 && it.link_in >= 0;
+        const bool to_link = k + 1 < nchunks && k == k_end - 1 && it.link_out >= 0;
+        const float2 *src = from_link ? links + it.link_in : scr;
+        float2 *dst = to_link ? links + it.link_out : scr;
+        unsigned avail = from_link ? 0u : 0x7fffffffu;  // rows of src known to be published
         __syncthreads();
         // profile row of residue a, shifted by one column: entry p < NB holds
         // the score of column c0+p+1, entry NB the score of column c0; so the
@@ -511,8 +534,9 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
         int a_next = (active && 0 < Rw.len) ? (int)res[Rw.off] : fill;
         float2 lb_next = make_float2(0.0f, bJ);
         if (left_in && it.rows > 0) {
-            lb_next.x = load_scr(&scr[lane].x);
-            lb_next.y = load_scr(&scr[lane].y);
+            if (from_link) avail = wait_rows(prog + it.prog_in, 1u);
+            lb_next.x = load_scr(&src[lane].x);
+            lb_next.y = load_scr(&src[lane].y);
         }
         // profile values of the first row: columns 1..4 and column 0
         const float *prow = prof + a_next * NBP;
@@ -523,8 +547,9 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
             const float2 lb = lb_next;
             a_next = (active && r + 1 < Rw.len) ? (int)res[Rw.off + r + 1] : fill;
             if (left_in && r + 1 < it.rows) {
-                lb_next.x = load_scr(&scr[(r + 1) * WAVE + lane].x);
-                lb_next.y = load_scr(&scr[(r + 1) * WAVE + lane].y);
+                if ((unsigned)(r + 2) > avail) avail = wait_rows(prog + it.prog_in, (unsigned)(r + 2));
+                lb_next.x = load_scr(&src[(r + 1) * WAVE + lane].x);
+                lb_next.y = load_scr(&src[(r + 1) * WAVE + lane].y);
             }
             float ic = lb.y;  // affine: I candidate entering the column; linear: H[r][c-1]
             // diagonal sum of the block's first column, T[r-1][c-1] + s; each
@@ -580,8 +605,10 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                 asm volatile("" : "+v"(rmax), "+v"(ic));
             }
             if (right_out) {
-                store_scr(&scr[r * WAVE + lane].x, Tp[NB - 1]);
-                store_scr(&scr[r * WAVE + lane].y, ic);
+                store_scr(&dst[r * WAVE + lane].x, Tp[NB - 1]);
+                store_scr(&dst[r * WAVE + lane].y, ic);
+                if (to_link && (((r + 1) % GROUP_PUB) == 0 || r + 1 == it.rows))
+                    publish_progress(prog + it.prog_out, (unsigned)(r + 1));
             }
             dl = lb.x;
         }
@@ -597,7 +624,8 @@ __global__ void __launch_bounds__(64) group_kernel(const uint8_t *__restrict__ r
                                                    const GroupEntry *__restrict__ ents,
                                                    const float *__restrict__ tables, int kp, float G, float E,
                                                    float2 *__restrict__ scratch, int64_t scr_stride,
-                                                   unsigned *__restrict__ queue, float *__restrict__ maxima) {
+                                                   unsigned *__restrict__ queue, float *__restrict__ maxima,
+                                                   float2 *__restrict__ links, unsigned *__restrict__ prog) {
     extern __shared__ float lds[];
     float2 *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
@@ -614,8 +642,15 @@ __global__ void __launch_bounds__(64) group_kernel(const uint8_t *__restrict__ r
         }
         if (item >= nitems) break;
         const GroupItem it = items[item];
-        const float m = group_tile<AFF, NB>(res, seqs, it, ents, tables, kp, lds, scr, G, E, fill);
-        if ((int)threadIdx.x < it.count) maxima[ents[it.entry_off + threadIdx.x].pair] = m;
+        const float m = group_tile<AFF, NB>(res, seqs, it, ents, tables, kp, lds, scr, links, prog, G, E, fill);
+        if ((int)threadIdx.x < it.count) {
+            const int pair = ents[it.entry_off + threadIdx.x].pair;
+            // a split job's maximum: the maximum over its items, combined with
+            // an integer atomic max on the (non-negative) float bits into a
+            // slot zeroed at the start of the execution
+            if (it.split) atomicMax(reinterpret_cast<int *>(maxima) + pair, __float_as_int(m));
+            else maxima[pair] = m;
+        }
     }
 }
 

@@ -51,6 +51,10 @@ struct DirUnit {
 
 // One job of the lane-per-pair scoring kernel: a column sequence shared by up
 // to 64 pairs (one per lane), each lane sweeping its own row sequence.
+// A job too long for one wave is split into one item per chunk, consecutive in
+// chunk order; item k reads the left edge of its chunk (T and the I candidate
+// of every row) from the link column item k-1 writes, row by row, and waits on
+// that column's progress counter (rows published).
 struct GroupItem {
     int32_t c_seq;      // column sequence (held as a substitution profile in LDS)
     int32_t entry_off;  // first GroupEntry of this job
@@ -58,7 +62,15 @@ struct GroupItem {
     int32_t rows;       // longest row sequence of the job
     int32_t nb;         // columns per chunk: 16, 32, 48, 64, 80 or 96
     int32_t trans;      // 1 when the column sequence is the query
+    int32_t k0, k1;     // chunks [k0, k1) of the column sequence
+    int64_t link_in;    // float2 offset of the link column read by chunk k0 (-1: none)
+    int64_t link_out;   // float2 offset of the link column written by chunk k1-1 (-1: none)
+    int32_t prog_in;    // progress counter of link_in (-1: none)
+    int32_t prog_out;   // progress counter of link_out (-1: none)
+    int32_t split;      // 1: the job spans several items (maxima combined with an atomic max)
+    int32_t pad;
 };
+constexpr int GROUP_PUB = 16;  // rows of a link column published at a time
 
 struct GroupEntry {
     int32_t row_seq;

@@ -165,3 +165,38 @@ def test_group_kernel_query_columns(gdev, oracle, kind):
     res0 = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=0)
     assert np.array_equal(res0.maxima, res.maxima)
     assert np.array_equal(res0.hits, res.hits)
+
+
+@pytest.fixture(scope="module")
+def sdev():
+    """A context whose length rule is tight enough for a small plan's jobs to be
+    split into one item per chunk (pipelined over rows), and loose enough for
+    one chunk per wave."""
+    from pypaswas_amd._native import Device
+    d = Device(0)
+    d.set_routing(1, 150.0)
+    yield d
+    d.close()
+
+
+@pytest.mark.parametrize("kind", ["dna-linear", "dna-affine", "protein"])
+def test_group_kernel_split_jobs(sdev, oracle, kind):
+    """Lane-per-pair jobs split into chunk items that hand their right edge to the
+    next item row by row: same maxima, starting points and walks as the oracle and
+    as the plain launch."""
+    print("case", kind, flush=True)
+    rng = np.random.default_rng(zlib.crc32(("split-" + kind).encode()))
+    alpha = AMINO if kind == "protein" else DNA
+    rows, cols = overwrite_set(rng, 70, 3, (200, 260), (280, 420), alpha, plant=0.8,
+                               sub=0.4 if kind == "protein" else 0.05)
+    qs, ts = cols, rows  # 3 queries (the split column sequences) x 70 targets
+    p = blosum_params(oracle, minimum=25.0) if kind == "protein" else dna_params(oracle, kind == "dna-affine",
+                                                                                  minimum=8.0)
+    info = plan_info(sdev, oracle, qs, ts, p)
+    assert info["orientation"] == 1 and info["split_job_items"] > 0, info
+    res = hip_all_vs_all(sdev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs > 0
+    res0 = hip_all_vs_all(sdev, oracle, qs, ts, p, mode=0)
+    assert np.array_equal(res0.maxima, res.maxima)
+    assert np.array_equal(res0.hits, res.hits)
