@@ -141,9 +141,12 @@ struct HostResult {
     float *maxima = nullptr;  // pinned host memory of maxima_cap floats
     size_t maxima_cap = 0;
     std::vector<float> maxima_vec;  // maxima of a served job (when maxima is NULL)
+    // records; the vectors only grow (a recycled result fills them without
+    // touching or zeroing new memory), n_sps and n_ops are the sizes in use
     std::vector<pw_starting_point> sps;
     std::vector<pw_hit> hits;
     std::vector<char> ops;
+    int64_t n_sps = 0, n_ops = 0;
 };
 
 struct pw_ctx {
@@ -1717,9 +1720,11 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostR
         nsp += o.nsp;
         nops += o.nops;
     }
-    R->sps.resize((size_t)nsp);
-    R->hits.resize((size_t)nsp);
-    R->ops.resize((size_t)nops);
+    if ((int64_t)R->sps.size() < nsp) R->sps.resize((size_t)nsp);
+    if ((int64_t)R->hits.size() < nsp) R->hits.resize((size_t)nsp);
+    if ((int64_t)R->ops.size() < nops) R->ops.resize((size_t)nops);
+    R->n_sps = nsp;
+    R->n_ops = nops;
     static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
     int64_t sp0 = 0, op0 = 0;
     for (const PairOut &o : outs) {
@@ -1919,11 +1924,11 @@ void fill_public(HostResult *R, const float *maxima, int64_t n_pairs, double cel
     pw_result &O = R->pub;
     O.n_pairs = n_pairs;
     O.maxima = maxima;
-    O.n_starting_points = (int64_t)R->sps.size();
+    O.n_starting_points = R->n_sps;
     O.starting_points = R->sps.data();
-    O.n_hits = (int64_t)R->hits.size();
+    O.n_hits = R->n_sps;
     O.hits = R->hits.data();
-    O.n_ops = (int64_t)R->ops.size();
+    O.n_ops = R->n_ops;
     O.ops = R->ops.data();
     O.n_traceback_pairs = ng;
     O.cells = cells;
@@ -1987,9 +1992,7 @@ int new_result(pw_ctx *ctx, HostResult **out) {
         R = new HostResult();
     }
     std::memset(&R->pub, 0, sizeof(R->pub));
-    R->sps.clear();
-    R->hits.clear();
-    R->ops.clear();
+    R->n_sps = R->n_ops = 0;
     R->maxima_vec.clear();
     *out = R;
     return PW_OK;

@@ -221,11 +221,11 @@ void split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j,
     }
     for (PairOut &o : mine) o.pair -= (int32_t)j->p0;
     assemble(mine, j->sp_cap, (int64_t)j->pq.size(), R);
-    for (pw_starting_point &sp : R->sps) {
+    for (int64_t k = 0; k < R->n_sps; ++k) {
+        pw_starting_point &sp = R->sps[(size_t)k];
         sp.sequence -= (uint32_t)j->q0;
         sp.target -= (uint32_t)j->t0;
-    }
-    for (pw_hit &h : R->hits) {
+        pw_hit &h = R->hits[(size_t)k];
         h.query -= j->q0;
         h.target -= j->t0;
     }
