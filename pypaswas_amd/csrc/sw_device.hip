@@ -254,6 +254,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
 
             float tup = upT, jin = upJ;
             uint32_t bits[R];
+            bool cand[R];  // direction pass: the cell is a candidate start
             if (!DIRS) {
                 // diagonal sums T[i-1][j-1] + s of all of this step's cells
                 // first, so that each row's T is replaced in place
@@ -293,10 +294,17 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         // priority among equal positive values, none when 0; a
                         // main-matrix cell stops on a zero diagonal predecessor
                         // and is a candidate start when M (or -M, for a marked
-                        // cell) reaches both thresholds
-                        const uint32_t main = DB_MAIN | (md == 0.0f ? DB_STOP : 0u) | (M >= thr ? DB_CAND : 0u) |
-                                              (M <= -thr ? DB_CANDM : 0u);
-                        bits[r] = T > 0.0f ? (T == M ? main : (T == I ? 2u : 3u)) : 0u;
+                        // cell) reaches both thresholds (the two candidate
+                        // flags go with the candidate record, not the byte).
+                        // Every value is formed, then selected: no divergent
+                        // branch per cell.
+                        uint32_t main = DB_MAIN | (md == 0.0f ? DB_STOP : 0u);
+                        uint32_t other = T == I ? 2u : 3u;
+                        asm volatile("" : "+v"(main), "+v"(other));
+                        const bool isM = T == M, pos = T > 0.0f;
+                        bits[r] = pos ? (isM ? main : other) : 0u;
+                        // M >= 0, so "M >= thr or M <= -thr" is M >= thr
+                        cand[r] = pos && isM && M >= thr;
                         const float jout = __builtin_fmaxf(jin + E, M + G);
                         td = Tp[r];
                         md = Mp[r];
@@ -317,9 +325,12 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         // diagonal cells are candidate starts
                         const bool isl = cur == l, isu = cur == u;
                         const float pv = isl ? tup : (isu ? Tp[r] : hd);
-                        const uint32_t diag = DB_MAIN | (cur >= thr ? DB_CAND : 0u) | (cur <= -thr ? DB_CANDM : 0u);
-                        const uint32_t b = (isl ? 3u : (isu ? 2u : diag)) | (pv == 0.0f ? DB_STOP : 0u);
-                        bits[r] = cur > 0.0f ? b : 0u;
+                        uint32_t stop = pv == 0.0f ? DB_STOP : 0u;
+                        asm volatile("" : "+v"(stop));
+                        const uint32_t b = (isl ? 3u : (isu ? 2u : (uint32_t)DB_MAIN)) | stop;
+                        const bool pos = cur > 0.0f;
+                        bits[r] = pos ? b : 0u;
+                        cand[r] = pos && !isl && !isu && cur >= thr;  // cur >= 0 (as M above)
                         rmax = __builtin_fmaxf(rmax, cur);
                         td = Tp[r];
                         Tp[r] = cur;
@@ -344,12 +355,12 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                 }
                 bool any = false;
 #pragma unroll
-                for (int r = 0; r < R; ++r) any |= (bits[r] & (DB_CAND | DB_CANDM)) != 0;
+                for (int r = 0; r < R; ++r) any |= cand[r];
                 if (__builtin_amdgcn_ballot_w64(any && j >= 0 && j < B.len) != 0) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int x = s * rows + lane * R + r;
-                        const bool c = (bits[r] & (DB_CAND | DB_CANDM)) && x < A.len && j >= 0 && j < B.len;
+                        const bool c = cand[r] && x < A.len && j >= 0 && j < B.len;
                         const uint64_t mask = __builtin_amdgcn_ballot_w64(c);
                         if (mask == 0) continue;
                         if (nbuf > CAND_BUF - WAVE) flush_cands();
@@ -357,9 +368,10 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                             const int pos = nbuf + (int)__builtin_amdgcn_mbcnt_hi(
                                                        (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                             Cand cd;
-                            cd.x = x;
+                            const float v = Tp[r];
+                            cd.x = x | (v >= thr ? CAND_PLAIN : 0) | (v <= -thr ? CAND_MARKED : 0);
                             cd.y = j;
-                            cd.v = Tp[r];
+                            cd.v = v;
                             cbuf[pos] = cd;
                         }
                         nbuf += __builtin_popcountll(mask);
@@ -952,18 +964,22 @@ __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs,
         const int tlen = seqs[it.t_seq].len;
         Cand *c = cands + it.cand_off;
         const int nc = it.ncand;
-        heap_sort(c, nc, [](const Cand &a, const Cand &b) { return order_key(a.x, a.y) > order_key(b.x, b.y); });
+        constexpr int32_t XM = ~(CAND_PLAIN | CAND_MARKED);
+        heap_sort(c, nc, [](const Cand &a, const Cand &b) {
+            return order_key(a.x & XM, a.y) > order_key(b.x & XM, b.y);
+        });
         SpRec *out = sps + it.cand_off;
         int nsp = 0;
         for (int i = 0; i < nc; ++i) {
-            int x = c[i].x, y = c[i].y;
+            const int fl = c[i].x & (CAND_PLAIN | CAND_MARKED);
+            int x = c[i].x & ~(CAND_PLAIN | CAND_MARKED), y = c[i].y;
             const float v = c[i].v;
             const int64_t off = cell_offset(it, tlen, x, y);
             const uint8_t b = dirs[off];
             float score;
             bool emit = false;
             if (!(b & DB_MARK)) {
-                if (b & DB_CAND) {
+                if (fl & CAND_PLAIN) {
                     emit = true;
                     score = v;
                     // mark the path
@@ -983,7 +999,7 @@ __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs,
                         cy = py;
                     }
                 }
-            } else if (b & DB_CANDM) {
+            } else if (fl & CAND_MARKED) {
                 emit = true;
                 score = -v;
             }

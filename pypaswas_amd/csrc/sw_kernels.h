@@ -122,10 +122,13 @@ struct Ctl {
     long long units_used;                     // work units of the direction pass
 };
 
+// A candidate start: x carries two flags in its top bits, whether the cell
+// qualifies by its value (unmarked) and by its negated value (once marked)
 struct Cand {
     int32_t x, y;
     float v;
 };
+constexpr int32_t CAND_PLAIN = 1 << 29, CAND_MARKED = 1 << 28;
 
 struct SpRec {
     uint32_t sequence, target, blockX, blockY, valueX, valueY;
@@ -146,10 +149,9 @@ struct HitRec {
 // direction byte of the direction pass
 //   bits 1:0  code: 0 none, 1 upper-left / main matrix, 2 upper / I matrix, 3 left / J matrix
 //   bit 2     the predecessor along the code has value 0 (a mark here becomes STOP)
-//   bit 3     candidate start with the unmarked value
-//   bit 4     candidate start with the sign-marked value
+//   bits 4:3  unused (the candidate flags travel in the candidate record)
 //   bit 5     marked by the traceback
 //   bit 6     consumed by the walk (IN_ALIGNMENT)
-constexpr uint8_t DB_MAIN = 1, DB_STOP = 4, DB_CAND = 8, DB_CANDM = 16, DB_MARK = 32, DB_WALK = 64;
+constexpr uint8_t DB_MAIN = 1, DB_STOP = 4, DB_MARK = 32, DB_WALK = 64;
 
 }  // namespace pw
