@@ -26,6 +26,7 @@
 #include <map>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pypaswas_hip.h"
@@ -1726,29 +1727,61 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostR
     R->n_sps = nsp;
     R->n_ops = nops;
     static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
-    int64_t sp0 = 0, op0 = 0;
-    for (const PairOut &o : outs) {
-        if (o.nsp > 0) std::memcpy(&R->sps[sp0], o.sps, (size_t)o.nsp * sizeof(SpRec));
-        if (o.nops > 0) std::memcpy(&R->ops[op0], o.ops, (size_t)o.nops);
-        for (int k = 0; k < o.nsp; ++k) {
-            const HitRec &hr = o.hits[k];
-            pw_hit &h = R->hits[sp0 + k];
-            h.query = hr.query;
-            h.target = hr.target;
-            h.shown = hr.shown;
-            h.s_start = hr.s_start;
-            h.s_end = hr.s_end;
-            h.t_start = hr.t_start;
-            h.t_end = hr.t_end;
-            h.matches = hr.matches;
-            h.mismatches = hr.mismatches;
-            h.length = hr.length;
-            h.score = hr.score;
-            h.sp_index = (int32_t)(sp0 + hr.sp_index);
-            h.ops_offset = op0 + hr.ops_offset;
+    // records of pairs [b, e) of outs, starting at sp0 / op0 in the result
+    auto fill = [R, &outs](size_t b, size_t e, int64_t sp0, int64_t op0) {
+        for (size_t i = b; i < e; ++i) {
+            const PairOut &o = outs[i];
+            if (o.nsp > 0) std::memcpy(&R->sps[sp0], o.sps, (size_t)o.nsp * sizeof(SpRec));
+            if (o.nops > 0) std::memcpy(&R->ops[op0], o.ops, (size_t)o.nops);
+            for (int k = 0; k < o.nsp; ++k) {
+                const HitRec &hr = o.hits[k];
+                pw_hit &h = R->hits[sp0 + k];
+                h.query = hr.query;
+                h.target = hr.target;
+                h.shown = hr.shown;
+                h.s_start = hr.s_start;
+                h.s_end = hr.s_end;
+                h.t_start = hr.t_start;
+                h.t_end = hr.t_end;
+                h.matches = hr.matches;
+                h.mismatches = hr.mismatches;
+                h.length = hr.length;
+                h.score = hr.score;
+                h.sp_index = (int32_t)(sp0 + hr.sp_index);
+                h.ops_offset = op0 + hr.ops_offset;
+            }
+            sp0 += o.nsp;
+            op0 += o.nops;
         }
-        sp0 += o.nsp;
-        op0 += o.nops;
+    };
+    // large results are filled by a few host threads, each a contiguous range
+    // of pairs at its precomputed offsets (the same records as one thread)
+    const double bytes = (double)nops + (double)nsp * (sizeof(SpRec) + sizeof(pw_hit));
+    const int nth = bytes >= 8e6 ? 4 : (bytes >= 2e6 ? 2 : 1);
+    if (nth == 1) {
+        fill(0, outs.size(), 0, 0);
+    } else {
+        std::vector<std::thread> th;
+        size_t b = 0;
+        int64_t sp0 = 0, op0 = 0;
+        double acc = 0;
+        for (int t = 0; t < nth; ++t) {
+            size_t e = b;
+            int64_t sp1 = sp0, op1 = op0;
+            const double goal = bytes * (t + 1) / nth;
+            while (e < outs.size() && (t == nth - 1 || acc < goal)) {
+                acc += (double)outs[e].nops + (double)outs[e].nsp * (sizeof(SpRec) + sizeof(pw_hit));
+                sp1 += outs[e].nsp;
+                op1 += outs[e].nops;
+                ++e;
+            }
+            if (t == nth - 1) th.emplace_back(fill, b, e, sp0, op0);
+            else if (e > b) th.emplace_back(fill, b, e, sp0, op0);
+            b = e;
+            sp0 = sp1;
+            op0 = op1;
+        }
+        for (std::thread &x : th) x.join();
     }
     // the reference keeps at most sp_cap starting points per pair of a batch in
     // total (SmithWaterman.py:683-685); report how many a cap would drop
