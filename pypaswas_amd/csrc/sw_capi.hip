@@ -123,6 +123,7 @@ struct PlanOpts {
     double group_max_share = 0.25;  // longest lane-per-pair job, as a share of the cells per device lane
     int orient = -1;  // column sequence of lane-per-pair jobs: 0 target, 1 query, -1 the cheaper one
     int dir_rmax = 0;  // > 0: largest residues per lane of the direction pass (2, 4 or 8)
+    double grid_share = 1.0;  // persistent grids cover at most this share of the device's resident waves
 };
 
 // Bucket k holds lengths up to 4/3 of its representative length rep[k]
@@ -1184,8 +1185,9 @@ int blocks_per_cu(pw_ctx *ctx, const void *kernel, size_t lds) {
     return per_cu;
 }
 
-int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work) {
-    const int64_t g = (int64_t)ctx->num_cus * blocks_per_cu(ctx, kernel, lds);
+int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work, double share = 1.0) {
+    const int64_t full = (int64_t)ctx->num_cus * blocks_per_cu(ctx, kernel, lds);
+    const int64_t g = std::max<int64_t>(1, (int64_t)(share * (double)full));
     return (int)std::max<int64_t>(1, std::min<int64_t>(work, g));
 }
 
@@ -1198,12 +1200,14 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
         g.gscr_stride = std::max<int64_t>(1, P->group_rows) * WAVE;
         for (int w = 0; w < NB_KINDS; ++w) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
-            if (n > 0) g.group_grid[w] = persistent_grid(ctx, group_kernel_for(aff, NB_WIDTHS[w]), group_lds(P, w), n);
+            if (n > 0)
+                g.group_grid[w] =
+                    persistent_grid(ctx, group_kernel_for(aff, NB_WIDTHS[w]), group_lds(P, w), n, P->opts.grid_share);
         }
     }
     if (P->n_tile > 0) {
         g.tile_grid = persistent_grid(ctx, (const void *)(aff ? score_kernel<true> : score_kernel<false>),
-                                      lds_bytes(P->E.KP), P->n_tile);
+                                      lds_bytes(P->E.KP), P->n_tile, P->opts.grid_share);
         g.tscr_stride = 3 * (P->max_b + 64);
     }
     const int64_t n = P->n_pairs;
@@ -1214,7 +1218,7 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
     // passing the gate could use (one per stripe of a split pair), up to the
     // resident limit; split stripes need their producer stripes running
     g.dir_grid = persistent_grid(ctx, (const void *)(aff ? dir_kernel<true> : dir_kernel<false>),
-                                 lds_bytes(P->E.KP, true), std::max<int64_t>(1, P->units_worst));
+                                 lds_bytes(P->E.KP, true), std::max<int64_t>(1, P->units_worst), P->opts.grid_share);
     g.dscr_stride = 3 * (P->max_t + 64);
     g.task_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * 4));
     return g;

@@ -25,6 +25,7 @@ namespace {
 
 constexpr int SERVE_SCHEMES = 8;
 constexpr double LARGE_JOB_CELLS = 5e7;
+constexpr double LARGE_LANE_GRID_SHARE = 0.5;
 constexpr int64_t ARENA_ALIGN = 256;
 
 // First-fit range allocator over [base, base + cap), with coalescing on release.
@@ -184,7 +185,11 @@ int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, in
     if (qres.empty()) qres.push_back(0);
     if (tres.empty()) tres.push_back(0);
     pw_plan *P = ln.P;
-    int rc = plan_host(ctx->err, sc, plan_opts(ctx, sc), P, qres.data(), qoff.data(), qlen.data(), (int32_t)qlen.size(),
+    PlanOpts opt = plan_opts(ctx, sc);
+    // the large-job lane's persistent kernels take at most half of the
+    // device's resident waves, so that the small-job lanes always find room
+    if (&ln == &S->lanes[0] && large_lanes(S->lanes.size()) > 0) opt.grid_share = LARGE_LANE_GRID_SHARE;
+    int rc = plan_host(ctx->err, sc, opt, P, qres.data(), qoff.data(), qlen.data(), (int32_t)qlen.size(),
                        tres.data(), toff.data(), tlen.data(), (int32_t)tlen.size(), pq.data(), pt.data(),
                        (int64_t)pq.size(), mss.data(), same.data(), 1);
     if (rc) return rc;
