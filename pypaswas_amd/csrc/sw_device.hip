@@ -6,16 +6,19 @@
 // (pyPaSWAS/Core/SmithWaterman.py:665-841); the layout and schedule are this
 // library's own:
 //
-//  * one 64-lane wavefront per tile; lane k owns R consecutive residues of
-//    the tile sequence A (a stripe of 64*R rows) and sweeps the other
-//    sequence B column by column, lane k running one column behind lane k-1
-//    (anti-diagonal wavefront inside the wave);
+//  * wave-per-pair tiles: one 64-lane wavefront per tile; lane k owns R
+//    consecutive residues of the tile sequence A (a stripe of 64*R rows) and
+//    sweeps the other sequence B column by column, lane k running one column
+//    behind lane k-1 (anti-diagonal wavefront inside the wave);
 //  * the values a lane needs from the row above (H, or T = max(M, I, J) and
 //    the J candidate for affine gaps) arrive from lane k-1 through a DPP
-//    wave_shr:1 move; lane 0 receives the previous stripe's bottom row,
-//    staged 64 columns at a time through a per-wave scratch row;
+//    wave_shr:1 move; lane 0 receives the previous stripe's bottom row and
+//    lane 63 hands its own down, 64 columns at a time through LDS and a
+//    scratch row in device memory;
 //  * the substitution matrix (transposed, KP x KP, FILL row/col -1e10) and a
 //    window of B's residue codes live in LDS;
+//  * lane-per-pair jobs (the scoring pass only): 64 pairs sharing one column
+//    sequence, one pair per lane, see group_tile;
 //  * all score arithmetic is IEEE FP32 adds and max, in the order that yields
 //    exactly the reference's values: max(a+s, b+s) == max(a, b)+s because FP32
 //    rounding is monotone, and negative gap-state values are never clamped to
@@ -31,15 +34,6 @@ __device__ __forceinline__ float shr1(float old_lane0, float v) {
     // DPP wave_shr:1 — lane k receives lane k-1's v; lane 0 keeps old_lane0.
     return __int_as_float(
         __builtin_amdgcn_update_dpp(__float_as_int(old_lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-
-__device__ __forceinline__ float rdlane(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-__device__ __forceinline__ float wrlane(float v, int l, float old) {
-    // v is wave-uniform; lane l takes it, the others keep old
-    return ((int)(threadIdx.x & 63) == l) ? v : old;
 }
 
 __device__ __forceinline__ float fmax3(float a, float b, float c) { return __builtin_fmaxf(__builtin_fmaxf(a, b), c); }
