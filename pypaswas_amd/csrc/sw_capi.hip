@@ -61,6 +61,9 @@ struct Lane {
     hipEvent_t ev[6] = {};
     hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t fork2 = nullptr, join2 = nullptr;  // maxima copy on s2 beside the traceback stages
+    // two more streams for the lane-per-pair kernels of a two-stream lane
+    hipStream_t sx[2] = {};
+    hipEvent_t joinx[2] = {};
 };
 
 // Work-area sizing rules (pw_set_limits).
@@ -195,7 +198,7 @@ struct Encoded {
 };
 
 // chunk widths of the lane-per-pair kernel
-constexpr int NB_WIDTHS[NB_KINDS] = {16, 32, 48, 64, 80, 96};
+constexpr int NB_WIDTHS[NB_KINDS] = {16, 24, 32, 40, 48, 56, 64, 72, 80, 88, 96};
 
 // Launch geometry of one execution, computed before any launch (and before
 // a graph capture, during which no occupancy query is made).
@@ -405,6 +408,10 @@ int create_lane(Lane &L, std::string &err, bool single = false) {
     hipError_t e = hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking);
     if (single) L.s2 = L.s;
     else if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.s2, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && !single; ++i) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.sx[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&L.joinx[i], hipEventDisableTiming);
+    }
     for (auto &ev : L.ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fork, hipEventDisableTiming);
@@ -421,6 +428,13 @@ int create_lane(Lane &L, std::string &err, bool single = false) {
 void destroy_lane(Lane &L) {
     if (L.s) (void)hipStreamSynchronize(L.s);
     if (L.s2 && L.s2 != L.s) (void)hipStreamSynchronize(L.s2);
+    for (int i = 0; i < 2; ++i) {
+        if (L.sx[i]) {
+            (void)hipStreamSynchronize(L.sx[i]);
+            (void)hipStreamDestroy(L.sx[i]);
+        }
+        if (L.joinx[i]) (void)hipEventDestroy(L.joinx[i]);
+    }
     for (auto &ev : L.ev)
         if (ev) (void)hipEventDestroy(ev);
     if (L.fork) (void)hipEventDestroy(L.fork);
@@ -734,7 +748,7 @@ constexpr int GROUP_MAX_ROWS = 1 << 15;
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
 // chunks of 64 (or the tuned width) beyond
 int group_nb(int clen, const Tuning *tu) {
-    if (clen <= 96) return std::max(16, (clen + 15) / 16 * 16);
+    if (clen <= 96) return std::max(16, (clen + 7) / 8 * 8);
     return (tu && tu->valid) ? tu->group_nb_long : 64;
 }
 
@@ -1165,10 +1179,15 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
 const void *group_kernel_for(bool affine, int nb) {
     switch (nb) {
     case 16: return affine ? (const void *)group_kernel<true, 16> : (const void *)group_kernel<false, 16>;
+    case 24: return affine ? (const void *)group_kernel<true, 24> : (const void *)group_kernel<false, 24>;
     case 32: return affine ? (const void *)group_kernel<true, 32> : (const void *)group_kernel<false, 32>;
+    case 40: return affine ? (const void *)group_kernel<true, 40> : (const void *)group_kernel<false, 40>;
     case 48: return affine ? (const void *)group_kernel<true, 48> : (const void *)group_kernel<false, 48>;
+    case 56: return affine ? (const void *)group_kernel<true, 56> : (const void *)group_kernel<false, 56>;
     case 64: return affine ? (const void *)group_kernel<true, 64> : (const void *)group_kernel<false, 64>;
+    case 72: return affine ? (const void *)group_kernel<true, 72> : (const void *)group_kernel<false, 72>;
     case 80: return affine ? (const void *)group_kernel<true, 80> : (const void *)group_kernel<false, 80>;
+    case 88: return affine ? (const void *)group_kernel<true, 88> : (const void *)group_kernel<false, 88>;
     default: return affine ? (const void *)group_kernel<true, 96> : (const void *)group_kernel<false, 96>;
     }
 }
@@ -1291,15 +1310,25 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
     const SeqRef *seqs = (const SeqRef *)P->seqs.p;
     const float *tabs = (const float *)P->tables.p;
     float *mx = (float *)W.maxima.p;
-    // the lane-per-pair kernels of the chunk widths present alternate between
-    // the lane's two streams, so that one kernel's tail overlaps the next
-    // kernel's work (only the multi-chunk width uses the per-block scratch)
+    // the lane-per-pair kernels of the chunk widths present are spread round
+    // robin over the lane's streams (four on a two-stream lane), so that one
+    // kernel's tail overlaps the others' work (only the multi-chunk width
+    // uses the per-block scratch)
+    hipStream_t gstreams[4] = {L.s, L.s2, L.sx[0], L.sx[1]};
+    const int ngs = (L.s2 == L.s) ? 1 : (L.sx[0] && L.sx[1] ? 4 : 2);
+    bool used[4] = {true, false, false, false};
     bool forked = false;
     auto fork = [&]() -> int {
-        if (forked || L.s2 == L.s) return PW_OK;
+        if (forked) return PW_OK;
         HIPCHK(ctx, hipEventRecord(L.fork, L.s));
-        HIPCHK(ctx, hipStreamWaitEvent(L.s2, L.fork, 0));
         forked = true;
+        return PW_OK;
+    };
+    auto use = [&](int i) -> int {
+        if (used[i]) return PW_OK;
+        if (fork()) return PW_EDEVICE;
+        HIPCHK(ctx, hipStreamWaitEvent(gstreams[i], L.fork, 0));
+        used[i] = true;
         return PW_OK;
     };
     if (P->n_groups > 0) {
@@ -1307,11 +1336,9 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
         for (int w = 0; w < NB_KINDS; ++w) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
             if (n <= 0) continue;
-            hipStream_t gs = L.s;
-            if (P->mode == 1 && (nk++ & 1) && L.s2 != L.s) {
-                if (fork()) return PW_EDEVICE;
-                gs = L.s2;
-            }
+            const int si = P->mode == 1 ? (nk++ % ngs) : 0;
+            if (use(si)) return PW_EDEVICE;
+            hipStream_t gs = gstreams[si];
             const void *kern = group_kernel_for(aff, NB_WIDTHS[w]);
             const GroupItem *items = (const GroupItem *)P->d_gitems.p + P->nb_begin[w];
             const GroupEntry *ents = (const GroupEntry *)P->d_gents.p;
@@ -1346,8 +1373,9 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             }
         } else {
             // persistent workers on the second stream, overlapping the lane-per-pair kernels
-            hipStream_t st = P->n_groups > 0 ? L.s2 : L.s;
-            if (st != L.s && fork()) return PW_EDEVICE;
+            const int si = (P->n_groups > 0 && ngs > 1) ? 1 : 0;
+            if (use(si)) return PW_EDEVICE;
+            hipStream_t st = gstreams[si];
             hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
                                tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
                                ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p);
@@ -1356,8 +1384,12 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
         }
     }
     if (forked) {
-        HIPCHK(ctx, hipEventRecord(L.join, L.s2));
-        HIPCHK(ctx, hipStreamWaitEvent(L.s, L.join, 0));
+        hipEvent_t joins[4] = {nullptr, L.join, L.joinx[0], L.joinx[1]};
+        for (int i = 1; i < ngs; ++i)
+            if (used[i]) {
+                HIPCHK(ctx, hipEventRecord(joins[i], gstreams[i]));
+                HIPCHK(ctx, hipStreamWaitEvent(L.s, joins[i], 0));
+            }
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[1], L.s));
     return PW_OK;

@@ -624,8 +624,9 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
 
 // One kernel per chunk width, so that each width gets its own register
 // budget; the host launches one per width present, over that width's jobs.
+// Width 72 is held to 168 VGPRs (3 waves per SIMD instead of 2; no spill).
 template <bool AFF, int NB>
-__global__ void __launch_bounds__(64) group_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
+__global__ void __launch_bounds__(64, NB == 72 ? 3 : 1) group_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
                                                    const GroupItem *__restrict__ items, int nitems,
                                                    const GroupEntry *__restrict__ ents,
                                                    const float *__restrict__ tables, int kp, float G, float E,

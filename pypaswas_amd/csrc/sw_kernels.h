@@ -60,7 +60,7 @@ struct GroupItem {
     int32_t entry_off;  // first GroupEntry of this job
     int32_t count;      // lanes in use (<= 64)
     int32_t rows;       // longest row sequence of the job
-    int32_t nb;         // columns per chunk: 16, 32, 48, 64, 80 or 96
+    int32_t nb;         // columns per chunk: 16 to 96 in steps of 8
     int32_t trans;      // 1 when the column sequence is the query
     int32_t k0, k1;     // chunks [k0, k1) of the column sequence
     int64_t link_in;    // float2 offset of the link column read by chunk k0 (-1: none)
@@ -112,7 +112,7 @@ constexpr int32_t TASK_OPS_OVERFLOW = 3;   // no room in the ops area
 
 // Device control block of one execution: work counters of the scoring pass
 // (one 64-byte line each) and the bump allocators of the traceback stages.
-constexpr int NB_KINDS = 6;
+constexpr int NB_KINDS = 11;
 struct Ctl {
     unsigned queue[16 * (1 + NB_KINDS)];
     int ngated;
