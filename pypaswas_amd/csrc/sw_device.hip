@@ -119,7 +119,9 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     const int nsteps = blen_it + WAVE - 1;
     const char *subB = reinterpret_cast<const char *>(subT);
     float rmax = 0.0f;
-    const float bJ = AFF ? G : 0.0f;  // J candidate entering row 1: max(-1e10 + E, 0 + G)
+    // J candidate entering row 1: max(-1e10 + E, 0 + G); the scoring pass
+    // keeps J clamped at 0 (see the scoring recurrences below), so there 0
+    const float bJ = (AFF && DIRS) ? G : 0.0f;
     const int64_t row_floats = link_row_floats(blen_it);
     if (s_last > nstripes) s_last = nstripes;
     // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
@@ -174,7 +176,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             a4[r] = ((i < A.len) ? (int)res[A.off + i] : fill) * 4;
         }
         // state of this lane's rows at the previous column: T, I and, for
-        // affine gaps, M (the direction pass) or M + G (the scoring pass)
+        // affine gaps, M (the direction pass) or d + G, d = T[i-1][j-1] + s (the scoring pass)
         float Tp[R], Mp[R], Ip[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -259,12 +261,18 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if (AFF) {
-                        const float M = __builtin_fmaxf(dg[r], 0.0f);
-                        const float I = __builtin_fmaxf(Ip[r] + E, Mp[r]);  // Mp holds M + G
-                        const float T = fmax3(M, I, jin);
+                        // With G, E <= 0 a negative gap value never reaches
+                        // T and M + G <= 0 whenever M is clamped, so the
+                        // gap candidates take the unclamped d + G and J is
+                        // clamped at 0 instead: I' = max(I' + E, d + G) <= I
+                        // and J' = max(J' + E, d + G, 0) = max(J, 0) keep the
+                        // positive parts of I and J (by induction), and
+                        // T = max(d, I', J') is the reference's T exactly.
+                        const float MG = dg[r] + G;
+                        const float I = __builtin_fmaxf(Ip[r] + E, Mp[r]);  // Mp holds d + G
+                        const float T = fmax3(dg[r], I, jin);
                         rmax = __builtin_fmaxf(rmax, T);
-                        const float MG = M + G;
-                        jin = __builtin_fmaxf(jin + E, MG);
+                        jin = fmax3(jin + E, MG, 0.0f);
                         Tp[r] = T;
                         Mp[r] = MG;
                         Ip[r] = I;
@@ -488,6 +496,13 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
 //   M = max(T[r-1][c-1] + s, 0)           T = max(M, I, J)
 //   I = max(I[r][c-1] + E, M[r][c-1] + G) J = max(J[r-1][c] + E, M[r-1][c] + G)
 // (linear: H = max(H[r-1][c-1] + s, max(H[r-1][c], H[r][c-1]) + G, 0)).
+// With G, E <= 0 (pw_set_scoring rejects positive gap scores) a negative gap
+// value never reaches T, and M[r][c] + G <= 0 whenever M is clamped.  So the
+// kernel feeds the unclamped d = T[r-1][c-1] + s into the gap candidates and
+// clamps J at 0 instead: I' = max(I' + E, d + G) <= I and
+// J' = max(J' + E, d + G, 0) = max(J, 0) keep the positive parts of I and J
+// (by induction over the row and column), and T = max(d, I', J') is the
+// reference's T exactly, with one instruction fewer per cell.
 // ---------------------------------------------------------------------------
 template <bool AFF, int NB>
 __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs, const GroupItem &it,
@@ -504,7 +519,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
     if (active) Rw = seqs[ents[it.entry_off + lane].row_seq];
     const int nchunks = C.len > NB ? (C.len + NB - 1) / NB : 1;
     const float *tab = gtab + (it.trans ? kp * kp : 0);
-    const float bJ = AFF ? G : 0.0f;
+    const float bI = AFF ? G : 0.0f;
     float rmax = 0.0f;
     const int k_end = min(it.k1, nchunks);
     for (int k = it.k0; k < k_end; ++k) {
@@ -534,11 +549,11 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
             Tp[c] = 0.0f;
-            Jc[c] = bJ;
+            Jc[c] = 0.0f;
         }
         float dl = 0.0f;  // T[r-1][c0-1]
         int a_next = (active && 0 < Rw.len) ? (int)res[Rw.off] : fill;
-        float2 lb_next = make_float2(0.0f, bJ);
+        float2 lb_next = make_float2(0.0f, bI);
         if (left_in && it.rows > 0) {
             if (from_link) avail = wait_rows(prog + it.prog_in, 1u);
             lb_next.x = load_scr(&src[lane].x);
@@ -586,10 +601,9 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                 for (int u = 0; u < 4; ++u) {
                     const int c = b + u;
                     if (AFF) {
-                        const float M = __builtin_fmaxf(dg[u], 0.0f);
-                        const float MG = M + G;
-                        const float T = fmax3(M, ic, Jc[c]);
-                        Jc[c] = __builtin_fmaxf(Jc[c] + E, MG);
+                        const float MG = dg[u] + G;
+                        const float T = fmax3(dg[u], ic, Jc[c]);
+                        Jc[c] = fmax3(Jc[c] + E, MG, 0.0f);
                         ic = __builtin_fmaxf(ic + E, MG);
                         Tp[c] = T;
                     } else {
@@ -598,7 +612,6 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                         ic = h;
                     }
                 }
-                rmax = fmax3(rmax, fmax3(Tp[b], Tp[b + 1], Tp[b + 2]), Tp[b + 3]);
                 // Materialise this block's column state here: without it the
                 // compiler sinks the T and J updates to the end of the row and
                 // keeps ~4 registers per column alive.  The empty asm emits no
@@ -608,6 +621,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                     if (AFF) asm volatile("" : "+v"(Tp[b + u]), "+v"(Jc[b + u]));
                     else asm volatile("" : "+v"(Tp[b + u]));
                 }
+                rmax = fmax3(rmax, fmax3(Tp[b], Tp[b + 1], Tp[b + 2]), Tp[b + 3]);
                 asm volatile("" : "+v"(rmax), "+v"(ic));
             }
             if (right_out) {

@@ -1,0 +1,90 @@
+"""The scoring pass's affine recurrence without the clamp of M.
+
+The reference (default_main_cpu.cl:224-387) computes, per cell,
+    M = max(T[i-1][j-1] + s, 0)            T = max(M, I, J)
+    I = max(I[i][j-1] + E, M[i][j-1] + G)  J = max(J[i-1][j] + E, M[i-1][j] + G)
+The HIP scoring kernels (score_kernel and group_kernel) instead feed the
+unclamped d = T[i-1][j-1] + s into the gap candidates and clamp J at 0:
+    I' = max(I'[i][j-1] + E, d[i][j-1] + G)
+    J' = max(J'[i-1][j] + E, d[i-1][j] + G, 0)
+    T  = max(d, I', J')
+which is exact when G, E <= 0 (pw_set_scoring rejects anything else).  These
+tests run both forms in float32 on random small matrices, the FILL padding
+score (-1e10) included, and require every T to be bit-identical.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+GAP_INIT = np.float32(-1e10)
+FILL = np.float32(-1e10)
+
+
+def reference_T(S, G, E):
+    n, m = S.shape
+    T = np.zeros((n + 1, m + 1), np.float32)
+    M = np.zeros((n + 1, m + 1), np.float32)
+    I = np.full((n + 1, m + 1), GAP_INIT, np.float32)
+    J = np.full((n + 1, m + 1), GAP_INIT, np.float32)
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            M[i, j] = max(np.float32(T[i - 1, j - 1] + S[i - 1, j - 1]), np.float32(0))
+            I[i, j] = max(np.float32(I[i, j - 1] + E), np.float32(M[i, j - 1] + G))
+            J[i, j] = max(np.float32(J[i - 1, j] + E), np.float32(M[i - 1, j] + G))
+            T[i, j] = max(M[i, j], I[i, j], J[i, j])
+    return T[1:, 1:]
+
+
+def clamp_free_T(S, G, E):
+    n, m = S.shape
+    T = np.zeros((n + 1, m + 1), np.float32)
+    D = np.full((n + 1, m + 1), np.float32(0), np.float32)  # d of the boundary: M = 0 there
+    I = np.full((n + 1, m + 1), GAP_INIT, np.float32)
+    J = np.zeros((n + 1, m + 1), np.float32)  # J' of the boundary row: max(G, 0) = 0
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            D[i, j] = np.float32(T[i - 1, j - 1] + S[i - 1, j - 1])
+            I[i, j] = max(np.float32(I[i, j - 1] + E), np.float32(D[i, j - 1] + G))
+            J[i, j] = max(np.float32(J[i - 1, j] + E), np.float32(D[i - 1, j] + G), np.float32(0))
+            T[i, j] = max(D[i, j], I[i, j], J[i, j])
+    return T[1:, 1:]
+
+
+def _scores(rng, n, m, kind):
+    if kind == "dna":
+        a = rng.integers(0, 4, n)
+        b = rng.integers(0, 4, m)
+        S = np.where(a[:, None] == b[None, :], 1.0, -3.0).astype(np.float32)
+    elif kind == "protein":
+        S = rng.integers(-4, 12, (n, m)).astype(np.float32)
+    else:  # fractional scores
+        S = (rng.standard_normal((n, m)) * 2.5).astype(np.float32)
+    # FILL padding rows and columns, as the kernels pad tiles
+    pr, pc = rng.integers(0, 3), rng.integers(0, 3)
+    if pr:
+        S[-pr:, :] = FILL
+    if pc:
+        S[:, -pc:] = FILL
+    return S
+
+
+@pytest.mark.parametrize("kind", ["dna", "protein", "fractional"])
+@pytest.mark.parametrize("gaps", [(-5.0, -1.0), (-11.0, -1.0), (0.0, -1.0), (-1.0, -0.5), (-3.0, -3.0)])
+def test_clamp_free_affine_matches_reference(kind, gaps):
+    G, E = np.float32(gaps[0]), np.float32(gaps[1])
+    rng = np.random.default_rng(zlib.crc32(repr((kind, gaps)).encode()))
+    for _ in range(6):
+        n, m = int(rng.integers(1, 24)), int(rng.integers(1, 24))
+        S = _scores(rng, n, m, kind)
+        ref = reference_T(S, G, E)
+        alt = clamp_free_T(S, G, E)
+        assert np.array_equal(ref.view(np.uint32), alt.view(np.uint32)), (kind, gaps, n, m)
+
+
+def test_clamp_free_needs_nonpositive_gaps():
+    """A positive gap-open score breaks the equivalence (so the library rejects
+    it): the check is not vacuous."""
+    S = np.array([[2.0, -3.0, -3.0], [-3.0, -3.0, -3.0]], np.float32)
+    G, E = np.float32(1.0), np.float32(-1.0)
+    assert not np.array_equal(reference_T(S, G, E), clamp_free_T(S, G, E))
