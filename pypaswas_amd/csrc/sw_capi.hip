@@ -51,6 +51,8 @@ struct Scoring {
     int dim = 0, char_offset = 0;
     float gap = 0, ext = 0, ll = 1, minimum = 0;
     bool affine = false;
+    bool integral = false;  // every substitution and gap score is a whole number
+    float max_abs = 0;      // largest magnitude of those scores
 };
 
 // Two streams and the events of one execution.  ev: 0 score start, 1 score
@@ -537,6 +539,12 @@ int make_scoring(std::string &err, const float *subst, int dim, int char_offset,
             return PW_EINVAL;
         }
     sc.subst.assign(subst, subst + (size_t)dim * dim);
+    sc.integral = std::floor(gap) == gap && std::floor(gap_ext) == gap_ext;
+    sc.max_abs = std::max(std::fabs(gap), std::fabs(gap_ext));
+    for (int i = 0; i < dim * dim; ++i) {
+        sc.integral = sc.integral && std::floor(subst[i]) == subst[i];
+        sc.max_abs = std::max(sc.max_abs, std::fabs(subst[i]));
+    }
     sc.dim = dim;
     sc.char_offset = char_offset;
     sc.gap = gap;
@@ -1176,20 +1184,37 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
     return PW_OK;
 }
 
-const void *group_kernel_for(bool affine, int nb) {
+// Whether the lane-per-pair kernels of P may run the row-shifted affine
+// recurrence: it needs whole-number scores and every shifted sum exact in
+// FP32.  Shifted values stay within (rows + columns + 2) x (largest score
+// magnitude) of zero, so that product is held below 2^22 (FILL padding
+// sums are -1e10-sized and never win a maximum in either form).
+bool group_shift_exact(const pw_plan *P) {
+    if (!P->sc.affine || !P->sc.integral) return false;
+    const double span = (double)(P->E.max_len_q + P->E.max_len_t + 2) * (3.0 * (double)P->sc.max_abs + 1.0);
+    return span < 4194304.0;
+}
+
+template <bool AFF, bool SH>
+const void *group_kernel_width(int nb) {
     switch (nb) {
-    case 16: return affine ? (const void *)group_kernel<true, 16> : (const void *)group_kernel<false, 16>;
-    case 24: return affine ? (const void *)group_kernel<true, 24> : (const void *)group_kernel<false, 24>;
-    case 32: return affine ? (const void *)group_kernel<true, 32> : (const void *)group_kernel<false, 32>;
-    case 40: return affine ? (const void *)group_kernel<true, 40> : (const void *)group_kernel<false, 40>;
-    case 48: return affine ? (const void *)group_kernel<true, 48> : (const void *)group_kernel<false, 48>;
-    case 56: return affine ? (const void *)group_kernel<true, 56> : (const void *)group_kernel<false, 56>;
-    case 64: return affine ? (const void *)group_kernel<true, 64> : (const void *)group_kernel<false, 64>;
-    case 72: return affine ? (const void *)group_kernel<true, 72> : (const void *)group_kernel<false, 72>;
-    case 80: return affine ? (const void *)group_kernel<true, 80> : (const void *)group_kernel<false, 80>;
-    case 88: return affine ? (const void *)group_kernel<true, 88> : (const void *)group_kernel<false, 88>;
-    default: return affine ? (const void *)group_kernel<true, 96> : (const void *)group_kernel<false, 96>;
+    case 16: return (const void *)group_kernel<AFF, 16, SH>;
+    case 24: return (const void *)group_kernel<AFF, 24, SH>;
+    case 32: return (const void *)group_kernel<AFF, 32, SH>;
+    case 40: return (const void *)group_kernel<AFF, 40, SH>;
+    case 48: return (const void *)group_kernel<AFF, 48, SH>;
+    case 56: return (const void *)group_kernel<AFF, 56, SH>;
+    case 64: return (const void *)group_kernel<AFF, 64, SH>;
+    case 72: return (const void *)group_kernel<AFF, 72, SH>;
+    case 80: return (const void *)group_kernel<AFF, 80, SH>;
+    case 88: return (const void *)group_kernel<AFF, 88, SH>;
+    default: return (const void *)group_kernel<AFF, 96, SH>;
     }
+}
+
+const void *group_kernel_for(const pw_plan *P, int nb) {
+    if (!P->sc.affine) return group_kernel_width<false, false>(nb);
+    return group_shift_exact(P) ? group_kernel_width<true, true>(nb) : group_kernel_width<true, false>(nb);
 }
 
 // Workgroups of 64 threads resident per CU (cached per kernel and LDS size).
@@ -1221,7 +1246,7 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
             if (n > 0)
                 g.group_grid[w] =
-                    persistent_grid(ctx, group_kernel_for(aff, NB_WIDTHS[w]), group_lds(P, w), n, P->opts.grid_share);
+                    persistent_grid(ctx, group_kernel_for(P, NB_WIDTHS[w]), group_lds(P, w), n, P->opts.grid_share);
         }
     }
     if (P->n_tile > 0) {
@@ -1339,7 +1364,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             const int si = P->mode == 1 ? (nk++ % ngs) : 0;
             if (use(si)) return PW_EDEVICE;
             hipStream_t gs = gstreams[si];
-            const void *kern = group_kernel_for(aff, NB_WIDTHS[w]);
+            const void *kern = group_kernel_for(P, NB_WIDTHS[w]);
             const GroupItem *items = (const GroupItem *)P->d_gitems.p + P->nb_begin[w];
             const GroupEntry *ents = (const GroupEntry *)P->d_gents.p;
             int kp = P->E.KP, ni = (int)n;

@@ -503,8 +503,21 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
 // J' = max(J' + E, d + G, 0) = max(J, 0) keep the positive parts of I and J
 // (by induction over the row and column), and T = max(d, I', J') is the
 // reference's T exactly, with one instruction fewer per cell.
+// Row-shifted form (SH, integral scores only, see group_shift_exact on the
+// host): with sigma_r = -r * E >= 0 the kernel carries T^ = T + sigma_r,
+// J^ = J' + sigma_r and I^ = I' + sigma_r - E, and its profile holds s - E.
+// Then, with d^ = T^[r-1][c-1] + (s - E) = d + sigma_r and
+// m = d^ + (G - E),
+//   P = I^ + E,  T^ = max(d^, P, J^),  I^ <- max(P, m),
+//   J^ <- max(J^, m, sigma_{r+1})
+// which is the recurrence above shifted by sigma_r, one add fewer per cell
+// (E leaves the J update and the I add is shared with T).  Integral scores
+// keep every sum exact, so the row maximum of T^ minus sigma_r is the row
+// maximum of T bit for bit.  The boundary row above holds T^ = E (T = 0,
+// sigma_-1 = E); the left boundary of row r holds T^ = sigma_r and
+// I^ = G - E + sigma_r.
 // ---------------------------------------------------------------------------
-template <bool AFF, int NB>
+template <bool AFF, int NB, bool SH>
 __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs, const GroupItem &it,
                             const GroupEntry *__restrict__ ents, const float *__restrict__ gtab, int kp,
                             float *prof, float2 *__restrict__ scr, float2 *__restrict__ links,
@@ -519,7 +532,13 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
     if (active) Rw = seqs[ents[it.entry_off + lane].row_seq];
     const int nchunks = C.len > NB ? (C.len + NB - 1) / NB : 1;
     const float *tab = gtab + (it.trans ? kp * kp : 0);
+    static_assert(AFF || !SH, "the row shift applies to affine gaps");
     const float bI = AFF ? G : 0.0f;
+    // wave-uniform values kept in scalar registers (readfirstlane), so that
+    // the shifted form costs no vector registers
+    auto uniform = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+    const float GmE = uniform(G - E);
+    const float t0 = SH ? E : 0.0f;  // T (T^) of the boundary row above the first row
     float rmax = 0.0f;
     const int k_end = min(it.k1, nchunks);
     for (int k = it.k0; k < k_end; ++k) {
@@ -541,19 +560,19 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
             const int a = idx / (NB + 1), p = idx - a * (NB + 1);
             const int col = c0 + (p < NB ? p + 1 : 0);
             const int b = col < C.len ? (int)res[C.off + col] : fill;
-            prof[a * NBP + p] = tab[b * kp + a];
+            prof[a * NBP + p] = SH ? tab[b * kp + a] - E : tab[b * kp + a];
         }
         __syncthreads();
         const bool left_in = k > 0, right_out = k + 1 < nchunks;
         float Tp[NB], Jc[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
-            Tp[c] = 0.0f;
+            Tp[c] = t0;
             Jc[c] = 0.0f;
         }
-        float dl = 0.0f;  // T[r-1][c0-1]
+        float dl = t0;  // T[r-1][c0-1]
         int a_next = (active && 0 < Rw.len) ? (int)res[Rw.off] : fill;
-        float2 lb_next = make_float2(0.0f, bI);
+        float2 lb_next = make_float2(0.0f, SH ? GmE : bI);
         if (left_in && it.rows > 0) {
             if (from_link) avail = wait_rows(prog + it.prog_in, 1u);
             lb_next.x = load_scr(&src[lane].x);
@@ -563,6 +582,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
         const float *prow = prof + a_next * NBP;
         float4 s4n = *reinterpret_cast<const float4 *>(prow);
         float s0n = prow[NB];
+        float sig1 = uniform(-E);  // sigma_{r+1} (SH)
 #pragma unroll 1
         for (int r = 0; r < it.rows; ++r) {
             const float2 lb = lb_next;
@@ -571,12 +591,16 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                 if ((unsigned)(r + 2) > avail) avail = wait_rows(prog + it.prog_in, (unsigned)(r + 2));
                 lb_next.x = load_scr(&src[(r + 1) * WAVE + lane].x);
                 lb_next.y = load_scr(&src[(r + 1) * WAVE + lane].y);
+            } else if (SH) {
+                lb_next = make_float2(sig1, GmE + sig1);  // left boundary of row r+1
             }
             float ic = lb.y;  // affine: I candidate entering the column; linear: H[r][c-1]
             // diagonal sum of the block's first column, T[r-1][c-1] + s; each
             // block forms the next block's from the profile values it already
             // holds, before its last column overwrites T[r-1][c-1]
             float dgc = dl + s0n;
+            // SH: the running maximum in the row's shifted terms, rmax + sigma_r
+            if (SH) rmax = rmax + (sig1 + E);
 #pragma unroll
             for (int b = 0; b < NB; b += 4) {
                 // the next block's profile values (the next row's first block
@@ -600,7 +624,14 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int c = b + u;
-                    if (AFF) {
+                    if (AFF && SH) {
+                        const float m = dg[u] + GmE;
+                        const float P = ic + E;
+                        const float T = fmax3(dg[u], P, Jc[c]);
+                        Jc[c] = fmax3(Jc[c], m, sig1);
+                        ic = __builtin_fmaxf(P, m);
+                        Tp[c] = T;
+                    } else if (AFF) {
                         const float MG = dg[u] + G;
                         const float T = fmax3(dg[u], ic, Jc[c]);
                         Jc[c] = fmax3(Jc[c] + E, MG, 0.0f);
@@ -624,6 +655,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                 rmax = fmax3(rmax, fmax3(Tp[b], Tp[b + 1], Tp[b + 2]), Tp[b + 3]);
                 asm volatile("" : "+v"(rmax), "+v"(ic));
             }
+            if (SH) rmax = rmax - (sig1 + E);
             if (right_out) {
                 store_scr(&dst[r * WAVE + lane].x, Tp[NB - 1]);
                 store_scr(&dst[r * WAVE + lane].y, ic);
@@ -631,6 +663,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
                     publish_progress(prog + it.prog_out, (unsigned)(r + 1));
             }
             dl = lb.x;
+            if (SH) sig1 = uniform(sig1 - E);
         }
     }
     return rmax;
@@ -639,7 +672,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
 // One kernel per chunk width, so that each width gets its own register
 // budget; the host launches one per width present, over that width's jobs.
 // Width 72 is held to 168 VGPRs (3 waves per SIMD instead of 2; no spill).
-template <bool AFF, int NB>
+template <bool AFF, int NB, bool SH>
 __global__ void __launch_bounds__(64, NB == 72 ? 3 : 1) group_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
                                                    const GroupItem *__restrict__ items, int nitems,
                                                    const GroupEntry *__restrict__ ents,
@@ -663,7 +696,7 @@ __global__ void __launch_bounds__(64, NB == 72 ? 3 : 1) group_kernel(const uint8
         }
         if (item >= nitems) break;
         const GroupItem it = items[item];
-        const float m = group_tile<AFF, NB>(res, seqs, it, ents, tables, kp, lds, scr, links, prog, G, E, fill);
+        const float m = group_tile<AFF, NB, SH>(res, seqs, it, ents, tables, kp, lds, scr, links, prog, G, E, fill);
         if ((int)threadIdx.x < it.count) {
             const int pair = ents[it.entry_off + threadIdx.x].pair;
             // a split job's maximum: the maximum over its items, combined with

@@ -51,6 +51,68 @@ def clamp_free_T(S, G, E):
     return T[1:, 1:]
 
 
+def shifted_max(S, G, E, nb):
+    """The row-shifted form of the lane-per-pair kernel, chunk by chunk (nb
+    columns each, the right edge T^ and I^ handed to the next chunk) with the
+    running maximum kept as rmax + sigma_r inside a row.  Returns the matrix
+    maximum and T^ - sigma_r for every cell."""
+    n, m = S.shape
+    GmE = np.float32(G - E)
+    edge = None  # per row: (T^ of the chunk's last column, I^ entering the next chunk)
+    rmax = np.float32(0)
+    out = np.zeros((n, m), np.float32)
+    for c0 in range(0, m, nb):
+        w = min(nb, m - c0)
+        Tp = np.full(w, E, np.float32)   # T^ of the row above row 0: T = 0, sigma_-1 = E
+        Jc = np.zeros(w, np.float32)
+        dl = np.float32(E)
+        new_edge = []
+        sig1 = np.float32(-E)
+        for r in range(n):
+            sig = np.float32(sig1 + E)
+            if edge is None:
+                lb = (sig, np.float32(GmE + sig))
+            else:
+                lb = edge[r]
+            ic = lb[1]
+            rmax = np.float32(rmax + sig)
+            Tn = np.empty(w, np.float32)
+            for c in range(w):
+                d = np.float32((dl if c == 0 else Tp[c - 1]) + np.float32(S[r, c0 + c] - E))
+                mm = np.float32(d + GmE)
+                P = np.float32(ic + E)
+                T = max(d, P, Jc[c])
+                Jc[c] = max(Jc[c], mm, sig1)
+                ic = max(P, mm)
+                Tn[c] = T
+                rmax = max(rmax, T)
+                out[r, c0 + c] = np.float32(T - sig)
+            dl = lb[0]
+            Tp = Tn
+            rmax = np.float32(rmax - sig)
+            new_edge.append((Tn[-1], ic))
+            sig1 = np.float32(sig1 - E)
+        edge = new_edge
+    return rmax, out
+
+
+@pytest.mark.parametrize("kind", ["dna", "protein"])
+@pytest.mark.parametrize("gaps", [(-5.0, -1.0), (-11.0, -1.0), (0.0, -2.0), (-3.0, -3.0)])
+def test_row_shifted_affine_matches_reference(kind, gaps):
+    """Integral scores: the shifted form gives every T and the maximum bit for
+    bit, across chunk edges."""
+    G, E = np.float32(gaps[0]), np.float32(gaps[1])
+    rng = np.random.default_rng(zlib.crc32(repr(("shift", kind, gaps)).encode()))
+    for _ in range(5):
+        n, m = int(rng.integers(1, 20)), int(rng.integers(1, 30))
+        S = _scores(rng, n, m, kind)
+        ref = reference_T(S, G, E)
+        for nb in (4, 8, 64):
+            mx, T = shifted_max(S, G, E, nb)
+            assert np.array_equal(ref.view(np.uint32), T.view(np.uint32)), (kind, gaps, n, m, nb)
+            assert mx == ref.max(), (kind, gaps, n, m, nb)
+
+
 def _scores(rng, n, m, kind):
     if kind == "dna":
         a = rng.integers(0, 4, n)
