@@ -1356,6 +1356,10 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
         used[i] = true;
         return PW_OK;
     };
+    // the fork point is the end of the work issued so far (the clears), taken
+    // before the first score kernel goes onto the main stream: the other
+    // streams must not wait for that kernel
+    if (ngs > 1 && fork()) return PW_EDEVICE;
     if (P->n_groups > 0) {
         int nk = 0;
         for (int w = 0; w < NB_KINDS; ++w) {
