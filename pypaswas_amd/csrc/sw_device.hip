@@ -119,9 +119,9 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     const int nsteps = blen_it + WAVE - 1;
     const char *subB = reinterpret_cast<const char *>(subT);
     float rmax = 0.0f;
-    // J candidate entering row 1: max(-1e10 + E, 0 + G); the scoring pass
-    // keeps J clamped at 0 (see the scoring recurrences below), so there 0
-    const float bJ = (AFF && DIRS) ? G : 0.0f;
+    // J candidate entering row 1: max(-1e10 + E, 0 + G), which both passes
+    // keep clamped at 0 (see the recurrences below)
+    const float bJ = 0.0f;
     const int64_t row_floats = link_row_floats(blen_it);
     if (s_last > nstripes) s_last = nstripes;
     // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
@@ -176,13 +176,15 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             a4[r] = ((i < A.len) ? (int)res[A.off + i] : fill) * 4;
         }
         // state of this lane's rows at the previous column: T, I and, for
-        // affine gaps, M (the direction pass) or d + G, d = T[i-1][j-1] + s (the scoring pass)
-        float Tp[R], Mp[R], Ip[R];
+        // affine gaps, d + G with d = T[i-1][j-1] + s (and, for the direction
+        // pass, d itself)
+        float Tp[R], Mp[R], Ip[R], Dp[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             Tp[r] = 0.0f;
-            Mp[r] = DIRS ? 0.0f : 0.0f + G;  // M + G of the virtual column left of the matrix
+            Mp[r] = 0.0f + G;  // M + G of the virtual column left of the matrix
             Ip[r] = GAP_INIT;
+            Dp[r] = 0.0f;      // M = 0 there
         }
         const bool above = s > 0;
         const bool below = s + 1 < nstripes;
@@ -288,30 +290,38 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if (AFF) {
-                        const float M = __builtin_fmaxf(td + sc[r], 0.0f);
-                        const float I = __builtin_fmaxf(Ip[r] + E, Mp[r] + G);
-                        const float T = fmax3(M, I, jin);
+                        // the scoring pass's clamp-free form: d unclamped in
+                        // the gap candidates, J clamped at 0, T exact; with
+                        // T > 0, T == M exactly when T == d, T == I exactly
+                        // when T == I', and M = 0 exactly when d <= 0, so the
+                        // direction byte and the candidate test are the
+                        // reference's
+                        const float d = td + sc[r];
+                        const float dG = d + G;
+                        const float I = __builtin_fmaxf(Ip[r] + E, Mp[r]);  // Mp holds d + G
+                        const float T = fmax3(d, I, jin);
                         rmax = __builtin_fmaxf(rmax, T);
                         // default_main_cpu.cl:283-345: source = M, I, J in that
                         // priority among equal positive values, none when 0; a
                         // main-matrix cell stops on a zero diagonal predecessor
-                        // and is a candidate start when M (or -M, for a marked
-                        // cell) reaches both thresholds (the two candidate
-                        // flags go with the candidate record, not the byte).
-                        // Every value is formed, then selected: no divergent
-                        // branch per cell.
-                        uint32_t main = DB_MAIN | (md == 0.0f ? DB_STOP : 0u);
+                        // (M = 0 there, i.e. d <= 0) and is a candidate start
+                        // when M (or -M, for a marked cell) reaches both
+                        // thresholds (the two candidate flags go with the
+                        // candidate record, not the byte).  Every value is
+                        // formed, then selected: no divergent branch per cell.
+                        uint32_t main = DB_MAIN | (md <= 0.0f ? DB_STOP : 0u);
                         uint32_t other = T == I ? 2u : 3u;
                         asm volatile("" : "+v"(main), "+v"(other));
-                        const bool isM = T == M, pos = T > 0.0f;
+                        const bool isM = T == d, pos = T > 0.0f;
                         bits[r] = pos ? (isM ? main : other) : 0u;
                         // M >= 0, so "M >= thr or M <= -thr" is M >= thr
-                        cand[r] = pos && isM && M >= thr;
-                        const float jout = __builtin_fmaxf(jin + E, M + G);
+                        cand[r] = pos && isM && d >= thr;
+                        const float jout = fmax3(jin + E, dG, 0.0f);
                         td = Tp[r];
-                        md = Mp[r];
+                        md = Dp[r];
                         Tp[r] = T;
-                        Mp[r] = M;
+                        Dp[r] = d;
+                        Mp[r] = dG;
                         Ip[r] = I;
                         jin = jout;
                     } else {
@@ -384,7 +394,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             // bottom row of this lane for column j
             const float botT = Tp[R - 1];
             const float botJ = AFF ? jin : 0.0f;
-            const float botM = DIRS ? Mp[R - 1] : 0.0f;
+            const float botM = DIRS ? Dp[R - 1] : 0.0f;  // d of the bottom row: the next stripe's STOP test
             if (below) {
                 const int c63 = t - (WAVE - 1);
                 if (c63 >= 0 && c63 < blen_it) {

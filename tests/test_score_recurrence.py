@@ -144,6 +144,58 @@ def test_clamp_free_affine_matches_reference(kind, gaps):
         assert np.array_equal(ref.view(np.uint32), alt.view(np.uint32)), (kind, gaps, n, m)
 
 
+def _direction_codes(S, G, E, thr, clamp_free):
+    """Direction byte (main 1 | STOP 4, I 2, J 3, 0 when T <= 0) and candidate
+    flag of every cell, as the direction pass forms them
+    (default_main_cpu.cl:283-345): the reference form from M, or the
+    clamp-free form from d, J clamped at 0."""
+    n, m = S.shape
+    T = np.zeros((n + 1, m + 1), np.float32)
+    M = np.zeros((n + 1, m + 1), np.float32)   # M (reference) or d (clamp-free); 0 on the boundary
+    I = np.full((n + 1, m + 1), GAP_INIT, np.float32)
+    J = np.full((n + 1, m + 1), np.float32(0) if clamp_free else GAP_INIT, np.float32)
+    code = np.zeros((n, m), np.uint8)
+    cand = np.zeros((n, m), bool)
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            d = np.float32(T[i - 1, j - 1] + S[i - 1, j - 1])
+            mm = d if clamp_free else max(d, np.float32(0))
+            left = M[i, j - 1] if (clamp_free or j > 1) else np.float32(0)
+            I[i, j] = max(np.float32(I[i, j - 1] + E), np.float32(left + G))
+            up = M[i - 1, j]
+            if clamp_free:
+                J[i, j] = max(np.float32(J[i - 1, j] + E), np.float32(up + G), np.float32(0))
+            else:
+                J[i, j] = max(np.float32(J[i - 1, j] + E), np.float32(up + G))
+            t = max(mm, I[i, j], J[i, j])
+            T[i, j] = t
+            M[i, j] = mm
+            md = M[i - 1, j - 1]
+            stop = (md <= 0) if clamp_free else (md == 0)
+            main = 1 | (4 if stop else 0)
+            other = 2 if t == I[i, j] else 3
+            pos = t > 0
+            code[i - 1, j - 1] = (main if t == mm else other) if pos else 0
+            cand[i - 1, j - 1] = pos and t == mm and mm >= thr
+    return code, cand
+
+
+@pytest.mark.parametrize("kind", ["dna", "protein", "fractional"])
+@pytest.mark.parametrize("gaps", [(-5.0, -1.0), (-11.0, -1.0), (0.0, -1.0), (-1.0, -0.5)])
+def test_clamp_free_direction_codes_match_reference(kind, gaps):
+    """The direction pass's clamp-free form writes the reference's direction
+    bytes and candidate flags."""
+    G, E = np.float32(gaps[0]), np.float32(gaps[1])
+    rng = np.random.default_rng(zlib.crc32(repr(("dirs", kind, gaps)).encode()))
+    for _ in range(5):
+        n, m = int(rng.integers(1, 20)), int(rng.integers(1, 20))
+        S = _scores(rng, n, m, kind)
+        thr = np.float32(rng.integers(1, 6))
+        ref = _direction_codes(S, G, E, thr, False)
+        alt = _direction_codes(S, G, E, thr, True)
+        assert np.array_equal(ref[0], alt[0]) and np.array_equal(ref[1], alt[1]), (kind, gaps, n, m)
+
+
 def test_clamp_free_needs_nonpositive_gaps():
     """A positive gap-open score breaks the equivalence (so the library rejects
     it): the check is not vacuous."""
