@@ -213,7 +213,8 @@ struct Geometry {
     int build_grid = 0;  // workgroups of 256 threads of the task builder
     int dir_grid = 0;
     int64_t dscr_stride = 0;
-    int task_grid = 0;  // workgroups of 64 threads of the start-selection and walk kernels
+    int task_grid = 0;  // workgroups of 64 threads of the start-selection and walk kernels (a pair per thread)
+    int wave_grid = 0;  // one-wave workgroups of their long-path forms (a pair per wave); 0: no long pairs
 };
 
 }  // namespace
@@ -1265,6 +1266,9 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
                                  lds_bytes(P->E.KP, true), std::max<int64_t>(1, P->units_worst), P->opts.grid_share);
     g.dscr_stride = 3 * (P->max_t + 64);
     g.task_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * 4));
+    g.wave_grid = long_path_lengths(P->E.max_len_q, P->E.max_len_t)
+                      ? (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)ctx->num_cus * 8))
+                      : 0;
     return g;
 }
 
@@ -1424,6 +1428,47 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
     return PW_OK;
 }
 
+// Start selection over the device-side task list: a thread per pair of short
+// paths, then (when the plan has pairs whose paths may be long) a wave per
+// long pair.
+int launch_start_selection(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, Ctl *ctl) {
+    const bool aff = P->sc.affine;
+    const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+    PairTask *tasks = (PairTask *)W.tasks.p;
+    auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
+    hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)ctl, dir_ptr(W),
+                       (Cand *)W.cands.p, (SpRec *)W.spslot.p);
+    HIPCHK(ctx, hipGetLastError());
+    if (g.wave_grid > 0) {
+        auto tw = aff ? tb_wave_kernel<true> : tb_wave_kernel<false>;
+        hipLaunchKernelGGL(tw, dim3(g.wave_grid), dim3(64), WALK_LDS, L.s, seqs, tasks, (const Ctl *)ctl,
+                           dir_ptr(W), (Cand *)W.cands.p, (SpRec *)W.spslot.p);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    return PW_OK;
+}
+
+// The walk over the device-side task list, split the same way.
+int launch_walk(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, Ctl *ctl, double minimum) {
+    const bool aff = P->sc.affine;
+    const uint8_t *res = (const uint8_t *)P->res.p;
+    const SeqRef *seqs = (const SeqRef *)P->seqs.p;
+    PairTask *tasks = (PairTask *)W.tasks.p;
+    auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
+    hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, dir_ptr(W),
+                       (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
+                       (char *)W.ops.p, W.ops_cap, minimum);
+    HIPCHK(ctx, hipGetLastError());
+    if (g.wave_grid > 0) {
+        auto ww = aff ? walk_wave_kernel<true> : walk_wave_kernel<false>;
+        hipLaunchKernelGGL(ww, dim3(g.wave_grid), dim3(64), WALK_LDS, L.s, res, seqs, tasks, ctl, dir_ptr(W),
+                           (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
+                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    return PW_OK;
+}
+
 // Direction pass, start selection and walk over the device-side task list.
 int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum,
                      bool events) {
@@ -1443,16 +1488,11 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
                        (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
     HIPCHK(ctx, hipGetLastError());
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
-    auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
-    hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)ctl, dir_ptr(W),
-                       (Cand *)W.cands.p, (SpRec *)W.spslot.p);
-    HIPCHK(ctx, hipGetLastError());
+    int rc = launch_start_selection(ctx, P, W, L, g, ctl);
+    if (rc) return rc;
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
-    auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
-    hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, dir_ptr(W),
-                       (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
-                       (char *)W.ops.p, W.ops_cap, minimum);
-    HIPCHK(ctx, hipGetLastError());
+    rc = launch_walk(ctx, P, W, L, g, ctl, minimum);
+    if (rc) return rc;
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
     return PW_OK;
 }
@@ -1689,10 +1729,8 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
                            (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
-        auto tk = aff ? tb_kernel<true> : tb_kernel<false>;
-        hipLaunchKernelGGL(tk, dim3(g.task_grid), dim3(64), 0, L.s, seqs, tasks, (const Ctl *)dctl,
-                           dir_ptr(W), (Cand *)W.cands.p, (SpRec *)W.spslot.p);
-        HIPCHK(ctx, hipGetLastError());
+        int rc0 = launch_start_selection(ctx, P, W, L, g, dctl);
+        if (rc0) return rc0;
         HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
         // exact ops capacity from the starting-point counts
         std::vector<PairTask> after((size_t)nc);
@@ -1712,11 +1750,8 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
             W.ops_cap = (int64_t)W.ops.cap;
         }
         HIPCHK(ctx, hipEventRecord(L.ev[4], L.s));
-        auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
-        hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, dctl, dir_ptr(W),
-                           (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
-                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum);
-        HIPCHK(ctx, hipGetLastError());
+        rc0 = launch_walk(ctx, P, W, L, g, dctl, minimum);
+        if (rc0) return rc0;
         HIPCHK(ctx, hipEventRecord(L.ev[5], L.s));
         HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, W.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
         HIPCHK(ctx, hipStreamSynchronize(L.s));
@@ -1888,6 +1923,7 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     add((uint64_t)g.tile_grid);
     add((uint64_t)g.dir_grid);
     add((uint64_t)g.task_grid);
+    add((uint64_t)g.wave_grid);
     (void)P;
     return s;
 }

@@ -951,7 +951,8 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
 }
 
 // ---------------------------------------------------------------------------
-// Start selection, marking and the walk: one thread per pair.
+// Start selection, marking and the walk: one thread per pair, or one wave per
+// pair for pairs whose paths may be long.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int64_t cell_offset(const PairTask &it, int tlen, int x, int y) {
@@ -971,28 +972,134 @@ __device__ __forceinline__ uint64_t order_key(uint32_t x, uint32_t y) {
 }
 
 template <class T, class Less>
+__device__ __forceinline__ void heap_sift(T *a, int root, int end, Less less) {
+    while (2 * root + 1 < end) {
+        int child = 2 * root + 1;
+        if (child + 1 < end && less(a[child], a[child + 1])) ++child;
+        const T vr = a[root], vc = a[child];
+        if (!less(vr, vc)) return;
+        a[root] = vc;
+        a[child] = vr;
+        root = child;
+    }
+}
+
+template <class T, class Less>
 __device__ void heap_sort(T *a, int n, Less less) {
-    auto sift = [&](int root, int end) {
-        while (2 * root + 1 < end) {
-            int child = 2 * root + 1;
-            if (child + 1 < end && less(a[child], a[child + 1])) ++child;
-            if (less(a[root], a[child])) {
-                T tmp = a[root];
-                a[root] = a[child];
-                a[child] = tmp;
-                root = child;
-            } else {
-                return;
+    for (int i = n / 2 - 1; i >= 0; --i) heap_sift(a, i, n, less);
+    for (int end = n - 1; end > 0; --end) {
+        const T v0 = a[0], ve = a[end];
+        a[0] = ve;
+        a[end] = v0;
+        heap_sift(a, 0, end, less);
+    }
+}
+
+// LDS window over one pair's direction area for the serial path walks of the
+// start selection and the walk: stripe s, steps [t0, t0 + nt) of the
+// tile-major layout (nt x 64R contiguous bytes), loaded by the whole wave with
+// 16-byte loads when a cell outside it is needed.  A path moves to smaller
+// steps (1 or 2 per cell), so a window ends at the step that missed.  With
+// RES the window also holds the query residues of the stripe's rows and the
+// target residues of the columns its steps cover (the walk counts matches).
+// One wave runs one pair: every lane follows the same path (wave-uniform
+// values, read from LDS by broadcast), lane 0 alone stores to memory.
+constexpr int WALK_WIN = 16384;
+
+// Pairs whose paths may be long (|q| + |t| >= LONG_PATH_MIN) take the
+// wave-per-pair kernels below; the others one thread each (tb_kernel,
+// walk_kernel), where a short path costs less than loading windows.  The two
+// kernels run one after the other, so the threshold keeps mid-length pairs
+// (proteins) out of the wave form: its time adds to the thread form's.
+constexpr int LONG_PATH_MIN = 8192;
+__device__ __host__ __forceinline__ bool long_path_lengths(int64_t qlen, int64_t tlen) {
+    return qlen + tlen >= LONG_PATH_MIN;
+}
+__device__ __forceinline__ bool long_path_pair(const SeqRef *__restrict__ seqs, const PairTask &it) {
+    return long_path_lengths(seqs[it.q_seq].len, seqs[it.t_seq].len);
+}
+constexpr int WALK_QWIN = WAVE * 8;                      // 64R rows, R <= 8
+constexpr int WALK_TWIN = WALK_WIN / WAVE + WAVE;        // steps of an R = 1 window plus the skew
+constexpr int WALK_LDS = WALK_WIN + WALK_QWIN + WALK_TWIN;
+
+struct PathWindow {
+    uint8_t *blk, *qw, *tw;  // LDS
+    int64_t dir_off, nsteps;
+    int R, s, t0, nt;
+    SeqRef q, t;
+};
+
+__device__ __forceinline__ void window_init(PathWindow &w, uint8_t *lds, const PairTask &it, SeqRef q, SeqRef t) {
+    w.blk = lds;
+    w.qw = lds + WALK_WIN;
+    w.tw = lds + WALK_WIN + WALK_QWIN;
+    w.dir_off = it.dir_off;
+    w.nsteps = (int64_t)t.len + WAVE - 1;
+    w.R = it.R;
+    w.s = -1;
+    w.t0 = 0;
+    w.nt = 0;
+    w.q = q;
+    w.t = t;
+}
+
+// LDS index of cell (x, y) (query row x, target column y), loading the window
+// around it first when it lies outside.
+template <bool RES>
+__device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restrict__ dirs,
+                                           const uint8_t *__restrict__ res, int x, int y) {
+    const int R = w.R, rows = WAVE * R;
+    const int s = x / rows, rem = x - s * rows, k = rem / R, r = rem - k * R;
+    const int t = y + k;
+    if (s != w.s || t < w.t0 || t >= w.t0 + w.nt) {
+        const int lane = threadIdx.x & 63;
+        const int B = WALK_WIN / rows;
+        const int t0 = max(0, t - B + 1);
+        const int nt = (int)min((int64_t)B, w.nsteps - t0);
+        // lane 0's earlier stores (marks) reach memory before the wave reads
+        // it back, and every lane's reads of the old window are done
+        __syncthreads();
+        const uint8_t *src = dirs + w.dir_off + ((int64_t)s * w.nsteps + t0) * rows;
+        const int bytes = nt * rows;
+        if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+            const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+            uint4 *d4 = reinterpret_cast<uint4 *>(w.blk);
+            for (int i = lane; i < bytes / 16; i += WAVE) d4[i] = s4[i];
+        } else {
+            for (int i = lane; i < bytes; i += WAVE) w.blk[i] = src[i];
+        }
+        if (RES) {
+            for (int i = lane; i < rows; i += WAVE) {
+                const int xx = s * rows + i;
+                w.qw[i] = xx < w.q.len ? res[w.q.off + xx] : (uint8_t)0;
+            }
+            for (int i = lane; i < nt + WAVE - 1; i += WAVE) {
+                const int yy = t0 - (WAVE - 1) + i;
+                w.tw[i] = (yy >= 0 && yy < w.t.len) ? res[w.t.off + yy] : (uint8_t)0;
             }
         }
-    };
-    for (int i = n / 2 - 1; i >= 0; --i) sift(i, n);
-    for (int end = n - 1; end > 0; --end) {
-        T tmp = a[0];
-        a[0] = a[end];
-        a[end] = tmp;
-        sift(0, end);
+        __syncthreads();
+        w.s = s;
+        w.t0 = t0;
+        w.nt = nt;
     }
+    return ((t - w.t0) * WAVE + k) * R + r;
+}
+
+// The direction byte of cell (x, y), wave-uniform.
+template <bool RES>
+__device__ __forceinline__ uint32_t window_read(PathWindow &w, const uint8_t *dirs, const uint8_t *res, int x, int y,
+                                                int &idx) {
+    idx = window_cell<RES>(w, dirs, res, x, y);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx]);
+}
+
+// Stores byte b for the cell at LDS index idx (its global offset derived from
+// the window's stripe and steps): LDS by every lane, memory by lane 0.
+__device__ __forceinline__ void window_write(PathWindow &w, uint8_t *__restrict__ dirs, int idx, uint32_t b) {
+    w.blk[idx] = (uint8_t)b;
+    if ((threadIdx.x & 63) == 0)
+        dirs[w.dir_off + ((int64_t)w.s * w.nsteps + w.t0) * (WAVE * w.R) + idx] = (uint8_t)b;
 }
 
 // default_main_cpu.cl:389-604 restated per pair: candidates in canonical
@@ -1000,7 +1107,8 @@ __device__ void heap_sort(T *a, int n, Less less) {
 // path is marked (linear: along the direction codes until a predecessor of
 // value 0; affine: along main-matrix cells only — I/J marks never arise
 // because they would need an existing I/J mark); a marked candidate becomes
-// a starting point only if its sign-marked value passes the tests.
+// a starting point only if its sign-marked value passes the tests.  One
+// thread per pair of short paths (long_path_pair: tb_wave_kernel).
 template <bool AFF>
 __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs, PairTask *__restrict__ tasks,
                                                 const Ctl *__restrict__ ctl, uint8_t *__restrict__ dirs,
@@ -1008,7 +1116,7 @@ __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs,
     const int n = task_count(ctl);
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
         const PairTask it = tasks[g];
-        if (it.status != TASK_OK) continue;
+        if (it.status != TASK_OK || long_path_pair(seqs, it)) continue;
         if (it.ncand > it.cand_cap) {
             tasks[g].status = TASK_CAND_OVERFLOW;
             continue;
@@ -1073,6 +1181,91 @@ __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs,
     }
 }
 
+// The same per pair of long paths, one wave per pair (persistent over the
+// task list), following the marking paths through an LDS window.
+template <bool AFF>
+__global__ void __launch_bounds__(64) tb_wave_kernel(const SeqRef *__restrict__ seqs, PairTask *__restrict__ tasks,
+                                                     const Ctl *__restrict__ ctl, uint8_t *__restrict__ dirs,
+                                                     Cand *__restrict__ cands, SpRec *__restrict__ sps) {
+    extern __shared__ uint8_t wlds[];
+    const int n = task_count(ctl);
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (int g = blockIdx.x; g < n; g += gridDim.x) {
+        const PairTask it = tasks[g];
+        if (it.status != TASK_OK || !long_path_pair(seqs, it)) continue;
+        if (it.ncand > it.cand_cap) {
+            if (lead) tasks[g].status = TASK_CAND_OVERFLOW;
+            continue;
+        }
+        Cand *c = cands + it.cand_off;
+        const int nc = it.ncand;
+        constexpr int32_t XM = ~(CAND_PLAIN | CAND_MARKED);
+        // every lane runs the same sort on the same addresses (identical
+        // values, so the wave's coinciding stores agree)
+        heap_sort(c, nc, [](const Cand &a, const Cand &b) {
+            return order_key(a.x & XM, a.y) > order_key(b.x & XM, b.y);
+        });
+        PathWindow w;
+        window_init(w, wlds, it, seqs[it.q_seq], seqs[it.t_seq]);
+        SpRec *out = sps + it.cand_off;
+        int nsp = 0;
+        for (int i = 0; i < nc; ++i) {
+            const Cand ci = c[i];
+            const int fl = __builtin_amdgcn_readfirstlane(ci.x & (CAND_PLAIN | CAND_MARKED));
+            const int x = __builtin_amdgcn_readfirstlane(ci.x & ~(CAND_PLAIN | CAND_MARKED));
+            const int y = __builtin_amdgcn_readfirstlane(ci.y);
+            const float v = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ci.v)));
+            int idx;
+            const uint32_t b = window_read<false>(w, dirs, nullptr, x, y, idx);
+            float score = 0.0f;
+            bool emit = false;
+            if (!(b & DB_MARK)) {
+                if (fl & CAND_PLAIN) {
+                    emit = true;
+                    score = v;
+                    // mark the path
+                    int cx = x, cy = y;
+                    for (;;) {
+                        int o;
+                        const uint32_t cb = window_read<false>(w, dirs, nullptr, cx, cy, o);
+                        window_write(w, dirs, o, cb | DB_MARK);
+                        const uint32_t code = cb & 3u;
+                        if (cb & DB_STOP) break;
+                        if (code == 0u) break;
+                        if (AFF && code != 1u) break;
+                        const int px = cx - (code != 2u ? 1 : 0), py = cy - (code != 3u ? 1 : 0);
+                        if (px < 0 || py < 0) break;
+                        int po;
+                        if (window_read<false>(w, dirs, nullptr, px, py, po) & DB_MARK) break;
+                        cx = px;
+                        cy = py;
+                    }
+                }
+            } else if (fl & CAND_MARKED) {
+                emit = true;
+                score = -v;
+            }
+            if (emit) {
+                if (lead) {
+                    SpRec r;
+                    r.sequence = (uint32_t)it.qidx;
+                    r.target = (uint32_t)it.tidx;
+                    r.blockX = (uint32_t)x >> 3;
+                    r.blockY = (uint32_t)y >> 3;
+                    r.valueX = (uint32_t)x & 7;
+                    r.valueY = (uint32_t)y & 7;
+                    r.score = score;
+                    r.maxScore = it.maxv;
+                    r.posScore = it.mp;
+                    out[nsp] = r;
+                }
+                ++nsp;
+            }
+        }
+        if (lead) tasks[g].nsp = nsp;
+    }
+}
+
 __device__ __forceinline__ uint32_t float_order(float f) {
     const uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -1107,7 +1300,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
     const int n = task_count(ctl);
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
         const PairTask it = tasks[g];
-        if (it.status != TASK_OK) continue;
+        if (it.status != TASK_OK || long_path_pair(seqs, it)) continue;
         const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
         const int nsp = it.nsp;
         const int64_t ql = q.len, tl = t.len;
@@ -1196,6 +1389,131 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
         tasks[g].ops_off = (int64_t)ooff;
         tasks[g].nops = nops;
         tasks[g].out_off = (int32_t)out0;
+    }
+}
+
+// The same per pair of long paths, one wave per pair through an LDS window
+// (match counts from the window's residues; lane 0 stores).
+template <bool AFF>
+__global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
+                                                       PairTask *__restrict__ tasks, Ctl *__restrict__ ctl,
+                                                       uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
+                                                       uint64_t *__restrict__ keys, SpRec *__restrict__ out_sps,
+                                                       HitRec *__restrict__ out_hits, char *__restrict__ ops,
+                                                       int64_t ops_cap, double minimum) {
+    extern __shared__ uint8_t wlds[];
+    const int n = task_count(ctl);
+    const int lane = threadIdx.x & 63;
+    const bool lead = lane == 0;
+    for (int g = blockIdx.x; g < n; g += gridDim.x) {
+        const PairTask it = tasks[g];
+        if (it.status != TASK_OK || !long_path_pair(seqs, it)) continue;
+        const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
+        const int nsp = it.nsp;
+        const int64_t ql = q.len, tl = t.len;
+        const int64_t cap = nsp > 0 ? min((int64_t)nsp * (ql + tl), ql * tl + nsp) : 0;
+        unsigned long long ooff = 0, out0 = 0;
+        if (lead && cap > 0) ooff = atomicAdd(&ctl->ops_used, (unsigned long long)cap);
+        ooff = __shfl(ooff, 0, 64);
+        if ((int64_t)ooff + cap > ops_cap) {
+            if (lead) tasks[g].status = TASK_OPS_OVERFLOW;
+            continue;
+        }
+        if (lead && nsp > 0) out0 = atomicAdd(&ctl->out_used, (unsigned long long)nsp);
+        out0 = __shfl(out0, 0, 64);
+        const SpRec *sp = sps + it.cand_off;
+        uint64_t *key = keys + it.cand_off;
+        for (int i = lane; i < nsp; i += WAVE) out_sps[out0 + i] = sp[i];
+        // every lane builds and sorts the same keys at the same addresses
+        // (identical values, so the wave's coinciding stores agree)
+        for (int i = 0; i < nsp; ++i) key[i] = ((uint64_t)float_order(sp[i].score) << 32) | (uint32_t)i;
+        // ascending score, stable (SmithWaterman.py:699)
+        heap_sort(key, nsp, [](const uint64_t &a, const uint64_t &b) { return a < b; });
+        PathWindow w;
+        window_init(w, wlds, it, q, t);
+        char *op = ops + ooff;
+        int64_t nops = 0;
+        HitRec *h = out_hits + out0;
+        for (int k = 0; k < nsp; ++k) {
+            const int i = __builtin_amdgcn_readfirstlane((int)(key[k] & 0xffffffffu));
+            const SpRec r = sp[i];
+            int x = __builtin_amdgcn_readfirstlane((int)(r.blockX * 8 + r.valueX));
+            int y = __builtin_amdgcn_readfirstlane((int)(r.blockY * 8 + r.valueY));
+            const float rscore = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.score)));
+            const int s_end = x, t_end = y;
+            int s_start = s_end + 1, t_start = t_end + 1;
+            int matches = 0, mismatches = 0, len = 0;
+            int idx;
+            int direction = eff_dir<AFF>((uint8_t)window_read<true>(w, dirs, res, x, y, idx));
+            bool show = true;
+            if (it.same_id) {
+                direction = 64;
+                show = false;
+            }
+            if ((double)rscore < minimum) show = false;
+            const int64_t start_ops = nops;
+            bool term = false;
+            while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
+                int o;
+                const uint32_t b = window_read<true>(w, dirs, res, x, y, o);
+                direction = eff_dir<AFF>((uint8_t)b);
+                window_write(w, dirs, o, b | DB_WALK);
+                if (direction == 96) {
+                    show = false;
+                } else if (direction == 1 || direction == 64) {
+                    // the window holds the residues of the cell just read
+                    const int qa = __builtin_amdgcn_readfirstlane((int)w.qw[x - w.s * WAVE * w.R]);
+                    const int tb = __builtin_amdgcn_readfirstlane((int)w.tw[y - (w.t0 - (WAVE - 1))]);
+                    if (qa == tb) ++matches; else ++mismatches;
+                    if (lead) op[nops] = 'D';
+                    ++nops;
+                    ++len;
+                    --s_start;
+                    --t_start;
+                    if (direction == 1) {
+                        --x;
+                        --y;
+                    }
+                } else if (direction == 3) {
+                    if (lead) op[nops] = 'L';
+                    ++nops;
+                    ++len;
+                    --s_start;
+                    --x;
+                } else if (direction == 2) {
+                    if (lead) op[nops] = 'U';
+                    ++nops;
+                    ++len;
+                    --t_start;
+                    --y;
+                } else {
+                    term = true;
+                }
+            }
+            if (!show) nops = start_ops;
+            if (lead) {
+                HitRec hr;
+                hr.query = it.qidx;
+                hr.target = it.tidx;
+                hr.shown = show ? 1 : 0;
+                hr.s_start = s_start;
+                hr.s_end = s_end;
+                hr.t_start = t_start;
+                hr.t_end = t_end;
+                hr.matches = matches;
+                hr.mismatches = mismatches;
+                hr.length = show ? len : 0;
+                hr.score = rscore;
+                hr.sp_index = i;
+                hr.ops_offset = start_ops;
+                h[k] = hr;
+            }
+        }
+        if (lead) {
+            tasks[g].ops_off = (int64_t)ooff;
+            tasks[g].nops = nops;
+            tasks[g].out_off = (int32_t)out0;
+        }
     }
 }
 

@@ -4,7 +4,8 @@ Pairs with several stripes and at least 256 swept columns run one stripe per
 wave, each stripe waiting for the bottom row of the one above.  These tests
 cover such pairs against the oracle (linear, affine, protein), and compare the
 split runs with runs where the split is impossible (plain launch for the
-scoring pass; no room for boundary rows in the direction pass).
+scoring pass; no room for boundary rows in the direction pass).  Pairs with
+|q| + |t| >= 8192 also take the wave-per-pair start selection and walk.
 """
 import numpy as np
 import pytest
@@ -58,6 +59,21 @@ def test_long_dna_vs_oracle(dev, oracle, affine):
     assert res.n_traceback_pairs >= 2
     plain = hip_all_vs_all(dev, oracle, qs, ts, p, mode=0)
     same_result(plain, res)
+
+
+@pytest.mark.parametrize("affine", [False, True])
+def test_long_path_pairs_vs_oracle(dev, oracle, affine):
+    """Pairs with |q| + |t| >= 8192 take the wave-per-pair start selection and
+    walk (LDS windows over the direction area); the shorter pairs of the same
+    batch the thread-per-pair kernels."""
+    print("long path pairs affine", affine, flush=True)
+    rng = np.random.default_rng(60 + int(affine))
+    qs, ts = long_pairs(rng, 2, (4300, 5200), DNA, region=1500)
+    qs.append(rand_seq(rng, 900, DNA))
+    p = dna_params(oracle, affine, ff=0.05)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs >= 2
 
 
 def test_long_protein_vs_oracle(dev, oracle):
