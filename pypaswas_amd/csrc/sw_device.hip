@@ -21,8 +21,14 @@
 //    sequence, one pair per lane, see group_tile;
 //  * all score arithmetic is IEEE FP32 adds and max, in the order that yields
 //    exactly the reference's values: max(a+s, b+s) == max(a, b)+s because FP32
-//    rounding is monotone, and negative gap-state values are never clamped to
-//    -1e10 because no output depends on a negative I or J.
+//    rounding is monotone, and the affine recurrence is rewritten only in
+//    ways that keep T, the direction bytes and the maxima bit for bit (gap
+//    candidates from the unclamped diagonal sum with J clamped at 0, see
+//    group_tile; a row shift for whole-number scores in the lane-per-pair
+//    kernels);
+//  * start selection and the walk follow paths serially, one thread per pair,
+//    or for long pairs one wave per pair through LDS windows of the direction
+//    area (PathWindow).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
