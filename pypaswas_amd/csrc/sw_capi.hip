@@ -1855,22 +1855,27 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostR
         }
     };
     // large results are filled by a few host threads, each a contiguous range
-    // of pairs at its precomputed offsets (the same records as one thread)
+    // of pairs at its precomputed offsets (the same records as one thread);
+    // the fill costs ~30 ns per pair plus ~0.1 ns per byte, a thread start
+    // ~20 us
     const double bytes = (double)nops + (double)nsp * (sizeof(SpRec) + sizeof(pw_hit));
-    const int nth = bytes >= 8e6 ? 4 : (bytes >= 2e6 ? 2 : 1);
+    const double est_us = 0.03 * (double)outs.size() + 1e-4 * bytes;
+    const int nth = est_us >= 200.0 ? 4 : (est_us >= 80.0 ? 2 : 1);
     if (nth == 1) {
         fill(0, outs.size(), 0, 0);
     } else {
         std::vector<std::thread> th;
         size_t b = 0;
         int64_t sp0 = 0, op0 = 0;
+        // ranges of equal estimated cost, in units of 0.1 ns (a pair: 300)
+        const double total = 300.0 * (double)outs.size() + bytes;
         double acc = 0;
         for (int t = 0; t < nth; ++t) {
             size_t e = b;
             int64_t sp1 = sp0, op1 = op0;
-            const double goal = bytes * (t + 1) / nth;
+            const double goal = total * (t + 1) / nth;
             while (e < outs.size() && (t == nth - 1 || acc < goal)) {
-                acc += (double)outs[e].nops + (double)outs[e].nsp * (sizeof(SpRec) + sizeof(pw_hit));
+                acc += 300.0 + (double)outs[e].nops + (double)outs[e].nsp * (sizeof(SpRec) + sizeof(pw_hit));
                 sp1 += outs[e].nsp;
                 op1 += outs[e].nops;
                 ++e;
