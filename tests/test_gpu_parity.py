@@ -145,6 +145,28 @@ def test_group_kernel_protein(gdev, oracle):
     assert np.array_equal(res0.hits, res.hits)
 
 
+@pytest.mark.parametrize("affine", [False, True])
+def test_fractional_scores(gdev, oracle, affine):
+    """Scores that are not whole numbers (and not binary fractions, so sums
+    round): the lane-per-pair kernels run their unshifted form (the row shift
+    needs whole numbers), the wave-per-pair kernels the same recurrences as
+    always; both match the oracle bit for bit."""
+    print("fractional affine", affine, flush=True)
+    rng = np.random.default_rng(7 + int(affine))
+    qs, ts = overwrite_set(rng, 70, 4, (60, 200), (20, 150), DNA)
+    p = oracle.Params(matrix=oracle.dna_rna_matrix(1.1, -2.7, -1.3, 0.2), gap=-4.3,
+                      gap_ext=(-0.9 if affine else None), lower_limit=1.0, minimum=7.5, filter_factor=0.2,
+                      highest=1.1)
+    info = plan_info(gdev, oracle, qs, ts, p)
+    assert info["lane_per_pair_cells"] > 0, info
+    res = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs > 0
+    res0 = hip_all_vs_all(gdev, oracle, qs, ts, p, mode=0)
+    assert np.array_equal(res0.maxima.view(np.uint32), res.maxima.view(np.uint32))
+    assert np.array_equal(res0.hits, res.hits)
+
+
 @pytest.mark.parametrize("kind", ["dna-linear", "dna-affine", "protein"])
 def test_group_kernel_query_columns(gdev, oracle, kind):
     """Few queries against many targets of similar length: the lane-per-pair jobs
