@@ -763,7 +763,7 @@ int group_nb(int clen, const Tuning *tu) {
 
 
 // two transposed substitution tables, the swept-residue window, the staging
-// of the stripe boundary rows (64 columns in, 64 out) and, for the direction
+// of the stripe boundary rows (up to 64 columns in, 64 out) and, for the direction
 // pass, the candidate buffer
 size_t lds_bytes(int KP, bool dirs = false) {
     return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE + 2 * WAVE * sizeof(float4) +

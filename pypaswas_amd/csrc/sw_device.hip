@@ -13,8 +13,8 @@
 //  * the values a lane needs from the row above (H, or T = max(M, I, J) and
 //    the J candidate for affine gaps) arrive from lane k-1 through a DPP
 //    wave_shr:1 move; lane 0 receives the previous stripe's bottom row and
-//    lane 63 hands its own down, 64 columns at a time through LDS and a
-//    scratch row in device memory;
+//    lane 63 hands its own down, HANDOFF (32) columns at a time through LDS
+//    and a scratch row in device memory;
 //  * the substitution matrix (transposed, KP x KP, FILL row/col -1e10) and a
 //    window of B's residue codes live in LDS;
 //  * lane-per-pair jobs (the scoring pass only): 64 pairs sharing one column
@@ -100,6 +100,13 @@ __device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need) 
     __builtin_amdgcn_s_waitcnt(0);
 }
 
+// Columns of a stripe boundary row handed over at a time (through LDS, then
+// device memory, then a progress count for a split pair): the stripe below
+// can start a chunk once lane 63 of the stripe above has finished it, so the
+// pipeline lag per stripe is HANDOFF + 63 steps.
+constexpr int HANDOFF = 32;
+static_assert((HANDOFF & (HANDOFF - 1)) == 0 && HANDOFF <= WAVE, "HANDOFF: a power of two up to the wave size");
+
 // ---------------------------------------------------------------------------
 // One tile job: A along lanes (stripes of 64*R rows), B swept.
 //   DIRS = false: returns the pair maximum (max over all cells of H, or of
@@ -132,9 +139,9 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     if (s_last > nstripes) s_last = nstripes;
     // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
     const float thr = __builtin_fmaxf(t1, mp);
-    // LDS staging of the stripe boundary rows, 64 columns at a time: the row
+    // LDS staging of the stripe boundary rows, HANDOFF columns at a time: the row
     // above (read by lane 0, one column per step) and this stripe's bottom row
-    // (written by lane 63, one column per step, stored as one row of 64)
+    // (written by lane 63, one column per step, stored HANDOFF at a time)
     float4 *stg_in = stg, *stg_out = stg + WAVE;
 
     // window of the swept residues for steps [t0, t0 + BWIN): every load is
@@ -196,11 +203,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         const bool below = s + 1 < nstripes;
         const float *sin = links ? links + (int64_t)(s - 1) * row_floats : scr;  // row above (when above)
         float *sout = links ? links + (int64_t)s * row_floats : scr;             // bottom row (when below)
-        // columns [c0, c0 + 64) of the row above into LDS (one column per lane)
+        // columns [c0, c0 + HANDOFF) of the row above into LDS (one column per lane)
         auto stage_in = [&](int c0) {
             float4 v = make_float4(0.0f, bJ, 0.0f, 0.0f);
             const int c = c0 + lane;
-            if (c < blen_it) {
+            if (lane < HANDOFF && c < blen_it) {
                 v.x = load_scr(&sin[3 * c]);
                 if (AFF) v.y = load_scr(&sin[3 * c + 1]);
                 if (DIRS && AFF) v.z = load_scr(&sin[3 * c + 2]);
@@ -210,7 +217,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             __syncthreads();
         };
         if (above) {
-            if (links && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(WAVE, blen_it));
+            if (links && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(HANDOFF, blen_it));
             stage_in(0);
         }
         // inputs of step 0: lane 0 reads column 0 of the row above; the other
@@ -249,11 +256,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             // input of step t+1 for lane 0: column tn of the row above
             float4 inj = make_float4(0.0f, bJ, 0.0f, 0.0f);
             if (above && tn < nsteps) {
-                if ((tn & (WAVE - 1)) == 0) {
-                    if (links && tn < blen_it) wait_progress(prog + (s - 1), (unsigned)min(tn + WAVE, blen_it));
+                if ((tn & (HANDOFF - 1)) == 0) {
+                    if (links && tn < blen_it) wait_progress(prog + (s - 1), (unsigned)min(tn + HANDOFF, blen_it));
                     stage_in(tn);
                 }
-                inj = stg_in[tn & (WAVE - 1)];
+                inj = stg_in[tn & (HANDOFF - 1)];
             }
 
             float tup = upT, jin = upJ;
@@ -404,12 +411,12 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             if (below) {
                 const int c63 = t - (WAVE - 1);
                 if (c63 >= 0 && c63 < blen_it) {
-                    if (lane == WAVE - 1) stg_out[c63 & (WAVE - 1)] = make_float4(botT, botJ, botM, 0.0f);
-                    if ((c63 & (WAVE - 1)) == WAVE - 1 || c63 == blen_it - 1) {
+                    if (lane == WAVE - 1) stg_out[c63 & (HANDOFF - 1)] = make_float4(botT, botJ, botM, 0.0f);
+                    if ((c63 & (HANDOFF - 1)) == HANDOFF - 1 || c63 == blen_it - 1) {
                         __syncthreads();
                         const float4 v = stg_out[lane];
-                        const int c = (c63 & ~(WAVE - 1)) + lane;
-                        if (c <= c63) {
+                        const int c = (c63 & ~(HANDOFF - 1)) + lane;
+                        if (lane < HANDOFF && c <= c63) {
                             store_scr(&sout[3 * c], v.x);
                             if (AFF) store_scr(&sout[3 * c + 1], v.y);
                             if (DIRS && AFF) store_scr(&sout[3 * c + 2], v.z);
