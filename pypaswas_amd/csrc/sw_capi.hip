@@ -1365,10 +1365,26 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
     // streams must not wait for that kernel
     if (ngs > 1 && fork()) return PW_EDEVICE;
     if (P->n_groups > 0) {
+        // launch order: the two narrowest widths (short jobs, done early) and
+        // the two widest (the longest jobs, the critical path) first, then the
+        // middle widths widest first, round robin over the streams
+        int pres[NB_KINDS], npres = 0;
+        for (int w = 0; w < NB_KINDS; ++w)
+            if (P->nb_end[w] > P->nb_begin[w]) pres[npres++] = w;
+        int order[NB_KINDS], no = 0;
+        if (npres >= 4) {
+            order[no++] = pres[0];
+            order[no++] = pres[1];
+            order[no++] = pres[npres - 2];
+            order[no++] = pres[npres - 1];
+            for (int i = npres - 3; i >= 2; --i) order[no++] = pres[i];
+        } else {
+            for (int i = 0; i < npres; ++i) order[no++] = pres[i];
+        }
         int nk = 0;
-        for (int w = 0; w < NB_KINDS; ++w) {
+        for (int oi = 0; oi < no; ++oi) {
+            const int w = order[oi];
             const int64_t n = P->nb_end[w] - P->nb_begin[w];
-            if (n <= 0) continue;
             const int si = P->mode == 1 ? (nk++ % ngs) : 0;
             if (use(si)) return PW_EDEVICE;
             hipStream_t gs = gstreams[si];
