@@ -80,13 +80,13 @@ struct Limits {
 // Device work areas of one execution, plus the pinned host staging of the
 // read-back.  want_* are the sizes the next execution allocates.
 struct Workspace {
-    DevBuf ctl, maxima, gated, tasks, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch,
+    DevBuf ctl, maxima, gated, tasks, touts, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch,
         dscratch, units, dlinks, dprog;
     int64_t want_dir = 0, want_cand = 0, want_ops = 0;
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
     int64_t units_cap = 0, dlinks_cap = 0, dprog_cap = 0;
-    PinBuf h_ctl, h_tasks, h_sps, h_hits, h_ops;
+    PinBuf h_ctl, h_tasks, h_touts, h_sps, h_hits, h_ops;
     // direction area lent by a caller (a range of a serving arena) instead of dirs
     uint8_t *dir_ext = nullptr;
     int64_t dir_ext_cap = 0;
@@ -397,10 +397,11 @@ void free_pinned(PinBuf &b) {
 }
 
 void free_workspace(pw_ctx *ctx, Workspace &W) {
-    DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
-                      &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks, &W.dprog};
+    DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.touts, &W.dirs, &W.cands, &W.spslot, &W.keys,
+                      &W.out_sps, &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks,
+                      &W.dprog};
     for (DevBuf *b : bufs) free_buf(ctx, *b);
-    PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_sps, &W.h_hits, &W.h_ops};
+    PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_touts, &W.h_sps, &W.h_hits, &W.h_ops};
     for (PinBuf *b : pins) free_pinned(*b);
 }
 
@@ -1287,6 +1288,7 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     ENSURE(ctx, W.ctl, sizeof(Ctl));
     ENSURE(ctx, W.maxima, (size_t)n * sizeof(float));
     ENSURE(ctx, W.tasks, (size_t)n * sizeof(PairTask));
+    ENSURE(ctx, W.touts, (size_t)n * sizeof(TaskOut));
     ENSURE(ctx, W.gated, (size_t)n * sizeof(int));
     if (W.dir_ext) {
         W.dir_cap = W.dir_ext_cap;
@@ -1473,13 +1475,13 @@ int launch_walk(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
     auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
     hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                        (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
-                       (char *)W.ops.p, W.ops_cap, minimum);
+                       (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p);
     HIPCHK(ctx, hipGetLastError());
     if (g.wave_grid > 0) {
         auto ww = aff ? walk_wave_kernel<true> : walk_wave_kernel<false>;
         hipLaunchKernelGGL(ww, dim3(g.wave_grid), dim3(64), WALK_LDS, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                            (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
-                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum);
+                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p);
         HIPCHK(ctx, hipGetLastError());
     }
     return PW_OK;
@@ -1612,11 +1614,13 @@ int read_tasks(pw_ctx *ctx, Workspace &W, Lane &L, int ntasks, std::vector<PairO
     if (ntasks <= 0) return PW_OK;
     const int64_t nout = std::min<int64_t>((int64_t)c.out_used, W.cand_cap);
     const int64_t nops = std::min<int64_t>((int64_t)c.ops_used, W.ops_cap);
-    ENSURE_PINNED(ctx, W.h_tasks, (size_t)ntasks * sizeof(PairTask));
+    // per-task summaries (32 bytes; the whole 120-byte tasks only when some
+    // did not finish and go to a tail round)
+    ENSURE_PINNED(ctx, W.h_touts, (size_t)ntasks * sizeof(TaskOut));
     ENSURE_PINNED(ctx, W.h_sps, (size_t)nout * sizeof(SpRec));
     ENSURE_PINNED(ctx, W.h_hits, (size_t)nout * sizeof(HitRec));
     ENSURE_PINNED(ctx, W.h_ops, (size_t)nops);
-    HIPCHK(ctx, hipMemcpyAsync(W.h_tasks.p, W.tasks.p, (size_t)ntasks * sizeof(PairTask), hipMemcpyDeviceToHost,
+    HIPCHK(ctx, hipMemcpyAsync(W.h_touts.p, W.touts.p, (size_t)ntasks * sizeof(TaskOut), hipMemcpyDeviceToHost,
                                L.s));
     if (nout > 0) {
         HIPCHK(ctx, hipMemcpyAsync(W.h_sps.p, W.out_sps.p, (size_t)nout * sizeof(SpRec), hipMemcpyDeviceToHost, L.s));
@@ -1625,11 +1629,21 @@ int read_tasks(pw_ctx *ctx, Workspace &W, Lane &L, int ntasks, std::vector<PairO
     }
     if (nops > 0) HIPCHK(ctx, hipMemcpyAsync(W.h_ops.p, W.ops.p, (size_t)nops, hipMemcpyDeviceToHost, L.s));
     HIPCHK(ctx, hipStreamSynchronize(L.s));
-    const PairTask *T = (const PairTask *)W.h_tasks.p;
+    const TaskOut *T = (const TaskOut *)W.h_touts.p;
+    bool unfinished = false;
+    for (int i = 0; i < ntasks && !unfinished; ++i) unfinished = T[i].status != TASK_OK;
+    const PairTask *full = nullptr;
+    if (unfinished) {
+        ENSURE_PINNED(ctx, W.h_tasks, (size_t)ntasks * sizeof(PairTask));
+        HIPCHK(ctx, hipMemcpyAsync(W.h_tasks.p, W.tasks.p, (size_t)ntasks * sizeof(PairTask), hipMemcpyDeviceToHost,
+                                   L.s));
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        full = (const PairTask *)W.h_tasks.p;
+    }
     for (int i = 0; i < ntasks; ++i) {
-        const PairTask &t = T[i];
+        const TaskOut &t = T[i];
         if (t.status != TASK_OK) {
-            pending.push_back(t);
+            pending.push_back(full[i]);
             continue;
         }
         if ((int64_t)t.out_off + t.nsp > nout || t.ops_off + t.nops > nops) {
@@ -1919,9 +1933,10 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
                                       double minimum) {
     std::vector<uint64_t> s;
     auto add = [&](uint64_t v) { s.push_back(v); };
-    const DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.dirs, &W.cands, &W.spslot, &W.keys, &W.out_sps,
-                            &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks,
-                            &W.dprog, &P->d_links, &P->d_prog};
+    const DevBuf *bufs[] = {&W.ctl,      &W.maxima,   &W.gated,    &W.tasks,    &W.touts,    &W.dirs,
+                            &W.cands,    &W.spslot,   &W.keys,     &W.out_sps,  &W.out_hits, &W.ops,
+                            &W.gscratch, &W.tscratch, &W.dscratch, &W.units,    &W.dlinks,   &W.dprog,
+                            &P->d_links, &P->d_prog};
     for (const DevBuf *b : bufs) {
         add((uint64_t)(uintptr_t)b->p);
         add(b->cap);

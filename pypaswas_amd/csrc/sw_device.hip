@@ -1309,11 +1309,15 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
                                                   uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
                                                   uint64_t *__restrict__ keys, SpRec *__restrict__ out_sps,
                                                   HitRec *__restrict__ out_hits, char *__restrict__ ops,
-                                                  int64_t ops_cap, double minimum) {
+                                                  int64_t ops_cap, double minimum, TaskOut *__restrict__ touts) {
     const int n = task_count(ctl);
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
         const PairTask it = tasks[g];
-        if (it.status != TASK_OK || long_path_pair(seqs, it)) continue;
+        if (it.status != TASK_OK) {
+            touts[g] = TaskOut{it.pair, it.status, 0, 0, 0, 0};
+            continue;
+        }
+        if (long_path_pair(seqs, it)) continue;
         const SeqRef q = seqs[it.q_seq], t = seqs[it.t_seq];
         const int nsp = it.nsp;
         const int64_t ql = q.len, tl = t.len;
@@ -1321,6 +1325,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
         const unsigned long long ooff = cap > 0 ? atomicAdd(&ctl->ops_used, (unsigned long long)cap) : 0ull;
         if ((int64_t)ooff + cap > ops_cap) {
             tasks[g].status = TASK_OPS_OVERFLOW;
+            touts[g] = TaskOut{it.pair, TASK_OPS_OVERFLOW, 0, 0, 0, 0};
             continue;
         }
         const unsigned long long out0 = nsp > 0 ? atomicAdd(&ctl->out_used, (unsigned long long)nsp) : 0ull;
@@ -1402,6 +1407,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
         tasks[g].ops_off = (int64_t)ooff;
         tasks[g].nops = nops;
         tasks[g].out_off = (int32_t)out0;
+        touts[g] = TaskOut{it.pair, TASK_OK, nsp, (int32_t)out0, (int64_t)ooff, nops};
     }
 }
 
@@ -1413,7 +1419,7 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
                                                        uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
                                                        uint64_t *__restrict__ keys, SpRec *__restrict__ out_sps,
                                                        HitRec *__restrict__ out_hits, char *__restrict__ ops,
-                                                       int64_t ops_cap, double minimum) {
+                                                       int64_t ops_cap, double minimum, TaskOut *__restrict__ touts) {
     extern __shared__ uint8_t wlds[];
     const int n = task_count(ctl);
     const int lane = threadIdx.x & 63;
@@ -1429,7 +1435,10 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
         if (lead && cap > 0) ooff = atomicAdd(&ctl->ops_used, (unsigned long long)cap);
         ooff = __shfl(ooff, 0, 64);
         if ((int64_t)ooff + cap > ops_cap) {
-            if (lead) tasks[g].status = TASK_OPS_OVERFLOW;
+            if (lead) {
+                tasks[g].status = TASK_OPS_OVERFLOW;
+                touts[g] = TaskOut{it.pair, TASK_OPS_OVERFLOW, 0, 0, 0, 0};
+            }
             continue;
         }
         if (lead && nsp > 0) out0 = atomicAdd(&ctl->out_used, (unsigned long long)nsp);
@@ -1526,6 +1535,7 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             tasks[g].ops_off = (int64_t)ooff;
             tasks[g].nops = nops;
             tasks[g].out_off = (int32_t)out0;
+            touts[g] = TaskOut{it.pair, TASK_OK, nsp, (int32_t)out0, (int64_t)ooff, nops};
         }
     }
 }

@@ -104,6 +104,18 @@ struct PairTask {
     int64_t prog_off;    // split: progress counters in the counter area
 };
 
+// What the host reads back per task: where the pair's records sit (written
+// by the walk; the status of a task that did not finish, so that only then the
+// whole PairTask has to be read).
+struct TaskOut {
+    int32_t pair;
+    int32_t status;
+    int32_t nsp;
+    int32_t out_off;
+    int64_t ops_off;
+    int64_t nops;
+};
+
 // status of a PairTask
 constexpr int32_t TASK_OK = 0;
 constexpr int32_t TASK_DEFERRED = 1;       // no room in the direction or candidate area
