@@ -347,12 +347,14 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         const float l = tup + G, u = Tp[r] + G;
                         const float cur = fmax3(__builtin_fmaxf(l, u), d, 0.0f);
                         // STOP when the predecessor along the code is 0; only
-                        // diagonal cells are candidate starts
+                        // diagonal cells are candidate starts.  With G <= 0 a
+                        // positive gap value cur = v + G has v >= cur > 0, so
+                        // only the diagonal predecessor hd can be 0 where the
+                        // byte is kept (cur > 0)
                         const bool isl = cur == l, isu = cur == u;
-                        const float pv = isl ? tup : (isu ? Tp[r] : hd);
-                        uint32_t stop = pv == 0.0f ? DB_STOP : 0u;
-                        asm volatile("" : "+v"(stop));
-                        const uint32_t b = (isl ? 3u : (isu ? 2u : (uint32_t)DB_MAIN)) | stop;
+                        uint32_t dgc = hd == 0.0f ? (uint32_t)(DB_MAIN | DB_STOP) : (uint32_t)DB_MAIN;
+                        asm volatile("" : "+v"(dgc));
+                        const uint32_t b = isl ? 3u : (isu ? 2u : dgc);
                         const bool pos = cur > 0.0f;
                         bits[r] = pos ? b : 0u;
                         cand[r] = pos && !isl && !isu && cur >= thr;  // cur >= 0 (as M above)

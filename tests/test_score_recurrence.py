@@ -202,3 +202,53 @@ def test_clamp_free_needs_nonpositive_gaps():
     S = np.array([[2.0, -3.0, -3.0], [-3.0, -3.0, -3.0]], np.float32)
     G, E = np.float32(1.0), np.float32(-1.0)
     assert not np.array_equal(reference_T(S, G, E), clamp_free_T(S, G, E))
+
+
+def _linear_codes(S, G, thr, diagonal_stop_only):
+    """Linear-gap direction bytes (default_main_cpu.cl:162-177): the value is
+    max(0, left, up, diag) and the code that of the first of left, up, diag
+    equal to it when positive, with STOP when the predecessor along the code
+    is 0 -- or, in the direction pass's form, when the diagonal predecessor
+    is 0 (gap codes never stop for G <= 0)."""
+    n, m = S.shape
+    H = np.zeros((n + 1, m + 1), np.float32)
+    code = np.zeros((n, m), np.uint8)
+    cand = np.zeros((n, m), bool)
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            hd, left, up = H[i - 1, j - 1], H[i, j - 1], H[i - 1, j]
+            d = np.float32(hd + S[i - 1, j - 1])
+            lv, uv = np.float32(left + G), np.float32(up + G)
+            cur = max(max(lv, uv), d, np.float32(0))
+            isl, isu = cur == lv, cur == uv
+            if diagonal_stop_only:
+                b = 3 if isl else (2 if isu else (1 | (4 if hd == 0 else 0)))
+            else:
+                pv = left if isl else (up if isu else hd)
+                b = (3 if isl else (2 if isu else 1)) | (4 if pv == 0 else 0)
+            code[i - 1, j - 1] = b if cur > 0 else 0
+            cand[i - 1, j - 1] = cur > 0 and not isl and not isu and cur >= thr
+            H[i, j] = cur
+    return code, cand
+
+
+@pytest.mark.parametrize("kind", ["dna", "protein", "fractional"])
+@pytest.mark.parametrize("G", [-5.0, -1.0, -0.5, 0.0])
+def test_linear_direction_stop_on_diagonal_only(kind, G):
+    G = np.float32(G)
+    rng = np.random.default_rng(zlib.crc32(repr(("lin", kind, float(G))).encode()))
+    for _ in range(6):
+        n, m = int(rng.integers(1, 20)), int(rng.integers(1, 20))
+        S = _scores(rng, n, m, kind)
+        thr = np.float32(rng.integers(1, 6))
+        ref = _linear_codes(S, G, thr, False)
+        alt = _linear_codes(S, G, thr, True)
+        assert np.array_equal(ref[0], alt[0]) and np.array_equal(ref[1], alt[1]), (kind, G, n, m)
+
+
+def test_linear_stop_rule_needs_nonpositive_gap():
+    """With G > 0 a gap cell can follow a zero cell, and the two forms part."""
+    S = np.array([[-3.0, -3.0], [-3.0, -3.0]], np.float32)
+    ref = _linear_codes(S, np.float32(1.0), np.float32(1.0), False)
+    alt = _linear_codes(S, np.float32(1.0), np.float32(1.0), True)
+    assert not np.array_equal(ref[0], alt[0])
