@@ -372,6 +372,10 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                     dst[0] = (uint8_t)bits[0];
                 } else if (R == 2) {
                     *reinterpret_cast<uint16_t *>(dst) = (uint16_t)(bits[0] | (bits[1 % R] << 8));
+                } else if (R == 6) {
+#pragma unroll
+                    for (int w = 0; w < 3; ++w)
+                        reinterpret_cast<uint16_t *>(dst)[w] = (uint16_t)(bits[(2 * w) % R] | (bits[(2 * w + 1) % R] << 8));
                 } else {
 #pragma unroll
                     for (int w = 0; w < R / 4; ++w) {
@@ -779,6 +783,7 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
         case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
         case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
         case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 6: tile<AFF, 6, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
         default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
         }
     }
@@ -792,10 +797,21 @@ __global__ void __launch_bounds__(256) clear_kernel(uint32_t *__restrict__ p, in
 
 // Query residues per lane of the direction pass: the smallest power of two
 // whose 64-lane stripe covers the query, at most rmax (a plan lowers rmax when
-// its pairs would otherwise give too few stripe units to fill the device).
+// its pairs would otherwise give too few stripe units to fill the device),
+// except that a query of 257-384 residues takes 6 (one stripe of 384 rows
+// instead of 512: a quarter fewer padded rows).
 __device__ __host__ __forceinline__ int dir_residues(int qlen, int rmax = 8) {
     int r = 1;
     while (r < rmax && WAVE * r < qlen) r *= 2;
+    if (r == 8 && qlen <= WAVE * 6) r = 6;
+    return r;
+}
+
+// Residues per lane whose direction area bounds that of every rmax above (an
+// arena reservation made before the plan's rmax is known).
+__device__ __host__ __forceinline__ int dir_residues_bound(int qlen) {
+    int r = 1;
+    while (r < 8 && WAVE * r < qlen) r *= 2;
     return r;
 }
 

@@ -450,7 +450,7 @@ extern "C" int pw_server_submit(pw_server *S, int32_t scheme, const uint8_t *qre
     else j->same.assign((size_t)n_pairs, 0);
     for (int64_t i = 0; i < n_pairs; ++i) {
         const int ql = qlen[pq[i]], tl = tlen[pt[i]];
-        j->dir_need += dir_area(ql, tl, dir_residues(ql));
+        j->dir_need += dir_area(ql, tl, dir_residues_bound(ql));
         j->cells += (double)ql * tl;
     }
     S->jobs[j->id] = j;

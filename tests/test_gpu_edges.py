@@ -135,3 +135,27 @@ def test_gate_passes_nothing(dev, oracle, mode):
     compare(oracle, qs, ts, p, res)
     assert res.maxima.max() > 0
     assert len(res.starting_points) == 0 and len(res.hits) == 0 and res.n_traceback_pairs == 0
+
+
+@pytest.mark.parametrize("rmax", [2, 4, 8])
+def test_direction_residues_per_lane(oracle, rmax):
+    """The direction pass with its residues per lane capped at 2, 4 and 8
+    (pw_set_routing), on protein queries of 40 to 700 residues: one- and
+    multi-stripe pairs of every stripe height the cap allows."""
+    from pypaswas_amd._native import Device
+    from tests._parity import AMINO
+    print("direction residues cap", rmax, flush=True)
+    rng = np.random.default_rng(zlib.crc32(b"dir-residues"))
+    qs, ts = planted_set(rng, 8, 4, (40, 700), (60, 400), AMINO, plant=0.9, sub=0.3, indel=0.03)
+    p = oracle.Params(matrix=oracle.BLOSUM62, gap=-11.0, gap_ext=-1.0, lower_limit=0.8, minimum=25.0,
+                      filter_factor=0.05, highest=float(max(max(oracle.BLOSUM62))), dna=False)
+    d = Device(0)
+    try:
+        d.set_routing(-1, 0.0, rmax)
+        info = plan_info(d, oracle, qs, ts, p)
+        assert info["direction_residues_per_lane"] == rmax, info
+        res = hip_all_vs_all(d, oracle, qs, ts, p, mode=1)
+        compare(oracle, qs, ts, p, res)
+        assert res.n_traceback_pairs > 0
+    finally:
+        d.close()
