@@ -14,8 +14,10 @@ Python side of the CPU restatement of pyPaSWAS's OpenCL-CPU path:
   (cross-checks ``oracle_walk`` in C);
 * ``HitList.append`` (``pyPaSWAS/Core/HitList.py:28-52``), the hit metrics of
   ``Hit.set_scores`` (``Hit.py:91-119``), the filter
-  (``SmithWaterman.py:843-848``) and the TXT record
-  (``Formatters.py:19-39``).
+  (``SmithWaterman.py:843-848``) and the records: TXT (``Formatters.py:19-39``),
+  SAM (``Formatters.py:51-106``, ``Hit.py:222-406``), trimmed FASTA
+  (``Formatters.py:110-149``, ``Hit.py:274-288``) and FASTA
+  (``Formatters.py:151-191``, ``Hit.py:305-307``).
 """
 from __future__ import annotations
 
@@ -146,26 +148,31 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def bind(path: str):
+    """Loads a build of sw_oracle.c from ``path`` with its function signatures."""
+    L = ctypes.CDLL(path)
+    L.oracle_sched_batch.restype = ctypes.c_long
+    L.oracle_sched_batch.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(_SwParams), ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    L.oracle_serial_pair.restype = ctypes.c_long
+    L.oracle_serial_pair.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_float,
+                                     ctypes.POINTER(_SwParams), ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_long]
+    L.oracle_walk.restype = ctypes.c_long
+    L.oracle_walk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_long,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                              ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.oracle_sched_batch.restype = ctypes.c_long
-        L.oracle_sched_batch.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(_SwParams), ctypes.c_int,
-                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
-        L.oracle_serial_pair.restype = ctypes.c_long
-        L.oracle_serial_pair.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_float,
-                                         ctypes.POINTER(_SwParams), ctypes.c_void_p, ctypes.c_void_p,
-                                         ctypes.c_void_p, ctypes.c_long]
-        L.oracle_walk.restype = ctypes.c_long
-        L.oracle_walk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_long,
-                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
-                                  ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
-                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]
-        _lib = L
+        _lib = bind(LIB_PATH)
     return _lib
 
 
@@ -209,7 +216,7 @@ class BatchResult:
 
 
 def sched_batch(queries: Sequence[str], targets: Sequence[str], p: Params, nthreads: int = 0,
-                sp_cap: int = 1 << 20) -> BatchResult:
+                sp_cap: int = 1 << 20, L=None) -> BatchResult:
     """One SmithWatermanCPU batch: all queries x all targets (SmithWaterman.py:441-540)."""
     N, T = len(queries), len(targets)
     X = pad8(max(len(q) for q in queries))
@@ -225,7 +232,7 @@ def sched_batch(queries: Sequence[str], targets: Sequence[str], p: Params, nthre
     maxima = np.zeros(N * T, dtype=np.float32)
     dirs = np.zeros(N * T * X * Y, dtype=np.uint8)
     sps = (SpRecord * sp_cap)()
-    n = lib().oracle_sched_batch(seqs, N, X, tgts, T, Y, minsc.ctypes.data, ctypes.byref(cp), nthreads,
+    n = (L or lib()).oracle_sched_batch(seqs, N, X, tgts, T, Y, minsc.ctypes.data, ctypes.byref(cp), nthreads,
                                  maxima.ctypes.data, dirs.ctypes.data, ctypes.addressof(sps), sp_cap)
     if n < 0:
         raise MemoryError("oracle_sched_batch: allocation failed")
@@ -233,21 +240,36 @@ def sched_batch(queries: Sequence[str], targets: Sequence[str], p: Params, nthre
                        X, Y, n)
 
 
-def serial_pair(q: str, t: str, p: Params, mp: Optional[float] = None, sp_cap: int = 1 << 16):
-    """Independent serial restatement for one pair (no padding)."""
+class CParams:
+    """The C form of a Params, built once and reused over many serial_pair calls."""
+
+    def __init__(self, p: Params):
+        self.keep: list = []
+        self.cp = _c_params(p, self.keep)
+
+
+def serial_pair(q: str, t: str, p: Params, mp: Optional[float] = None, sp_cap: int = 256, L=None,
+                cparams: Optional[CParams] = None):
+    """Independent serial restatement for one pair (no padding).  Every starting
+    point is returned (the call is repeated with room for all of them when more
+    than sp_cap were found)."""
     q = q.upper(); t = t.upper()
     if mp is None:
         mp = min_start_score(p, len(q), len(t))
-    keep: list = []
-    cp = _c_params(p, keep)
-    mx = ctypes.c_float()
-    dirs = np.zeros(len(q) * len(t), dtype=np.uint8)
-    sps = (SpRecord * sp_cap)()
-    n = lib().oracle_serial_pair(q.encode(), len(q), t.encode(), len(t), np.float32(mp), ctypes.byref(cp),
-                                 ctypes.addressof(mx), dirs.ctypes.data, ctypes.addressof(sps), sp_cap)
-    if n < 0:
-        raise MemoryError("oracle_serial_pair: allocation failed")
-    return float(mx.value), dirs.reshape(len(q), len(t)), [sps[i] for i in range(min(n, sp_cap))]
+    cparams = cparams or CParams(p)  # held for the call: cp points into its arrays
+    cp = cparams.cp
+    while True:
+        mx = ctypes.c_float()
+        dirs = np.zeros(len(q) * len(t), dtype=np.uint8)
+        sps = (SpRecord * sp_cap)()
+        n = (L or lib()).oracle_serial_pair(q.encode(), len(q), t.encode(), len(t), np.float32(mp),
+                                            ctypes.byref(cp), ctypes.addressof(mx), dirs.ctypes.data,
+                                            ctypes.addressof(sps), sp_cap)
+        if n < 0:
+            raise MemoryError("oracle_serial_pair: allocation failed")
+        if n <= sp_cap:
+            return float(mx.value), dirs.reshape(len(q), len(t)), [sps[i] for i in range(n)]
+        sp_cap = int(n)
 
 
 # --------------------------------------------------------------------------
@@ -342,7 +364,7 @@ def walk_py(directions: np.ndarray, sps: Sequence[SpRecord], queries: Sequence[s
 
 
 def walk_c(directions: np.ndarray, sps: Sequence[SpRecord], queries: Sequence[str], targets: Sequence[str],
-           qids: Sequence[str], tids: Sequence[str], minimum: float) -> List[WalkHit]:
+           qids: Sequence[str], tids: Sequence[str], minimum: float, L=None) -> List[WalkHit]:
     """Same walk through ``oracle_walk`` (C)."""
     N, T, X, Y = directions.shape
     nsp = len(sps)
@@ -358,7 +380,7 @@ def walk_c(directions: np.ndarray, sps: Sequence[SpRecord], queries: Sequence[st
     cap = sum(len(queries[sp.sequence]) + len(targets[sp.target]) for sp in sps) + 1
     ops = ctypes.create_string_buffer(cap)
     D = np.ascontiguousarray(directions)
-    lib().oracle_walk(D.ctypes.data, X, Y, ctypes.addressof(arr), nsp, pair_of.ctypes.data, same.ctypes.data,
+    (L or lib()).oracle_walk(D.ctypes.data, X, Y, ctypes.addressof(arr), nsp, pair_of.ctypes.data, same.ctypes.data,
                       float(minimum), qarr, tarr, ctypes.addressof(res), order.ctypes.data, ops, cap)
     directions[...] = D
     out = []
@@ -413,6 +435,7 @@ class OHit:
     query_identity: float = 0.0
     query_coverage: float = 0.0
     base_score: float = 0.0
+    qseq: str = ""  # the query record's sequence (SAM SEQ, trimmed FASTA)
 
     def keys(self):
         return [(self.qid, self.tid, self.s[0], self.t[0]), (self.qid, self.tid, self.s[1], self.t[1]),
@@ -423,12 +446,12 @@ def _valid(loc, n):
     return 0 <= loc[0] < n and 0 <= loc[1] < n and loc[0] < loc[1]
 
 
-def make_hit(w: WalkHit, qid: str, tid: str, qlen: int, tlen: int) -> OHit:
+def make_hit(w: WalkHit, qid: str, tid: str, qlen: int, tlen: int, qseq: str = "") -> OHit:
     s, t = (w.s_start, w.s_end), (w.t_start, w.t_end)
     if not _valid(s, qlen) or not _valid(t, tlen):
         raise InvalidLocation((s, t))
     h = OHit(qid, tid, qlen, tlen, s, t, w.score, w.matches, w.mismatches, w.sequence_match, w.alignment,
-             w.target_match)
+             w.target_match, qseq=qseq)
     # Hit.set_scores, Hit.py:111-119
     h.relative_score = float(h.score) / float(tlen) if qlen > tlen else float(h.score) / float(qlen)
     h.query_identity = float(w.matches) / float(s[1] - s[0] + 1)
@@ -476,6 +499,99 @@ def txt_records(hitlist: OHitList) -> str:
     return "".join(l + "\n" for l in lines)
 
 
+def _hits_in_output_order(hitlist: OHitList):
+    """DefaultFormatter._get_hits, Formatters.py:36-39: (qid, tid, score), ties in insertion order."""
+    return sorted(hitlist.real_hits.values(), key=lambda h: (h.qid, h.tid, h.score))
+
+
+def _cigar_char(aln: str, qm: str, i: int) -> str:
+    """Hit._get_cigar_char, Hit.py:393-406."""
+    if aln[i] == "|":
+        return "="
+    if aln[i] == ".":
+        return "X"
+    return "D" if qm[i] == "-" else "I"
+
+
+def sam_line(h: OHit) -> str:
+    """Hit.get_sam_line, Hit.py:222-272, with _get_sam_cigar (:338-372) and the tags (:318-391).
+
+    A hit with an id ending in 'RC' (query or target) is a reverse-complement
+    hit: the alignment and query strings are reversed (the reference does it in
+    place, :258-260), SEQ is the reverse complement of the whole query, POS is
+    len(target) - t_end, and the CIGAR always starts with a soft clip (even 0S).
+    """
+    qid, tid = h.qid, h.tid
+    rc_q, rc_t = qid[-2:] == "RC", tid[-2:] == "RC"
+    rc = rc_q or rc_t
+    name = qid[:-3] if rc_q else qid
+    ref = tid[:-3] if rc_t else tid
+    aln, qm = h.alignment, h.sequence_match
+    pos = str(h.t[0] + 1)
+    seq = h.qseq
+    if rc:
+        aln, qm = aln[::-1], qm[::-1]
+        seq = reverse_complement(h.qseq)
+        pos = str(h.tlen - h.t[1])
+    if not qm or not h.target_match or not aln:
+        raise InvalidLocation("empty alignment strings")
+    parts = []
+    if not rc:
+        if h.s[0] > 0:
+            parts.append(str(h.s[0]) + "S")
+    else:
+        parts.append(str(len(h.qseq) - h.s[1] - 1) + "S")
+    cur, count, i = aln[0], 0, 0
+    for i in range(len(aln)):
+        if aln[i] == cur:
+            count += 1
+        else:
+            parts.append(str(count) + _cigar_char(aln, qm, i - 1))
+            cur, count = aln[i], 1
+    parts.append(str(count) + _cigar_char(aln, qm, i))
+    tail = h.s[0] if rc else h.qlen - h.s[1] - 1
+    if tail > 0:
+        parts.append(str(tail) + "S")
+    flag = "16" if rc else "0"
+    return "\t".join([name, flag, ref, pos, "255", "".join(parts), "*", "0", "0", seq, "*",
+                      "AS:i:" + str(int(h.score)), "RS:f:" + str(h.relative_score), "BS:f:" + str(h.base_score),
+                      "QC:f:" + str(h.query_coverage), "QI:f:" + str(h.query_identity)])
+
+
+def sam_records(hitlist: OHitList) -> str:
+    """SamFormatter.print_results, Formatters.py:51-106: @HD, one @SQ per target id
+    (RC suffix removed) in first-seen order, @PG, then one line per hit."""
+    sq = {}
+    lines = []
+    for h in _hits_in_output_order(hitlist):
+        if h.tid not in sq:
+            key = h.tid if h.tid[-2:] != "RC" else h.tid[:-3]
+            ref = h.tid if h.tid[-2:] != "RC" else h.tid[:-3]
+            sq[key] = "\t".join(["@SQ", "SN:" + ref, "LN:" + str(h.tlen)])
+        lines.append(sam_line(h))
+    out = ["@HD\tVN:1.4\tSO:unknown"] + list(sq.values()) + ["@PG\tID:0\tPN:paswas\tVN:3.0"] + lines
+    return "".join(l + "\n" for l in out)
+
+
+def trimmed_records(hitlist: OHitList) -> str:
+    """TrimmerFormatter (Formatters.py:110-149) with Hit.get_trimmed_line (Hit.py:274-288):
+    '>' target id (RC suffix removed), a space, the query id, then the query with the
+    columns [t_start, t_end] removed -- the query cut at the *target* coordinates."""
+    out = []
+    for h in _hits_in_output_order(hitlist):
+        if not h.sequence_match or not h.target_match or not h.alignment:
+            raise InvalidLocation("empty alignment strings")
+        tid = h.tid[:-3] if h.tid[-2:] == "RC" else h.tid
+        out.append(">" + tid + " " + h.qid + "\n" + h.qseq[:h.t[0]] + h.qseq[h.t[1] + 1:])
+    return "".join(l + "\n" for l in out)
+
+
+def fasta_records(hitlist: OHitList) -> str:
+    """FASTA formatter (Formatters.py:151-191) with Hit.get_full_fasta (Hit.py:305-307):
+    the whole query record once per hit."""
+    return "".join(">{}\n{}\n".format(h.qid, h.qseq) for h in _hits_in_output_order(hitlist))
+
+
 def reverse_complement(s: str) -> str:
     """IUPAC reverse complement (Biopython Seq.reverse_complement on DNA letters)."""
     comp = {"A": "T", "C": "G", "G": "C", "T": "A", "U": "A", "N": "N", "R": "Y", "Y": "R", "S": "S", "W": "W",
@@ -500,7 +616,8 @@ def align_all(records_q, records_t, p: Params, walk=walk_c, nthreads: int = 0):
     for w in ws:
         if not w.shown:
             continue
-        h = make_hit(w, records_q[w.qi][0], records_t[w.ti][0], len(queries[w.qi]), len(targets[w.ti]))
+        h = make_hit(w, records_q[w.qi][0], records_t[w.ti][0], len(queries[w.qi]), len(targets[w.ti]),
+                     queries[w.qi])
         if passes_filter(h, p):
             hl.append(h)
     return hl, br, ws

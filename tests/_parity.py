@@ -138,3 +138,71 @@ def decode_ops(ops: bytes, off: int, n: int, q: str, t: str, s_start: int, t_sta
         else:
             sm.append("-"); tm.append(t[y]); al.append(" "); y += 1
     return "".join(sm), "".join(al), "".join(tm)
+
+
+def oracle_pair(O, q, t, qid, tid, p, mp=None, cparams=None):
+    """One pair through the serial restatement and the walk on its own directions.
+
+    Returns (maximum, starting points as tuples, walks as tuples) in the forms
+    compare() uses.  Pairs are independent: the IN_ALIGNMENT marks of a walk
+    stay inside its pair, and the pair's walks keep their relative order in the
+    stable ascending sort of SmithWaterman.py:699, so per-pair results equal
+    those of any batch holding the pair.
+    """
+    mx, d, sps = O.serial_pair(q, t, p, mp=mp, cparams=cparams)
+    sp_t = [(int(s.blockX), int(s.blockY), int(s.valueX), int(s.valueY), np.float32(s.score).view(np.uint32),
+             np.float32(s.maxScore).view(np.uint32), np.float32(s.posScore).view(np.uint32)) for s in sps]
+    walks = []
+    if sps:
+        ws = O.walk_c(d.reshape(1, 1, len(q), len(t)).copy(), sps, [q], [t], [qid], [tid], p.minimum)
+        walks = [(w.shown, w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches,
+                  np.float32(w.score).view(np.uint32), w.sequence_match if w.shown else "",
+                  w.alignment if w.shown else "", w.target_match if w.shown else "") for w in ws]
+    return np.float32(mx), sp_t, walks
+
+
+def compare_pairs(O, queries, targets, pq, pt, p, res, qids=None, tids=None, threads=16, mss=None):
+    """Bit-exact parity of a HIP result over an explicit pair list against the
+    serial oracle run pair by pair (in a thread pool; the C restatement
+    releases the interpreter lock).  Returns the number of pairs with
+    starting points."""
+    import concurrent.futures as cf
+    qids = qids or ["q%d" % i for i in range(len(queries))]
+    tids = tids or ["t%d" % i for i in range(len(targets))]
+    n = len(pq)
+    hip_sp, hip_w = {}, {}
+    for s in res.starting_points:
+        hip_sp.setdefault((int(s["sequence"]), int(s["target"])), []).append(
+            (int(s["blockX"]), int(s["blockY"]), int(s["valueX"]), int(s["valueY"]),
+             np.float32(s["score"]).view(np.uint32), np.float32(s["maxScore"]).view(np.uint32),
+             np.float32(s["posScore"]).view(np.uint32)))
+    for h in res.hits:
+        qi, ti = int(h["query"]), int(h["target"])
+        shown = bool(h["shown"])
+        sm = al = tm = ""
+        if shown:
+            sm, al, tm = decode_ops(res.ops, int(h["ops_offset"]), int(h["length"]), queries[qi], targets[ti],
+                                    int(h["s_start"]), int(h["t_start"]))
+        hip_w.setdefault((qi, ti), []).append(
+            (shown, int(h["s_start"]), int(h["s_end"]), int(h["t_start"]), int(h["t_end"]), int(h["matches"]),
+             int(h["mismatches"]), np.float32(h["score"]).view(np.uint32), sm, al, tm))
+
+    cparams = O.CParams(p)
+
+    def one(i):
+        qi, ti = int(pq[i]), int(pt[i])
+        mp = None if mss is None else float(mss[i])
+        return i, oracle_pair(O, queries[qi], targets[ti], qids[qi], tids[ti], p, mp, cparams)
+
+    with_sp = 0
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        for i, (mx, sp_t, walks) in ex.map(one, range(n)):
+            key = (int(pq[i]), int(pt[i]))
+            assert np.float32(res.maxima[i]).view(np.uint32) == mx.view(np.uint32), \
+                "maximum differs for pair {}: {} vs {}".format(key, res.maxima[i], mx)
+            assert hip_sp.get(key, []) == sp_t, "starting points differ for pair {}".format(key)
+            assert hip_w.get(key, []) == walks, "walk differs for pair {}:\n{}\n{}".format(
+                key, hip_w.get(key, [])[:3], walks[:3])
+            with_sp += bool(sp_t)
+    assert sum(len(v) for v in hip_sp.values()) == len(res.starting_points)
+    return with_sp

@@ -225,3 +225,45 @@ def test_protein_blosum_restatements(oracle):
             mx, d, _ = O.serial_pair(q, t, p)
             assert mx == br.maxima[qi, ti]
             assert np.array_equal(d, br.directions[qi, ti, :len(q), :len(t)])
+
+
+def _fake_result(O, qs, ts, p):
+    """The schedule-faithful oracle's batch in the shape of a HIP result (maxima,
+    36-byte records, hits with op codes), to exercise the per-pair checker."""
+    from types import SimpleNamespace
+    from pypaswas_amd._native import HIT_DTYPE, SP_DTYPE
+    br = O.sched_batch(qs, ts, p)
+    ws = O.walk_c(br.directions.copy(), br.starting_points, qs, ts, ["q%d" % i for i in range(len(qs))],
+                  ["t%d" % i for i in range(len(ts))], p.minimum)
+    sps = np.zeros(len(br.starting_points), SP_DTYPE)
+    for k, s in enumerate(br.starting_points):
+        sps[k] = (s.sequence, s.target, s.blockX, s.blockY, s.valueX, s.valueY, s.score, s.maxScore, s.posScore)
+    hits = np.zeros(len(ws), HIT_DTYPE)
+    ops = bytearray()
+    for k, w in enumerate(ws):
+        off = len(ops)
+        if w.shown:
+            for a, b in reversed(list(zip(w.sequence_match, w.target_match))):
+                ops += b"L" if b == "-" else (b"U" if a == "-" else b"D")
+        hits[k] = (w.qi, w.ti, int(w.shown), w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches,
+                   len(w.alignment) if w.shown else 0, w.score, 0, off)
+    return SimpleNamespace(maxima=br.maxima.reshape(-1), starting_points=sps, hits=hits, ops=bytes(ops))
+
+
+@pytest.mark.parametrize("affine", [False, True])
+def test_per_pair_checker_accepts_the_batch_and_rejects_a_change(oracle, affine):
+    """tests._parity.compare_pairs (serial restatement pair by pair) accepts the
+    schedule-faithful batch and notices a changed walk."""
+    from tests._parity import compare_pairs
+    O = oracle
+    rng = np.random.default_rng(23 + affine)
+    qs, ts = planted_set(rng, 5, 4, (30, 120), (20, 70), DNA, plant=1.0, sub=0.05)
+    p = dna_p(O, affine, minimum=6.0, lower_limit=0.7, filter_factor=0.0)
+    res = _fake_result(O, qs, ts, p)
+    pq = np.repeat(np.arange(5), 4)
+    pt = np.tile(np.arange(4), 5)
+    assert compare_pairs(O, qs, ts, pq, pt, p, res, threads=4) > 0
+    k = int(np.flatnonzero(res.hits["shown"])[0])
+    res.hits["t_end"][k] += 1
+    with pytest.raises(AssertionError):
+        compare_pairs(O, qs, ts, pq, pt, p, res, threads=4)
