@@ -10,6 +10,7 @@
 #ifndef PYPASWAS_HIP_H
 #define PYPASWAS_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -173,6 +174,27 @@ int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const 
                    const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
                    const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
                    double *ms_out);
+
+/* Device memory pw_score_pairs_noalloc may use, per GPU (64 MiB). */
+#define PW_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* pw_score_pairs without any allocation: no counterpart in pyPaSWAS.  For
+ * callers that cannot allocate device memory (code that must not call
+ * hipMalloc, such as tight serving loops or harnesses that forbid it).  Same
+ * arguments, results and errors as pw_score_pairs; the plan is built in host
+ * memory and every device array of the call (the encoded sequences, tables and
+ * work list, control block, maxima, the split pairs' boundary rows and the
+ * kernels' per-workgroup scratch) lives in one static device array of
+ * PW_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends; no
+ * device or pinned memory is allocated or freed.  A job that needs more than
+ * that (inputs plus one workgroup's scratch per kernel) is rejected with
+ * PW_EINVAL; persistent grids are lowered to fit, which changes speed, never
+ * the maxima.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (from two contexts or threads) race and must not be made. */
+int pw_score_pairs_noalloc(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
+                           const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
+                           const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
+                           double *ms_out);
 
 /* Autotuner.  Times the scoring kernels on synthetic pairs for the gap model
  * of the current scoring scheme: the wave-per-pair tile kernel with 1, 2, 4

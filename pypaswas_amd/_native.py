@@ -24,10 +24,13 @@ PW_EINVAL = -1
 PW_ENOMEM = -2
 PW_EDEVICE = -3
 
+# device memory of pw_score_pairs_noalloc per GPU (PW_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
            "pw_execute", "pw_release_plan", "pw_plan_info", "pw_execute_graph", "pw_set_limits", "pw_set_routing", "pw_score_pairs",
-           "pw_autotune", "pw_get_tuning", "pw_set_tuning", "pw_tune_plan", "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
+           "pw_score_pairs_noalloc", "pw_autotune", "pw_get_tuning", "pw_set_tuning", "pw_tune_plan", "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
            "pw_server_stats", "pw_server_destroy", "pw_free_result", "pw_last_error")
 
 
@@ -96,6 +99,7 @@ def load() -> ctypes.CDLL:
         L.pw_set_routing.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
+        L.pw_score_pairs_noalloc.argtypes = L.pw_score_pairs.argtypes
         L.pw_free_result.argtypes = [vp, ctypes.POINTER(ResultRec)]
         L.pw_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_get_tuning.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.POINTER(ctypes.c_int32),
@@ -362,6 +366,21 @@ class Device:
                                      _ptr(targets.length), len(targets.length), _ptr(pq), _ptr(pt), len(pq),
                                      int(mode), _ptr(out), ctypes.byref(ms))
         self._check(rc, "pw_score_pairs")
+        return out, float(ms.value)
+
+    def score_pairs_noalloc(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
+                            mode: int = 1):
+        """score_pairs through pw_score_pairs_noalloc: no device memory allocated, the
+        library's static scratch of NOALLOC_SCRATCH_BYTES per GPU instead."""
+        pq = np.ascontiguousarray(pq, dtype=np.int32)
+        pt = np.ascontiguousarray(pt, dtype=np.int32)
+        out = np.zeros(len(pq), dtype=np.float32)
+        ms = ctypes.c_double()
+        rc = self.lib.pw_score_pairs_noalloc(self.ctx, _ptr(queries.data), _ptr(queries.off), _ptr(queries.length),
+                                             len(queries.length), _ptr(targets.data), _ptr(targets.off),
+                                             _ptr(targets.length), len(targets.length), _ptr(pq), _ptr(pt), len(pq),
+                                             int(mode), _ptr(out), ctypes.byref(ms))
+        self._check(rc, "pw_score_pairs_noalloc")
         return out, float(ms.value)
 
 

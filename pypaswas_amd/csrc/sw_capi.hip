@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -185,6 +186,10 @@ struct pw_ctx {
     double route_share = 0.25;    // pw_set_routing: longest lane-per-pair job (share of cells per lane)
     int route_dir_rmax = 0;       // pw_set_routing: residues per lane of the direction pass (0: by plan)
     std::vector<HostResult *> result_pool;  // freed results kept for reuse
+    // this device's copy of pw_static_scratch (hipGetSymbolAddress at pw_init),
+    // the only device memory pw_score_pairs_noalloc uses
+    void *static_scratch = nullptr;
+    std::string static_err;
 };
 
 static thread_local std::string g_init_err;
@@ -452,6 +457,13 @@ void destroy_lane(Lane &L) {
 
 }  // namespace
 
+// Device memory of pw_score_pairs_noalloc: a file-scope array that each GPU
+// the library runs on holds from module load until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (on the host it is
+// only a stand-in for the symbol); pw_init looks up the device address of this
+// GPU's copy and keeps it in the context.
+static __device__ __align__(256) unsigned char pw_static_scratch[PW_NOALLOC_SCRATCH_BYTES];
+
 extern "C" int pw_init(int device, pw_ctx **out) {
     if (!out) {
         g_init_err = "pw_init: out is NULL";
@@ -489,6 +501,13 @@ extern "C" int pw_init(int device, pw_ctx **out) {
         destroy_lane(ctx->lane);
         delete ctx;
         return PW_EDEVICE;
+    }
+    // with this device current: the address of its copy of the static scratch
+    e = hipGetSymbolAddress(&ctx->static_scratch, HIP_SYMBOL(pw_static_scratch));
+    if (e != hipSuccess) {
+        ctx->static_scratch = nullptr;
+        ctx->static_err = std::string("hipGetSymbolAddress(pw_static_scratch): ") + hipGetErrorString(e);
+        (void)hipGetLastError();
     }
     *out = ctx;
     return PW_OK;
@@ -2531,6 +2550,136 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     }
     if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
     free_plan(ctx, P);
+    return rc;
+}
+
+namespace {
+
+// Places the device arrays of a score-only execution of P in the static
+// scratch at base (cap bytes, 256-byte aligned parts): the plan's inputs, the
+// control block, the maxima, the split pairs' boundary rows and progress
+// counters, and the per-workgroup scratch of the scoring kernels.  Persistent
+// grids are lowered until the per-workgroup scratch fits (workers pull their
+// items from device counters, and a split pair's stripes or a split job's
+// chunks are dequeued in order, so any grid of at least one workgroup gives
+// the same maxima).  Returns PW_EINVAL when the inputs alone, or one
+// workgroup's scratch per kernel, do not fit.
+int noalloc_layout(pw_ctx *ctx, pw_plan *P, Workspace &W, Geometry &g, char *base, size_t cap) {
+    const Encoded &E = P->E;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    struct Part {
+        DevBuf *d;
+        size_t bytes;
+    };
+    const Part parts[] = {{&P->res, E.codes.size()},
+                          {&P->seqs, E.refs.size() * sizeof(SeqRef)},
+                          {&P->tables, E.tables.size() * sizeof(float)},
+                          {&P->d_items, P->items.size() * sizeof(ScoreItem)},
+                          {&P->d_trans, P->trans.size()},
+                          {&P->d_gitems, P->gitems.size() * sizeof(GroupItem)},
+                          {&P->d_gents, P->gents.size() * sizeof(GroupEntry)},
+                          {&P->d_links, (size_t)P->score_links * sizeof(float)},
+                          {&P->d_prog, (size_t)P->score_prog * sizeof(unsigned)},
+                          {&W.ctl, sizeof(Ctl)},
+                          {&W.maxima, (size_t)P->n_pairs * sizeof(float)}};
+    size_t used = 0;
+    for (const Part &q : parts) {
+        q.d->p = base + used;
+        q.d->cap = q.bytes;
+        used += up(q.bytes);
+    }
+    int gmax = 0;
+    for (int w = 0; w < NB_KINDS; ++w) gmax = std::max(gmax, g.group_grid[w]);
+    const size_t gper = up((size_t)g.gscr_stride * sizeof(float2)), tper = up((size_t)g.tscr_stride * sizeof(float));
+    const size_t fixed = used + (gmax > 0 ? gper : 0) + (g.tile_grid > 0 ? tper : 0);
+    if (fixed > cap) {
+        ctx->err = "pw_score_pairs_noalloc: the job needs at least " + std::to_string(fixed) +
+                   " bytes of device memory, more than the static scratch of " + std::to_string(cap) + " bytes";
+        return PW_EINVAL;
+    }
+    // what is left is shared by the two scratch arrays in proportion to the
+    // grids the plan asked for
+    const size_t rest = cap - used;
+    const double want_g = gmax > 0 ? (double)gmax * gper : 0.0, want_t = g.tile_grid > 0 ? (double)g.tile_grid * tper : 0.0;
+    int ng = 0, nt = 0;
+    if (gmax > 0) {
+        const size_t share = want_t > 0 ? (size_t)((double)rest * want_g / (want_g + want_t)) : rest;
+        ng = (int)std::max<size_t>(1, std::min<size_t>((size_t)gmax, share / gper));
+    }
+    if (g.tile_grid > 0) {
+        const size_t left = rest - (size_t)ng * (gmax > 0 ? gper : 0);
+        nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)g.tile_grid, left / tper));
+    }
+    for (int w = 0; w < NB_KINDS; ++w) g.group_grid[w] = std::min(g.group_grid[w], ng);
+    g.tile_grid = std::min(g.tile_grid, nt);
+    W.gscratch.p = ng > 0 ? base + used : nullptr;
+    W.gscratch.cap = (size_t)ng * (ng > 0 ? gper : 0);
+    used += W.gscratch.cap;
+    W.tscratch.p = nt > 0 ? base + used : nullptr;
+    W.tscratch.cap = (size_t)nt * (nt > 0 ? tper : 0);
+    used += W.tscratch.cap;
+    return used <= cap ? PW_OK : PW_EINVAL;
+}
+
+}  // namespace
+
+extern "C" int pw_score_pairs_noalloc(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen,
+                                      int32_t nq, const uint8_t *tres, const int64_t *toff, const int32_t *tlen,
+                                      int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode,
+                                      float *maxima_out, double *ms_out) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (nq < 0 || nt < 0 || (n_pairs > 0 && (!pq || !pt || !maxima_out))) {
+        ctx->err = "pw_score_pairs_noalloc: missing argument";
+        return PW_EINVAL;
+    }
+    if (!ctx->static_scratch) {
+        ctx->err = "pw_score_pairs_noalloc: no static scratch on this device (" + ctx->static_err + ")";
+        return PW_EDEVICE;
+    }
+    // the plan is built in host memory only; its device arrays are views of
+    // the static scratch, never released to the context's buffer cache
+    std::unique_ptr<pw_plan> P(new pw_plan());
+    rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P.get(), qres, qoff, qlen, nq, tres, toff, tlen, nt,
+                   pq, pt, n_pairs, nullptr, nullptr, mode);
+    if (rc) return rc;
+    Workspace W;
+    Geometry g = geometry(ctx, P.get());
+    rc = noalloc_layout(ctx, P.get(), W, g, (char *)ctx->static_scratch, PW_NOALLOC_SCRATCH_BYTES);
+    if (rc) return rc;
+    Lane &L = ctx->lane;
+    const Encoded &E = P->E;
+    const std::pair<DevBuf *, const void *> inputs[] = {
+        {&P->res, E.codes.data()},        {&P->seqs, E.refs.data()},      {&P->tables, E.tables.data()},
+        {&P->d_items, P->items.data()},   {&P->d_trans, P->trans.data()}, {&P->d_gitems, P->gitems.data()},
+        {&P->d_gents, P->gents.data()}};
+    for (const auto &in : inputs) {
+        if (!in.first->cap) continue;
+        const hipError_t ce = hipMemcpyAsync(in.first->p, in.second, in.first->cap, hipMemcpyHostToDevice, L.s);
+        if (ce != hipSuccess) {
+            ctx->err = std::string("hipMemcpyAsync inputs: ") + hipGetErrorString(ce);
+            rc = PW_EDEVICE;
+            break;
+        }
+    }
+    if (!rc) rc = launch_clears(ctx, P.get(), W, L);
+    if (!rc) rc = launch_score(ctx, P.get(), W, L, g, true, nullptr);
+    if (!rc && n_pairs > 0) {
+        hipError_t e = hipMemcpyAsync(maxima_out, W.maxima.p, (size_t)n_pairs * sizeof(float), hipMemcpyDeviceToHost,
+                                      L.s);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipMemcpyAsync maxima: ") + hipGetErrorString(e);
+            rc = PW_EDEVICE;
+        }
+    }
+    // the host copies of the inputs (in P) and the scratch must outlive the work
+    hipError_t e = hipStreamSynchronize(L.s);
+    if (L.s2 != L.s) (void)hipStreamSynchronize(L.s2);
+    if (!rc && e != hipSuccess) {
+        ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+        rc = PW_EDEVICE;
+    }
+    if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
     return rc;
 }
 
