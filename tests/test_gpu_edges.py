@@ -137,18 +137,31 @@ def test_gate_passes_nothing(dev, oracle, mode):
     assert len(res.starting_points) == 0 and len(res.hits) == 0 and res.n_traceback_pairs == 0
 
 
+@pytest.mark.parametrize("affine", [False, True])
 @pytest.mark.parametrize("rmax", [2, 4, 8])
-def test_direction_residues_per_lane(oracle, rmax):
+def test_direction_residues_per_lane(oracle, rmax, affine):
     """The direction pass with its residues per lane capped at 2, 4 and 8
-    (pw_set_routing), on protein queries of 40 to 700 residues: one- and
-    multi-stripe pairs of every stripe height the cap allows."""
+    (pw_set_routing), linear and affine, on queries of 40 to 700 residues:
+    one- and multi-stripe pairs of every stripe height the cap allows.  Under
+    the cap 8 a query of 257-384 residues runs with 6 residues per lane; the
+    test checks that such a query has a pair passing the traceback gate."""
     from pypaswas_amd._native import Device
     from tests._parity import AMINO
-    print("direction residues cap", rmax, flush=True)
-    rng = np.random.default_rng(zlib.crc32(b"dir-residues"))
-    qs, ts = planted_set(rng, 8, 4, (40, 700), (60, 400), AMINO, plant=0.9, sub=0.3, indel=0.03)
-    p = oracle.Params(matrix=oracle.BLOSUM62, gap=-11.0, gap_ext=-1.0, lower_limit=0.8, minimum=25.0,
-                      filter_factor=0.05, highest=float(max(max(oracle.BLOSUM62))), dna=False)
+    print("direction residues cap", rmax, "affine", affine, flush=True)
+    rng = np.random.default_rng(zlib.crc32(b"dir-residues") + affine)
+    if affine:
+        qs, ts = planted_set(rng, 8, 4, (40, 700), (60, 400), AMINO, plant=0.9, sub=0.3, indel=0.03)
+        p = oracle.Params(matrix=oracle.BLOSUM62, gap=-11.0, gap_ext=-1.0, lower_limit=0.8, minimum=25.0,
+                          filter_factor=0.05, highest=float(max(max(oracle.BLOSUM62))), dna=False)
+    else:
+        qs, ts = planted_set(rng, 8, 4, (40, 700), (60, 400), DNA, plant=0.9, sub=0.05, indel=0.02)
+        p = dna_params(oracle, False, minimum=10.0, ff=0.05)
+        p.lower_limit = 0.8
+    # a 300-residue query carrying an exact copy of target 0's start: its pairs
+    # with target 0 pass the gate and run with 6 residues per lane under the cap 8
+    alpha = AMINO if affine else DNA
+    piece = ts[0][:100]
+    qs[0] = rand_seq(rng, 300 - len(piece), alpha) + piece
     d = Device(0)
     try:
         d.set_routing(-1, 0.0, rmax)
@@ -157,5 +170,10 @@ def test_direction_residues_per_lane(oracle, rmax):
         res = hip_all_vs_all(d, oracle, qs, ts, p, mode=1)
         compare(oracle, qs, ts, p, res)
         assert res.n_traceback_pairs > 0
+        mx = res.maxima.reshape(len(qs), len(ts))
+        mss = np.array([[oracle.min_start_score(p, len(q), len(t)) for t in ts] for q in qs], dtype=np.float32)
+        gated = (mx >= p.minimum) & (mx >= mss)
+        six = [i for i, q in enumerate(qs) if 256 < len(q) <= 384 and gated[i].any()]
+        assert six, "no gated pair with a 257-384-residue query"
     finally:
         d.close()
