@@ -168,6 +168,17 @@ int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t candidates_p
  * or 8 (0: chosen per plan so that the pairs give enough work units). */
 int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share, int32_t dir_residues);
 
+/* Bound of the device-side waits of the stripe and chunk hand-over of split
+ * pairs and split lane-per-pair jobs (no counterpart in pyPaSWAS, whose
+ * traceback spins on semaphores without a bound, default_main_cpu.cl:457-483):
+ * a wave that has polled a hand-over counter `polls` times gives up, the
+ * execution's other waves leave their items, and the call returns PW_EDEVICE
+ * with its results discarded, instead of the GPU hanging on a scheduling
+ * mistake.  polls < 0 restores the default (2^22 polls, seconds: far beyond
+ * any working hand-over); 0 gives up on the first wait that is not already
+ * satisfied (for tests of the error path).  Applies to later calls. */
+int pw_set_wait_limit(pw_ctx *ctx, int64_t polls);
+
 /* Score pass only: pair maxima, the quantity the traceback gate of
  * default_main_cpu.cl:408-410 reads.  maxima_out is [n_pairs]. */
 int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,

@@ -29,7 +29,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
-           "pw_execute", "pw_release_plan", "pw_plan_info", "pw_execute_graph", "pw_set_limits", "pw_set_routing", "pw_score_pairs",
+           "pw_execute", "pw_release_plan", "pw_plan_info", "pw_execute_graph", "pw_set_limits", "pw_set_routing", "pw_set_wait_limit", "pw_score_pairs",
            "pw_score_pairs_noalloc", "pw_autotune", "pw_get_tuning", "pw_set_tuning", "pw_tune_plan", "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
            "pw_server_stats", "pw_server_destroy", "pw_free_result", "pw_last_error")
 
@@ -97,6 +97,7 @@ def load() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.POINTER(ResultRec))]
         L.pw_set_limits.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
         L.pw_set_routing.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]
+        L.pw_set_wait_limit.argtypes = [vp, ctypes.c_int64]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
         L.pw_score_pairs_noalloc.argtypes = L.pw_score_pairs.argtypes
@@ -354,6 +355,10 @@ class Device:
         """Routing rules of plans prepared afterwards (pw_set_routing); defaults restore them."""
         self._check(self.lib.pw_set_routing(self.ctx, int(orient), float(group_max_share), int(dir_residues)),
                     "pw_set_routing")
+
+    def set_wait_limit(self, polls: int = -1):
+        """Polls before a device hand-over wait gives up (pw_set_wait_limit); -1 restores the default."""
+        self._check(self.lib.pw_set_wait_limit(self.ctx, int(polls)), "pw_set_wait_limit")
 
     def score_pairs(self, queries: PackedSeqs, targets: PackedSeqs, pq: np.ndarray, pt: np.ndarray,
                     mode: int = 1):

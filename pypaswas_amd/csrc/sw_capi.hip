@@ -190,6 +190,7 @@ struct pw_ctx {
     // the only device memory pw_score_pairs_noalloc uses
     void *static_scratch = nullptr;
     std::string static_err;
+    unsigned wait_polls = WAIT_POLLS;  // pw_set_wait_limit: polls before a hand-over wait gives up
 };
 
 static thread_local std::string g_init_err;
@@ -606,6 +607,16 @@ extern "C" int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t c
     // the shared work areas follow the new rules from the next call on
     ctx->ws.want_dir = ctx->ws.want_cand = ctx->ws.want_ops = 0;
     ctx->ws.cand0 = 0;
+    return PW_OK;
+}
+
+extern "C" int pw_set_wait_limit(pw_ctx *ctx, int64_t polls) {
+    if (!ctx) return PW_EINVAL;
+    if (polls > (int64_t)UINT32_MAX) {
+        ctx->err = "pw_set_wait_limit: at most 2^32 - 1 polls";
+        return PW_EINVAL;
+    }
+    ctx->wait_polls = polls < 0 ? WAIT_POLLS : (unsigned)polls;
     return PW_OK;
 }
 
@@ -1419,7 +1430,9 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             unsigned *q = ctl->queue + 16 * (1 + w);
             float2 *lk = (float2 *)P->d_links.p;
             unsigned *pg = (unsigned *)P->d_prog.p;
-            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &gp, &e, &scr, &ss, &q, &mx, &lk, &pg};
+            unsigned *ab = &ctl->abort_wait;
+            unsigned wl = ctx->wait_polls;
+            void *args[] = {&res, &seqs, &items, &ni, &ents, &tabs, &kp, &gp, &e, &scr, &ss, &q, &mx, &lk, &pg, &ab, &wl};
             HIPCHK(ctx, hipLaunchKernel(kern, dim3(g.group_grid[w]), dim3(64), args, group_lds(P, w), gs));
             if (launches) *launches += 1;
         }
@@ -1437,7 +1450,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
                 hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, L.s, res, seqs, items + b.first, (int)n,
                                    trans + b.first, tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
                                    (float *)W.tscratch.p, g.tscr_stride, (unsigned *)nullptr, mx, (float *)nullptr,
-                                   (unsigned *)nullptr);
+                                   (unsigned *)nullptr, &ctl->abort_wait, ctx->wait_polls);
                 HIPCHK(ctx, hipGetLastError());
                 if (launches) *launches += 1;
             }
@@ -1448,7 +1461,8 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             hipStream_t st = gstreams[si];
             hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
                                tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
-                               ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p);
+                               ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p, &ctl->abort_wait,
+                               ctx->wait_polls);
             HIPCHK(ctx, hipGetLastError());
             if (launches) *launches += 1;
         }
@@ -1522,7 +1536,7 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
     hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP, true), L.s, res, seqs, tasks,
                        (const DirUnit *)W.units.p, ctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
                        P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
-                       (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
+                       (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap, ctx->wait_polls);
     HIPCHK(ctx, hipGetLastError());
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
     int rc = launch_start_selection(ctx, P, W, L, g, ctl);
@@ -1775,7 +1789,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP, true), L.s, res, seqs, tasks,
                            (const DirUnit *)W.units.p, dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
                            P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
-                           (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap);
+                           (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap, ctx->wait_polls);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
         int rc0 = launch_start_selection(ctx, P, W, L, g, dctl);
@@ -1949,9 +1963,10 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostR
 
 // Launch parameters a captured graph depends on.
 std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, const Lane &L, const Geometry &g,
-                                      double minimum) {
+                                      double minimum, unsigned wait_polls) {
     std::vector<uint64_t> s;
     auto add = [&](uint64_t v) { s.push_back(v); };
+    add(wait_polls);
     const DevBuf *bufs[] = {&W.ctl,      &W.maxima,   &W.gated,    &W.tasks,    &W.touts,    &W.dirs,
                             &W.cands,    &W.spslot,   &W.keys,     &W.out_sps,  &W.out_hits, &W.ops,
                             &W.gscratch, &W.tscratch, &W.dscratch, &W.units,    &W.dlinks,   &W.dprog,
@@ -2019,7 +2034,7 @@ int execute_launch(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimu
     int rc = size_workspace(ctx, P, W, g);
     if (rc) return rc;
     if (use_graph) {
-        std::vector<uint64_t> sig = graph_signature(P, W, L, g, minimum);
+        std::vector<uint64_t> sig = graph_signature(P, W, L, g, minimum, ctx->wait_polls);
         if (!P->graph || sig != P->graph_sig) {
             if (P->graph) {
                 (void)hipGraphExecDestroy(P->graph);
@@ -2076,6 +2091,11 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
         F.ms_walk = elapsed(L.ev[4], L.ev[5]);
     }
     const Ctl c = *(const Ctl *)W.h_ctl.p;
+    if (c.abort_wait) {
+        ctx->err = "a stripe or chunk hand-over wait on the device gave up after " + std::to_string(ctx->wait_polls) +
+                   " polls; the execution's results are discarded";
+        return PW_EDEVICE;
+    }
     F.ng = c.ngated;
     if (F.ng < 0 || F.ng > P->n_pairs) {
         ctx->err = "gate count out of range";
@@ -2509,6 +2529,24 @@ extern "C" int pw_release_plan(pw_ctx *ctx, pw_plan *plan) {
     return PW_OK;
 }
 
+namespace {
+
+// After a score-only execution on L.s has finished: PW_EDEVICE when one of its
+// waves gave up on a hand-over wait (the maxima are then discarded).
+int check_wait_abort(pw_ctx *ctx, const Workspace &W, Lane &L) {
+    unsigned ab = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&ab, &((const Ctl *)W.ctl.p)->abort_wait, sizeof ab, hipMemcpyDeviceToHost, L.s));
+    HIPCHK(ctx, hipStreamSynchronize(L.s));
+    if (ab) {
+        ctx->err = "a stripe or chunk hand-over wait on the device gave up after " + std::to_string(ctx->wait_polls) +
+                   " polls; the maxima are discarded";
+        return PW_EDEVICE;
+    }
+    return PW_OK;
+}
+
+}  // namespace
+
 extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen, int32_t nq,
                               const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt,
                               const int32_t *pq, const int32_t *pt, int64_t n_pairs, int32_t mode, float *maxima_out,
@@ -2548,6 +2586,7 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
             rc = PW_EDEVICE;
         }
     }
+    if (!rc) rc = check_wait_abort(ctx, W, L);
     if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
     free_plan(ctx, P);
     return rc;
@@ -2679,6 +2718,7 @@ extern "C" int pw_score_pairs_noalloc(pw_ctx *ctx, const uint8_t *qres, const in
         ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
         rc = PW_EDEVICE;
     }
+    if (!rc) rc = check_wait_abort(ctx, W, L);
     if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
     return rc;
 }

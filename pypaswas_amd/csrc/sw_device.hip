@@ -70,34 +70,42 @@ __device__ __forceinline__ void store_scr(float *p, float v) {
 // the new column count; the consumer wave spins (with s_sleep) on the count
 // and only then issues its row loads.  Waits end: the producer stripe was
 // dequeued earlier by a running wave, and it publishes every chunk up to the
-// last column.
+// last column.  Every wait is bounded all the same (wait_rows), so that a
+// planning or routing mistake ends in a device error reported by the host
+// instead of a hung GPU: once one wave gave up, every later wait of the
+// execution gives up at its first unsatisfied poll and every kernel drains.
 __device__ __forceinline__ void publish_progress(unsigned *p, unsigned v) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// As wait_progress, returning the count it saw (at least need).
-__device__ __forceinline__ unsigned wait_rows(const unsigned *p, unsigned need) {
-    for (;;) {
-        const unsigned v = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (v >= need) {
-            __builtin_amdgcn_s_waitcnt(0);
-            return v;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
+__device__ __forceinline__ unsigned load_counter(const unsigned *p) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-__device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need) {
-    for (;;) {
-        const unsigned v = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (v >= need) break;
+// Waits until *p >= need and returns the count it saw.  After `limit` polls,
+// or at once when another wave already gave up, the wave sets *abort and
+// returns `need` as if the rows were there: the item then runs to its end on
+// stale boundary values (addresses never depend on values, so every access
+// stays in bounds and the work stays bounded), and the host discards the
+// execution when it reads the flag.
+__device__ __forceinline__ unsigned wait_rows(const unsigned *p, unsigned need, unsigned *abort, unsigned limit) {
+    unsigned v = load_counter(p);
+    for (unsigned k = 0; v < need; ++k) {
+        if (k >= limit || load_counter(abort) != 0u) {
+            if ((threadIdx.x & 63) == 0) __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return need;
+        }
         __builtin_amdgcn_s_sleep(2);
+        v = load_counter(p);
     }
     __builtin_amdgcn_s_waitcnt(0);
+    return v;
+}
+
+__device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need, unsigned *abort, unsigned limit) {
+    (void)wait_rows(p, need, abort, limit);
 }
 
 // Columns of a stripe boundary row handed over at a time (through LDS, then
@@ -125,7 +133,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                       const float *subT, int lkp, uint8_t *bwin, float4 *stg, float *__restrict__ scr, float G,
                       float E, int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
                       Cand *__restrict__ cands, int cand_cap, int s_first, int s_last, float *__restrict__ links,
-                      unsigned *__restrict__ prog) {
+                      unsigned *__restrict__ prog, unsigned *abort, unsigned wlimit) {
     const int lane = threadIdx.x & 63;
     const int rows = WAVE * R;
     const int nstripes = (alen_it + rows - 1) / rows;
@@ -217,7 +225,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             __syncthreads();
         };
         if (above) {
-            if (links && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(HANDOFF, blen_it));
+            if (links && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(HANDOFF, blen_it), abort, wlimit);
             stage_in(0);
         }
         // inputs of step 0: lane 0 reads column 0 of the row above; the other
@@ -257,7 +265,8 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             float4 inj = make_float4(0.0f, bJ, 0.0f, 0.0f);
             if (above && tn < nsteps) {
                 if ((tn & (HANDOFF - 1)) == 0) {
-                    if (links && tn < blen_it) wait_progress(prog + (s - 1), (unsigned)min(tn + HANDOFF, blen_it));
+                    if (links && tn < blen_it)
+                        wait_progress(prog + (s - 1), (unsigned)min(tn + HANDOFF, blen_it), abort, wlimit);
                     stage_in(tn);
                 }
                 inj = stg_in[tn & (HANDOFF - 1)];
@@ -453,12 +462,13 @@ template <bool AFF, int R>
 __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *seqs, const ScoreItem &it,
                                             const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float4 *stg,
                                             float *scr,
-                                            float G, float E, int fill, int trans, float *links, unsigned *prog) {
+                                            float G, float E, int fill, int trans, float *links, unsigned *prog,
+                                            unsigned *abort, unsigned wlimit) {
     const float *tab = lds_tab + (trans ? kp2 : 0);
     float *lk = it.rows_off >= 0 ? links + it.rows_off : nullptr;
     unsigned *pg = it.rows_off >= 0 ? prog + it.prog_off : nullptr;
     return tile<AFF, R, false>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, stg, scr, G, E,
-                               fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg);
+                               fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg, abort, wlimit);
 }
 
 // Scoring pass.  queue != nullptr: persistent workers pull items from a
@@ -475,7 +485,8 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
                                                    const float *__restrict__ tables, int kp, int lkp, float G,
                                                    float E, float *__restrict__ scratch, int64_t scr_stride,
                                                    unsigned *__restrict__ queue, float *__restrict__ maxima,
-                                                   float *__restrict__ links, unsigned *__restrict__ prog) {
+                                                   float *__restrict__ links, unsigned *__restrict__ prog,
+                                                   unsigned *abort, unsigned wlimit) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
@@ -499,10 +510,10 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
         const int tr = trans_flags ? trans_flags[item] : 0;
         float m;
         switch (it.R) {
-        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
-        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
-        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
-        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog); break;
+        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
         }
         m = wave_max(m);
         if (threadIdx.x == 0) {
@@ -550,7 +561,8 @@ template <bool AFF, int NB, bool SH>
 __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs, const GroupItem &it,
                             const GroupEntry *__restrict__ ents, const float *__restrict__ gtab, int kp,
                             float *prof, float2 *__restrict__ scr, float2 *__restrict__ links,
-                            unsigned *__restrict__ prog, float G, float E, int fill) {
+                            unsigned *__restrict__ prog, float G, float E, int fill, unsigned *abort,
+                            unsigned wlimit) {
     constexpr int NBP = NB + 4;
     const int lane = threadIdx.x & 63;
     const SeqRef C = seqs[it.c_seq];
@@ -603,7 +615,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
         int a_next = (active && 0 < Rw.len) ? (int)res[Rw.off] : fill;
         float2 lb_next = make_float2(0.0f, SH ? GmE : bI);
         if (left_in && it.rows > 0) {
-            if (from_link) avail = wait_rows(prog + it.prog_in, 1u);
+            if (from_link) avail = wait_rows(prog + it.prog_in, 1u, abort, wlimit);
             lb_next.x = load_scr(&src[lane].x);
             lb_next.y = load_scr(&src[lane].y);
         }
@@ -617,7 +629,7 @@ __device__ float group_tile(const uint8_t *__restrict__ res, const SeqRef *__res
             const float2 lb = lb_next;
             a_next = (active && r + 1 < Rw.len) ? (int)res[Rw.off + r + 1] : fill;
             if (left_in && r + 1 < it.rows) {
-                if ((unsigned)(r + 2) > avail) avail = wait_rows(prog + it.prog_in, (unsigned)(r + 2));
+                if ((unsigned)(r + 2) > avail) avail = wait_rows(prog + it.prog_in, (unsigned)(r + 2), abort, wlimit);
                 lb_next.x = load_scr(&src[(r + 1) * WAVE + lane].x);
                 lb_next.y = load_scr(&src[(r + 1) * WAVE + lane].y);
             } else if (SH) {
@@ -708,7 +720,8 @@ __global__ void __launch_bounds__(64, NB == 72 ? 3 : 1) group_kernel(const uint8
                                                    const float *__restrict__ tables, int kp, float G, float E,
                                                    float2 *__restrict__ scratch, int64_t scr_stride,
                                                    unsigned *__restrict__ queue, float *__restrict__ maxima,
-                                                   float2 *__restrict__ links, unsigned *__restrict__ prog) {
+                                                   float2 *__restrict__ links, unsigned *__restrict__ prog,
+                                                   unsigned *abort, unsigned wlimit) {
     extern __shared__ float lds[];
     float2 *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
@@ -725,7 +738,8 @@ __global__ void __launch_bounds__(64, NB == 72 ? 3 : 1) group_kernel(const uint8
         }
         if (item >= nitems) break;
         const GroupItem it = items[item];
-        const float m = group_tile<AFF, NB, SH>(res, seqs, it, ents, tables, kp, lds, scr, links, prog, G, E, fill);
+        const float m =
+            group_tile<AFF, NB, SH>(res, seqs, it, ents, tables, kp, lds, scr, links, prog, G, E, fill, abort, wlimit);
         if ((int)threadIdx.x < it.count) {
             const int pair = ents[it.entry_off + threadIdx.x].pair;
             // a split job's maximum: the maximum over its items, combined with
@@ -753,7 +767,7 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
                                                  int lkp, float G, float E, float *__restrict__ scratch,
                                                  int64_t scr_stride, uint8_t *__restrict__ dirs,
                                                  Cand *__restrict__ cands, float *__restrict__ links,
-                                                 unsigned *__restrict__ prog, int64_t units_cap) {
+                                                 unsigned *__restrict__ prog, int64_t units_cap, unsigned wlimit) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
@@ -763,6 +777,7 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
     const int fill = kp - 1;
     const int n = (int)min((int64_t)__hip_atomic_load(&ctl->units_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
                            units_cap);
+    unsigned *abort = &ctl->abort_wait;
     for (;;) {
         unsigned v = 0;
         if (threadIdx.x == 0) v = atomicAdd(&ctl->dir_queue, 1u);
@@ -780,11 +795,11 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
         float *lk = du.stripe < 0 ? nullptr : links + it.rows_off;
         unsigned *pg = du.stripe < 0 ? nullptr : prog + it.prog_off;
         switch (it.R) {
-        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        case 6: tile<AFF, 6, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
-        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg); break;
+        case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
+        case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
+        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
+        case 6: tile<AFF, 6, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
+        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
         }
     }
 }

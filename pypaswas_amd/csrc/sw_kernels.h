@@ -132,7 +132,15 @@ struct Ctl {
     unsigned long long dir_used, cand_used, ops_used, out_used;
     unsigned long long link_used, prog_used;  // split-pair areas of the direction pass
     long long units_used;                     // work units of the direction pass
+    unsigned abort_wait;  // set by a wave whose stripe or chunk hand-over wait gave up (see WAIT_POLLS)
 };
+
+// Polls a wave makes on a hand-over counter before it gives up, sets
+// Ctl::abort_wait and leaves its work item (the host then reports a device
+// error and discards the execution).  A poll is one device-scope load plus a
+// short sleep, about a microsecond, so the default bound is seconds, far
+// beyond any hand-over of a working schedule (a few microseconds per chunk).
+constexpr unsigned WAIT_POLLS = 1u << 22;
 
 // A candidate start: x carries two flags in its top bits, whether the cell
 // qualifies by its value (unmarked) and by its negated value (once marked)
