@@ -69,6 +69,8 @@ typedef struct {
     double ms_readback;                  /* host: reading tasks and records back (incl. copies) */
     double ms_tail;                      /* host: tail rounds of pairs that did not fit the work areas */
     double ms_assemble;                  /* host: records into pair order */
+    double ms_plan;                      /* host: encoding and work list of a one-shot call (pw_align_pairs) */
+    double ms_upload;                    /* host: staging and copy of the plan to the device (one-shot call) */
 } pw_result;
 
 /* SmithWatermanOcl._initialize_device (pyPaSWAS/Core/SmithWatermanOcl.py:116-129)

@@ -57,7 +57,8 @@ class ResultRec(ctypes.Structure):
                 ("sp_truncated", ctypes.c_int64), ("cells", ctypes.c_double), ("ms_score", ctypes.c_double),
                 ("ms_directions", ctypes.c_double), ("ms_traceback", ctypes.c_double), ("ms_walk", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("score_launches", ctypes.c_int64),
-                ("ms_readback", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_assemble", ctypes.c_double)]
+                ("ms_readback", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_assemble", ctypes.c_double),
+                ("ms_plan", ctypes.c_double), ("ms_upload", ctypes.c_double)]
 
 
 _lib = None
@@ -162,6 +163,8 @@ class AlignResult:
     ms_readback: float = 0.0
     ms_tail: float = 0.0
     ms_assemble: float = 0.0
+    ms_plan: float = 0.0
+    ms_upload: float = 0.0
 
 
 SP_DTYPE = np.dtype([("sequence", "<u4"), ("target", "<u4"), ("blockX", "<u4"), ("blockY", "<u4"),
@@ -285,7 +288,7 @@ class Device:
         res = AlignResult(maxima, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
                           float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
                           float(r.ms_walk), float(r.ms_total), int(r.score_launches), float(r.ms_readback),
-                          float(r.ms_tail), float(r.ms_assemble))
+                          float(r.ms_tail), float(r.ms_assemble), float(r.ms_plan), float(r.ms_upload))
         # with no view left holding it (an empty result), the record goes back here
         del hold
         return res

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -97,6 +98,8 @@ uint8_t *dir_ptr(const Workspace &W) { return W.dir_ext ? W.dir_ext : (uint8_t *
 
 }  // namespace
 
+struct PlanScratch;  // host buffers of the planner (defined with the planner)
+
 namespace {
 
 // Measured shape table (pw_autotune), one per gap model.  Buckets of the
@@ -130,6 +133,7 @@ struct PlanOpts {
     int orient = -1;  // column sequence of lane-per-pair jobs: 0 target, 1 query, -1 the cheaper one
     int dir_rmax = 0;  // > 0: largest residues per lane of the direction pass (2, 4 or 8)
     double grid_share = 1.0;  // persistent grids cover at most this share of the device's resident waves
+    PlanScratch *scratch = nullptr;  // reusable host buffers of the planner (the context's; may be null)
 };
 
 // Bucket k holds lengths up to 4/3 of its representative length rep[k]
@@ -191,6 +195,8 @@ struct pw_ctx {
     void *static_scratch = nullptr;
     std::string static_err;
     unsigned wait_polls = WAIT_POLLS;  // pw_set_wait_limit: polls before a hand-over wait gives up
+    PlanScratch *plan_scratch = nullptr;  // planner buffers reused across this context's plans
+    pw_plan *oneshot = nullptr;           // the plan of the one-shot calls, reused (its buffers keep their size)
 };
 
 static thread_local std::string g_init_err;
@@ -514,10 +520,15 @@ extern "C" int pw_init(int device, pw_ctx **out) {
     return PW_OK;
 }
 
+namespace {
+void free_context_plans(pw_ctx *ctx);
+}
+
 extern "C" int pw_destroy(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
     (void)hipSetDevice(ctx->device);
     destroy_lane(ctx->lane);
+    free_context_plans(ctx);
     free_workspace(ctx, ctx->ws);
     free_pinned(ctx->h_stage);
     release_dead(ctx);
@@ -835,6 +846,27 @@ void free_plan(pw_ctx *ctx, pw_plan *P) {
     delete P;
 }
 
+// Host threads for n elements: up to 16, at most one per min_per_thread elements.
+int host_threads(int64_t n, int64_t min_per_thread) {
+    const int64_t hw = std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)std::thread::hardware_concurrency()));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(hw, n / std::max<int64_t>(1, min_per_thread)));
+}
+
+// Runs f(lo, hi, k) over [0, n) in contiguous ranges, thread k of the
+// host_threads(n, min_per_thread) threads taking the k-th range.
+template <class F>
+void parallel_ranges(int64_t n, int64_t min_per_thread, F f) {
+    const int nth = host_threads(n, min_per_thread);
+    if (nth <= 1) {
+        f((int64_t)0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)nth);
+    for (int k = 0; k < nth; ++k) th.emplace_back([&f, n, nth, k] { f(n * k / nth, n * (k + 1) / nth, k); });
+    for (std::thread &t : th) t.join();
+}
+
 // Host half of pw_prepare: encodes the residues and builds the scoring work
 // list.  No device call.
 int plan_worklist(const PlanOpts &opt, pw_plan *P);
@@ -865,45 +897,240 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
     else P->mss.assign((size_t)n_pairs, 0.0f);
     if (same_id) P->same.assign(same_id, same_id + n_pairs);
     else P->same.assign((size_t)n_pairs, 0);
-    P->cells = 0;
-    P->max_t = 0;
-    P->dir_worst = 0;
-    P->dir_largest = 0;
-    P->units_worst = P->dlinks_worst = P->dprog_worst = 0;
-    // residues per lane of the direction pass: the largest of 8, 4, 2 for
-    // which the pairs, all gated, would give at least 4 work units per SIMD
-    // (more, shorter stripes for few long pairs: their waves run latency-bound)
-    P->dir_rmax = opt.dir_rmax > 0 ? opt.dir_rmax : 2;
-    for (int r : {8, 4}) {
-        if (opt.dir_rmax > 0) break;
-        int64_t units = 0;
-        for (int64_t i = 0; i < n_pairs; ++i) {
+    // per-pair sizes of the traceback stages for each cap of the direction
+    // pass's residues per lane (2, 4, 8), in one pass over the pairs on
+    // several host threads; the cap is then the largest of 8, 4, 2 for which
+    // the pairs, all gated, would give at least 4 work units per SIMD (more,
+    // shorter stripes for few long pairs: their waves run latency-bound)
+    struct Acc {
+        int64_t cells = 0, max_t = 0;
+        int64_t units[3] = {}, dir_worst[3] = {}, dir_largest[3] = {}, dlinks[3] = {}, dprog[3] = {};
+    };
+    const int caps[3] = {2, 4, 8};
+    std::vector<Acc> acc(16);  // per thread (parallel_ranges runs at most 16)
+    parallel_ranges(n_pairs, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
+        Acc a;
+        for (int64_t i = lo; i < hi; ++i) {
             const int ql = qlen[pq[i]], tl = tlen[pt[i]];
-            const int ns = split_stripes(ql, tl, dir_residues(ql, r));
-            units += ns > 0 ? ns : 1;
+            a.cells += (int64_t)ql * tl;
+            a.max_t = std::max<int64_t>(a.max_t, tl);
+            for (int c = 0; c < 3; ++c) {
+                const int R = dir_residues(ql, caps[c]);
+                const int64_t area = dir_area(ql, tl, R);
+                a.dir_worst[c] += area;
+                a.dir_largest[c] = std::max(a.dir_largest[c], area);
+                const int ns = split_stripes(ql, tl, R);
+                a.units[c] += ns > 0 ? ns : 1;
+                if (ns > 0) {
+                    a.dlinks[c] += (int64_t)(ns - 1) * link_row_floats(tl);
+                    a.dprog[c] += ns;
+                }
+            }
         }
-        if (units >= (int64_t)std::max(1, opt.num_cus) * 4 * 4) {
-            P->dir_rmax = r;
-            break;
+        acc[k] = a;
+    });
+    Acc tot;
+    for (const Acc &a : acc) {
+        tot.cells += a.cells;
+        tot.max_t = std::max(tot.max_t, a.max_t);
+        for (int c = 0; c < 3; ++c) {
+            tot.units[c] += a.units[c];
+            tot.dir_worst[c] += a.dir_worst[c];
+            tot.dir_largest[c] = std::max(tot.dir_largest[c], a.dir_largest[c]);
+            tot.dlinks[c] += a.dlinks[c];
+            tot.dprog[c] += a.dprog[c];
         }
     }
-    for (int64_t i = 0; i < n_pairs; ++i) {
-        const int ql = qlen[pq[i]], tl = tlen[pt[i]];
-        P->cells += (double)ql * (double)tl;
-        P->max_t = std::max<int64_t>(P->max_t, tl);
-        const int R = dir_residues(ql, P->dir_rmax);
-        const int64_t a = dir_area(ql, tl, R);
-        P->dir_worst += a;
-        P->dir_largest = std::max(P->dir_largest, a);
-        const int ns = split_stripes(ql, tl, R);
-        P->units_worst += ns > 0 ? ns : 1;
-        if (ns > 0) {
-            P->dlinks_worst += (int64_t)(ns - 1) * link_row_floats(tl);
-            P->dprog_worst += ns;
-        }
+    int ci = 0;
+    if (opt.dir_rmax > 0) {
+        ci = opt.dir_rmax >= 8 ? 2 : (opt.dir_rmax >= 4 ? 1 : 0);
+    } else {
+        for (int c : {2, 1})
+            if (tot.units[c] >= (int64_t)std::max(1, opt.num_cus) * 4 * 4) {
+                ci = c;
+                break;
+            }
     }
+    P->dir_rmax = opt.dir_rmax > 0 ? opt.dir_rmax : caps[ci];
+    P->cells = (double)tot.cells;  // an exact integer below 2^53
+    P->max_t = tot.max_t;
+    P->dir_worst = tot.dir_worst[ci];
+    P->dir_largest = tot.dir_largest[ci];
+    P->units_worst = tot.units[ci];
+    P->dlinks_worst = tot.dlinks[ci];
+    P->dprog_worst = tot.dprog[ci];
     P->opts = opt;
     return plan_worklist(opt, P);
+}
+
+// Lane-per-pair jobs of one orientation of a batched plan: for every column
+// sequence (the target for o = 0, the query for o = 1) its pairs in pair
+// order, the candidates among them stably sorted by row length, longest
+// first, cut into 64-lane jobs that are kept (or kept split into chunk
+// items) or sent to the tile kernel, and the cost estimate of the choice.
+// Per column c the arrays hold, from off[c]: the tile-kernel pairs (ntile[c]:
+// the non-candidates in pair order, then the pairs of rejected jobs in job
+// order), the kept pairs (ngrp[c], 64 per job) and, at the position of each
+// kept job's first pair, whether it is split.  Columns are independent and
+// run on several host threads.
+struct OrientPlan {
+    double cost = 0;
+    std::vector<int64_t> off, ntile, ngrp;
+    std::vector<int32_t> by_col, tile, grp;  // pair indices (n_pairs <= INT32_MAX)
+    std::vector<uint8_t> split;
+    std::vector<int> nb;
+};
+
+}  // namespace
+
+// Host buffers of the planner, kept by a context across plans so that their
+// pages are not faulted in again for every call.
+struct PlanScratch {
+    OrientPlan op[2];
+};
+
+namespace {
+
+// The one-shot plan and the planner buffers of a context being destroyed.
+void free_context_plans(pw_ctx *ctx) {
+    if (ctx->oneshot) free_plan(ctx, ctx->oneshot);
+    ctx->oneshot = nullptr;
+    delete ctx->plan_scratch;
+    ctx->plan_scratch = nullptr;
+}
+
+void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, OrientPlan &R) {
+    const int32_t nq = P->nq, nt = P->nt;
+    const int64_t n = P->n_pairs;
+    const int32_t *pq = P->pq.data(), *pt = P->pt.data(), *qlen = P->qlen.data(), *tlen = P->tlen.data();
+    const int32_t ncol = o ? nq : nt;
+    auto col_of = [&](int64_t i) { return o ? pq[i] : pt[i]; };
+    auto row_len = [&](int64_t i) { return o ? tlen[pt[i]] : qlen[pq[i]]; };
+    // pairs grouped by column, in pair order: per-thread column counts, then
+    // each thread fills its own slots
+    const int nth = host_threads(n, 1 << 16);  // the thread count parallel_ranges(n, 1 << 16, ...) uses
+    std::vector<int64_t> cnt((size_t)nth * ncol, 0);
+    parallel_ranges(n, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
+        int64_t *c = cnt.data() + (size_t)k * ncol;
+        for (int64_t i = lo; i < hi; ++i) ++c[col_of(i)];
+    });
+    R.off.assign((size_t)ncol + 1, 0);
+    std::vector<int64_t> pos((size_t)nth * ncol);
+    int64_t run = 0;
+    for (int32_t c = 0; c < ncol; ++c) {
+        R.off[c] = run;
+        for (int k = 0; k < nth; ++k) {
+            pos[(size_t)k * ncol + c] = run;
+            run += cnt[(size_t)k * ncol + c];
+        }
+    }
+    R.off[ncol] = run;
+    R.by_col.resize((size_t)n);
+    parallel_ranges(n, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
+        int64_t *ps = pos.data() + (size_t)k * ncol;
+        for (int64_t i = lo; i < hi; ++i) R.by_col[ps[col_of(i)]++] = (int32_t)i;
+    });
+    R.tile.resize((size_t)n);
+    R.grp.resize((size_t)n);
+    R.split.resize((size_t)n);
+    R.ntile.assign((size_t)ncol, 0);
+    R.ngrp.assign((size_t)ncol, 0);
+    R.nb.assign((size_t)ncol, 16);
+    std::vector<double> ccost((size_t)ncol, 0.0);
+    // columns in blocks, dynamically over the threads
+    std::atomic<int32_t> next_col{0};
+    const int cth = host_threads(n, 1 << 15);
+    std::vector<std::thread> th;
+    auto work = [&]() {
+        std::vector<int32_t> cand, sorted;
+        std::vector<int64_t> bucket;
+        for (;;) {
+            const int32_t c0 = next_col.fetch_add(16);
+            if (c0 >= ncol) break;
+            for (int32_t cs = c0; cs < std::min(ncol, c0 + 16); ++cs) {
+                const int64_t base = R.off[cs], m = R.off[cs + 1] - base;
+                if (m == 0) continue;
+                const int32_t *L = R.by_col.data() + base;
+                const int cl = o ? qlen[cs] : tlen[cs];
+                double cost = 0;
+                int64_t nti = 0, ngr = 0;
+                cand.clear();
+                int lmin = INT32_MAX, lmax = 0;
+                for (int64_t u = 0; u < m; ++u) {
+                    const int64_t i = L[u];
+                    const int rl = row_len(i);
+                    if (!opt.force_tile && rl <= GROUP_MAX_ROWS) {
+                        cand.push_back((int32_t)i);
+                        lmin = std::min(lmin, rl);
+                        lmax = std::max(lmax, rl);
+                    } else {
+                        cost += TILE_COST_FACTOR * (double)rl * cl;
+                        R.tile[base + nti++] = (int32_t)i;
+                    }
+                }
+                // stable sort by row length, longest first: counting sort over
+                // the column's length range when that is small
+                const int64_t nc = (int64_t)cand.size();
+                if (nc > 1) {
+                    const int64_t range = (int64_t)lmax - lmin + 1;
+                    if (range <= 4 * nc + 1024) {
+                        bucket.assign((size_t)range + 1, 0);
+                        for (int32_t i : cand) ++bucket[(size_t)(lmax - row_len(i)) + 1];
+                        for (int64_t b = 1; b <= range; ++b) bucket[b] += bucket[b - 1];
+                        sorted.resize((size_t)nc);
+                        for (int32_t i : cand) sorted[bucket[(size_t)(lmax - row_len(i))]++] = i;
+                        cand.swap(sorted);
+                    } else {
+                        std::stable_sort(cand.begin(), cand.end(),
+                                         [&](int32_t x, int32_t y) { return row_len(x) > row_len(y); });
+                    }
+                }
+                // keep a 64-lane job only if its lanes are busy for most of its
+                // rows (sum of row lengths vs 64 x the longest) and its cells per
+                // lane stay below a share of the plan's cells per lane of the
+                // device: the job runs as one wave, which alone on a SIMD needs as
+                // long per cell as the whole SIMD needs per lane-cell, so a longer
+                // job would set the makespan (split tiles finish it sooner);
+                // otherwise its pairs go to the tile kernel
+                const int nb = (opt.force_nb > 0 && cl > 96) ? opt.force_nb : group_nb(cl, opt.tune);
+                const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
+                R.nb[cs] = nb;
+                for (int64_t g0 = 0; g0 < nc; g0 += 64) {
+                    const int64_t g1 = std::min(nc, g0 + 64);
+                    double sum = 0;
+                    for (int64_t u = g0; u < g1; ++u) sum += row_len(cand[u]);
+                    const int top = std::max(1, row_len(cand[g0]));
+                    const double fill = sum / (64.0 * top);
+                    const double job_steps = (double)top * cl;
+                    bool keep =
+                        opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
+                    // too long for one wave: one item per chunk, pipelined over rows
+                    const bool split = !keep && !opt.no_group_split && fill >= GROUP_MIN_FILL && nch > 1 &&
+                                       (double)top * nb <= opt.group_max_share * per_lane &&
+                                       (double)GROUP_PUB * (nch - 1) <= 2.0 * GROUP_SPLIT_MAX_WAIT * top;
+                    keep = keep || split;
+                    const double wait = split ? (double)GROUP_PUB * (nch - 1) / (2.0 * top) : 0.0;
+                    cost += keep ? 64.0 * job_steps * (1.0 + wait) : TILE_COST_FACTOR * sum * cl;
+                    if (keep) {
+                        R.split[base + ngr] = split ? 1 : 0;
+                        for (int64_t u = g0; u < g1; ++u) R.grp[base + ngr++] = cand[u];
+                    } else {
+                        for (int64_t u = g0; u < g1; ++u) R.tile[base + nti++] = cand[u];
+                    }
+                }
+                R.ntile[cs] = nti;
+                R.ngrp[cs] = ngr;
+                ccost[cs] = cost;
+            }
+        }
+    };
+    for (int k = 1; k < cth; ++k) th.emplace_back(work);
+    work();
+    for (std::thread &t : th) t.join();
+    // lane slots of the jobs (64 x longest row x columns) plus the tile
+    // kernel's cells at its lower rate per cell, column by column
+    R.cost = 0;
+    for (int32_t c = 0; c < ncol; ++c) R.cost += ccost[c];
 }
 
 // Builds the scoring work list of a plan whose inputs are in place (plain
@@ -930,10 +1157,10 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
 
     std::vector<ScoreItem> &items = P->items;
     std::vector<uint8_t> &trans = P->trans;
-    items.assign((size_t)n_pairs, ScoreItem{});
-    trans.assign((size_t)n_pairs, 0);
     int64_t max_b = 0;
     if (mode == 0) {
+        items.assign((size_t)n_pairs, ScoreItem{});
+        trans.assign((size_t)n_pairs, 0);
         // The reference's batching: consecutive pairs form a batch while the
         // batch's full matrices fit in 80% of device memory (SmithWaterman.py:
         // 202-221, 420-432: 1 direction byte + 4 (linear) or 12 (affine) bytes
@@ -973,93 +1200,94 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
         std::vector<GroupItem> gitems;
         std::vector<GroupEntry> &gents = P->gents;
         std::vector<double> gcost;
-        auto form_groups = [&](int o, bool commit) -> double {
-            const int32_t ncol = o ? nq : nt;
-            auto col_of = [&](int64_t i) { return o ? pq[i] : pt[i]; };
-            auto row_len = [&](int64_t i) { return o ? tlen[pt[i]] : qlen[pq[i]]; };
-            std::vector<std::vector<int64_t>> by_col((size_t)ncol);
-            for (int64_t i = 0; i < n_pairs; ++i) by_col[col_of(i)].push_back(i);
-            // lane slots of the jobs (64 x longest row x columns) plus the tile
-            // kernel's cells at its lower rate per cell
-            double cost = 0;
-            for (int32_t cs = 0; cs < ncol; ++cs) {
-                std::vector<int64_t> &L = by_col[cs];
-                if (L.empty()) continue;
-                const int cl = o ? qlen[cs] : tlen[cs];
-                std::vector<int64_t> cand;
-                for (int64_t i : L)
-                    if (!opt.force_tile && row_len(i) <= GROUP_MAX_ROWS) cand.push_back(i);
-                    else {
-                        cost += TILE_COST_FACTOR * (double)row_len(i) * cl;
-                        if (commit) tile_pairs.push_back(i);
-                    }
-                std::stable_sort(cand.begin(), cand.end(),
-                                 [&](int64_t x, int64_t y) { return row_len(x) > row_len(y); });
-                // keep a 64-lane job only if its lanes are busy for most of its
-                // rows (sum of row lengths vs 64 x the longest) and its cells per
-                // lane stay below a share of the plan's cells per lane of the
-                // device: the job runs as one wave, which alone on a SIMD needs as
-                // long per cell as the whole SIMD needs per lane-cell, so a longer
-                // job would set the makespan (split tiles finish it sooner);
-                // otherwise its pairs go to the tile kernel
-                const int nb = (opt.force_nb > 0 && cl > 96) ? opt.force_nb : group_nb(cl, opt.tune);
-                const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
-                std::vector<int64_t> grp;
-                std::vector<uint8_t> grp_split;  // per kept job
-                for (size_t g0 = 0; g0 < cand.size(); g0 += 64) {
-                    const size_t g1 = std::min(cand.size(), g0 + 64);
-                    double sum = 0;
-                    for (size_t u = g0; u < g1; ++u) sum += row_len(cand[u]);
-                    const int top = std::max(1, row_len(cand[g0]));
-                    const double fill = sum / (64.0 * top);
-                    const double job_steps = (double)top * cl;
-                    bool keep =
-                        opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
-                    // too long for one wave: one item per chunk, pipelined over rows
-                    const bool split = !keep && !opt.no_group_split && fill >= GROUP_MIN_FILL && nch > 1 &&
-                                       (double)top * nb <= opt.group_max_share * per_lane &&
-                                       (double)GROUP_PUB * (nch - 1) <= 2.0 * GROUP_SPLIT_MAX_WAIT * top;
-                    keep = keep || split;
-                    const double wait = split ? (double)GROUP_PUB * (nch - 1) / (2.0 * top) : 0.0;
-                    cost += keep ? 64.0 * job_steps * (1.0 + wait) : TILE_COST_FACTOR * sum * cl;
-                    if (commit) {
-                        for (size_t u = g0; u < g1; ++u) (keep ? grp : tile_pairs).push_back(cand[u]);
-                        if (keep) grp_split.push_back(split ? 1 : 0);
-                    }
-                }
-                if (!commit) continue;
-                for (size_t g0 = 0; g0 < grp.size(); g0 += 64) {
-                    const int cnt = (int)std::min<size_t>(64, grp.size() - g0);
-                    GroupItem gi;
-                    gi.c_seq = o ? cs : nq + cs;
-                    gi.entry_off = (int32_t)gents.size();
-                    gi.count = cnt;
-                    gi.rows = row_len(grp[g0]);
-                    gi.nb = nb;
-                    gi.trans = o;
-                    gi.k0 = 0;
-                    gi.k1 = nch;
-                    gi.link_in = gi.link_out = -1;
-                    gi.prog_in = gi.prog_out = -1;
-                    gi.split = grp_split[g0 / 64];
-                    gi.pad = 0;
-                    for (int u = 0; u < cnt; ++u) {
-                        const int64_t i = grp[g0 + u];
-                        gents.push_back(GroupEntry{o ? nq + pt[i] : pq[i], (int32_t)i});
-                    }
-                    gitems.push_back(gi);
-                    // critical path: the whole job in one wave, or the row pipeline of a split one
-                    gcost.push_back(gi.split ? (double)(gi.rows + GROUP_PUB * (nch - 1)) * nb : (double)gi.rows * nch * nb);
-                    for (int u = 0; u < cnt; ++u) P->group_cells += (double)row_len(grp[g0 + u]) * cl;
-                    if (nch > 1 && !gi.split) P->group_rows = std::max<int64_t>(P->group_rows, gi.rows);
-                    P->max_nb = std::max(P->max_nb, nb);
-                }
-            }
-            return cost;
-        };
+        std::unique_ptr<PlanScratch> own_scratch;
+        PlanScratch *scr = opt.scratch;
+        if (!scr) {
+            own_scratch.reset(new PlanScratch());
+            scr = own_scratch.get();
+        }
+        OrientPlan *op = scr->op;
         int orient = opt.orient;
-        if (orient < 0) orient = form_groups(1, false) < form_groups(0, false) ? 1 : 0;
-        form_groups(orient, true);
+        if (orient < 0) {
+            orient_plan(opt, P, 1, per_lane, op[1]);
+            orient_plan(opt, P, 0, per_lane, op[0]);
+            orient = op[1].cost < op[0].cost ? 1 : 0;
+        } else {
+            orient_plan(opt, P, orient, per_lane, op[orient]);
+        }
+        const OrientPlan &R = op[orient];
+        {
+            // jobs of every column in column order, filled on several host
+            // threads at offsets from prefix sums over the columns
+            const int o = orient;
+            const int32_t ncol = o ? nq : nt;
+            auto row_len = [&](int64_t i) { return o ? tlen[pt[i]] : qlen[pq[i]]; };
+            std::vector<int64_t> job0((size_t)ncol + 1, 0), ent0((size_t)ncol + 1, 0), tile0((size_t)ncol + 1, 0);
+            for (int32_t cs = 0; cs < ncol; ++cs) {
+                job0[cs + 1] = job0[cs] + (R.ngrp[cs] + 63) / 64;
+                ent0[cs + 1] = ent0[cs] + R.ngrp[cs];
+                tile0[cs + 1] = tile0[cs] + R.ntile[cs];
+            }
+            gitems.resize((size_t)job0[ncol]);
+            gcost.resize((size_t)job0[ncol]);
+            gents.resize((size_t)ent0[ncol]);
+            tile_pairs.resize((size_t)tile0[ncol]);
+            // per-thread results (parallel_ranges runs at most 16 threads)
+            std::vector<int64_t> cells_t(16, 0), rows_t(16, 0);
+            std::vector<int> nb_t(16, 16);
+            parallel_ranges(ncol, std::max<int64_t>(1, (int64_t)ncol * (1 << 16) / std::max<int64_t>(1, n_pairs)),
+                            [&](int64_t c_lo, int64_t c_hi, int k) {
+                int64_t cells = 0, rows = 0;
+                int max_nb = 16;
+                for (int64_t cs = c_lo; cs < c_hi; ++cs) {
+                    const int64_t base = R.off[cs];
+                    std::copy(R.tile.begin() + base, R.tile.begin() + base + R.ntile[cs],
+                              tile_pairs.begin() + tile0[cs]);
+                    const int cl = o ? qlen[cs] : tlen[cs];
+                    const int nb = R.nb[cs];
+                    const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
+                    const int64_t ng = R.ngrp[cs];
+                    int64_t j = job0[cs], e = ent0[cs];
+                    for (int64_t g0 = 0; g0 < ng; g0 += 64, ++j) {
+                        const int cnt = (int)std::min<int64_t>(64, ng - g0);
+                        const int32_t *grp = R.grp.data() + base + g0;
+                        GroupItem gi;
+                        gi.c_seq = o ? (int32_t)cs : nq + (int32_t)cs;
+                        gi.entry_off = (int32_t)e;
+                        gi.count = cnt;
+                        gi.rows = row_len(grp[0]);
+                        gi.nb = nb;
+                        gi.trans = o;
+                        gi.k0 = 0;
+                        gi.k1 = nch;
+                        gi.link_in = gi.link_out = -1;
+                        gi.prog_in = gi.prog_out = -1;
+                        gi.split = R.split[base + g0];
+                        gi.pad = 0;
+                        for (int u = 0; u < cnt; ++u) {
+                            const int64_t i = grp[u];
+                            gents[e++] = GroupEntry{o ? nq + pt[i] : pq[i], (int32_t)i};
+                            cells += (int64_t)row_len(i) * cl;
+                        }
+                        gitems[j] = gi;
+                        // critical path: the whole job in one wave, or the row pipeline of a split one
+                        gcost[j] = gi.split ? (double)(gi.rows + GROUP_PUB * (nch - 1)) * nb : (double)gi.rows * nch * nb;
+                        if (nch > 1 && !gi.split) rows = std::max<int64_t>(rows, gi.rows);
+                        max_nb = std::max(max_nb, nb);
+                    }
+                }
+                cells_t[k] = cells;
+                rows_t[k] = rows;
+                nb_t[k] = max_nb;
+            });
+            int64_t cells = 0;
+            for (int k = 0; k < (int)cells_t.size(); ++k) {
+                cells += cells_t[k];
+                P->group_rows = std::max<int64_t>(P->group_rows, rows_t[k]);
+                P->max_nb = std::max(P->max_nb, nb_t[k]);
+            }
+            P->group_cells = (double)cells;  // an exact integer below 2^53
+        }
         P->orient = orient;
         // longest jobs first for the persistent workers
         std::vector<int64_t> gorder(gitems.size());
@@ -1179,11 +1407,9 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
 int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync) {
     const Encoded &E = P->E;
     const int64_t n = P->n_pairs;
-    std::vector<int2> pairs((size_t)n);
-    for (int64_t i = 0; i < n; ++i) pairs[i] = make_int2(P->pq[i], P->pt[i]);
     struct Part {
         DevBuf *d;
-        const void *h;
+        const void *h;  // nullptr: the pair list, packed from pq and pt
         size_t bytes;
         size_t off;
     };
@@ -1195,7 +1421,7 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
                     {&P->d_gitems, P->gitems.data(), P->gitems.size() * sizeof(GroupItem), 0},
                     {&P->d_gents, P->gents.data(), P->gents.size() * sizeof(GroupEntry), 0},
                     {&P->d_mss, P->mss.data(), (size_t)n * sizeof(float), 0},
-                    {&P->d_pairs, pairs.data(), (size_t)n * sizeof(int2), 0},
+                    {&P->d_pairs, nullptr, (size_t)n * sizeof(int2), 0},
                     {&P->d_same, P->same.data(), (size_t)n, 0}};
     size_t total = 0;
     for (Part &q : parts) {
@@ -1205,10 +1431,28 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
     ENSURE(ctx, P->blob, total);
     ENSURE_PINNED(ctx, stage, total);
     for (Part &q : parts) {
-        if (q.bytes) std::memcpy((char *)stage.p + q.off, q.h, q.bytes);
         q.d->p = (char *)P->blob.p + q.off;
         q.d->cap = q.bytes;
     }
+    // staging on several host threads: each copies its share of every part
+    // (and packs its share of the pair list)
+    char *st = (char *)stage.p;
+    const int32_t *pq = P->pq.data(), *pt = P->pt.data();
+    parallel_ranges((int64_t)total, 1 << 20, [&](int64_t lo, int64_t hi, int) {
+        for (const Part &q : parts) {
+            const int64_t a = std::max<int64_t>(lo, (int64_t)q.off), b = std::min<int64_t>(hi, (int64_t)(q.off + q.bytes));
+            if (a >= b) continue;
+            if (q.h) {
+                std::memcpy(st + a, (const char *)q.h + (a - (int64_t)q.off), (size_t)(b - a));
+            } else {
+                // whole int2 records: a thread writes every record that starts in its range
+                const int64_t r0 = (a - (int64_t)q.off + (int64_t)sizeof(int2) - 1) / (int64_t)sizeof(int2);
+                const int64_t r1 = std::min<int64_t>(n, (b - (int64_t)q.off + (int64_t)sizeof(int2) - 1) / (int64_t)sizeof(int2));
+                int2 *dst = (int2 *)(st + q.off);
+                for (int64_t i = r0; i < r1; ++i) dst[i] = make_int2(pq[i], pt[i]);
+            }
+        }
+    });
     HIPCHK(ctx, hipMemcpyAsync(P->blob.p, stage.p, total, hipMemcpyHostToDevice, s));
     ENSURE(ctx, P->d_links, (size_t)P->score_links * sizeof(float));
     ENSURE(ctx, P->d_prog, (size_t)P->score_prog * sizeof(unsigned));
@@ -2149,6 +2393,8 @@ void fill_public(HostResult *R, const float *maxima, int64_t n_pairs, double cel
     O.ms_readback = F.ms_readback;
     O.ms_tail = F.ms_tail;
     O.ms_assemble = F.ms_assemble;
+    O.ms_plan = 0;  // set by the one-shot call that planned the execution
+    O.ms_upload = 0;
     O.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - X.wall0).count();
 }
 
@@ -2177,6 +2423,8 @@ PlanOpts plan_opts(pw_ctx *ctx, const Scoring &sc) {
     o.orient = ctx->route_orient;
     o.group_max_share = ctx->route_share;
     o.dir_rmax = ctx->route_dir_rmax;
+    if (!ctx->plan_scratch) ctx->plan_scratch = new PlanScratch();
+    o.scratch = ctx->plan_scratch;
     return o;
 }
 
@@ -2557,7 +2805,8 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         ctx->err = "pw_score_pairs: missing argument";
         return PW_EINVAL;
     }
-    pw_plan *P = new pw_plan();
+    if (!ctx->oneshot) ctx->oneshot = new pw_plan();
+    pw_plan *P = ctx->oneshot;
     rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    nullptr, nullptr, mode);
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
@@ -2588,7 +2837,6 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     }
     if (!rc) rc = check_wait_abort(ctx, W, L);
     if (!rc && ms_out) *ms_out = elapsed(L.ev[0], L.ev[1]);
-    free_plan(ctx, P);
     return rc;
 }
 
@@ -2734,18 +2982,30 @@ extern "C" int pw_align_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
         return PW_EINVAL;
     }
     *out = nullptr;
-    pw_plan *P = new pw_plan();
+    const auto t0 = std::chrono::steady_clock::now();
+    // the context's one-shot plan: planned anew, its host vectors and device
+    // arrays reused at their current sizes
+    if (!ctx->oneshot) ctx->oneshot = new pw_plan();
+    pw_plan *P = ctx->oneshot;
     rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
                    min_start_score, same_id, mode);
+    const double ms_plan = ms_since(t0);
+    const auto t1 = std::chrono::steady_clock::now();
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
+    const double ms_upload = ms_since(t1);
     if (!rc) {
         HostResult *R = nullptr;
         new_result(ctx, &R);
         rc = execute(ctx, P, ctx->ws, ctx->lane, minimum, sp_cap, false, R);
-        if (rc) pw_free_result(ctx, &R->pub);
-        else *out = &R->pub;
+        if (rc) {
+            pw_free_result(ctx, &R->pub);
+        } else {
+            R->pub.ms_plan = ms_plan;
+            R->pub.ms_upload = ms_upload;
+            R->pub.ms_total = ms_since(t0);
+            *out = &R->pub;
+        }
     }
-    free_plan(ctx, P);
     return rc;
 }
 
