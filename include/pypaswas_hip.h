@@ -71,6 +71,9 @@ typedef struct {
     double ms_assemble;                  /* host: records into pair order */
     double ms_plan;                      /* host: encoding and work list of a one-shot call (pw_align_pairs) */
     double ms_upload;                    /* host: staging and copy of the plan to the device (one-shot call) */
+    const char *aligned;                 /* [3 * n_ops] for each shown hit at 3 * ops_offset: its query row,
+                                            alignment row ('|', '.', ' ') and target row, `length` chars each
+                                            (SmithWaterman.py:777-830) */
 } pw_result;
 
 /* SmithWatermanOcl._initialize_device (pyPaSWAS/Core/SmithWatermanOcl.py:116-129)

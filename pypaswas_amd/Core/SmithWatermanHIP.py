@@ -45,43 +45,16 @@ def job_arrays(queries, targets, highest, filter_factor):
     nq, nt = len(qstr), len(tstr)
     pq = np.repeat(np.arange(nq, dtype=np.int32), nt)
     pt = np.tile(np.arange(nt, dtype=np.int32), nq)
-    qlen = np.array([len(s) for s in qstr], dtype=np.int64)
-    tlen = np.array([len(s) for s in tstr], dtype=np.int64)
-    mss = min_start_scores(highest, filter_factor, qlen[pq], tlen[pt])
-    qids = np.array([ident(r) for r in queries], dtype=object)
-    tids = np.array([ident(r) for r in targets], dtype=object)
-    same = (qids[pq] == tids[pt]).astype(np.uint8)
+    qlen = np.array([len(s) for s in qstr], dtype=np.float64)
+    tlen = np.array([len(s) for s in tstr], dtype=np.float64)
+    # query-major nq x nt grids, raveled in the order of pq / pt
+    mss = min_start_scores(highest, filter_factor, qlen[:, None], tlen[None, :]).ravel()
+    # ids as integer codes over both lists, so that the per-pair comparison is numeric
+    codes = {}
+    qcode = np.array([codes.setdefault(ident(r), len(codes)) for r in queries], dtype=np.int64)
+    tcode = np.array([codes.setdefault(ident(r), len(codes)) for r in targets], dtype=np.int64)
+    same = (qcode[:, None] == tcode[None, :]).ravel().view(np.uint8)
     return PackedSeqs.pack(qstr), PackedSeqs.pack(tstr), pq, pt, mss, same
-
-
-def decode_alignment(ops, offset, length, query, target, s_start, t_start):
-    """Rebuilds the three alignment strings from the walk's column codes.
-
-    The walk records one code per column from the alignment end back to its
-    start, so the codes are read in reverse while x and y advance from the start.
-    """
-    x, y = s_start, t_start
-    sm, al, tm = [], [], []
-    for k in range(length - 1, -1, -1):
-        op = ops[offset + k]
-        if op == 68:  # 'D'
-            a, b = query[x], target[y]
-            sm.append(a)
-            tm.append(b)
-            al.append(SmithWaterman.MATCH_CHAR if a.lower() == b.lower() else SmithWaterman.MISMATCH_CHAR)
-            x += 1
-            y += 1
-        elif op == 76:  # 'L'
-            sm.append(query[x])
-            tm.append(SmithWaterman.GAP_CHAR_SEQ)
-            al.append(SmithWaterman.GAP_CHAR_ALIGN)
-            x += 1
-        else:  # 'U'
-            sm.append(SmithWaterman.GAP_CHAR_SEQ)
-            tm.append(target[y])
-            al.append(SmithWaterman.GAP_CHAR_ALIGN)
-            y += 1
-    return ''.join(sm), ''.join(al), ''.join(tm)
 
 
 class SmithWatermanHIP(SmithWaterman):
@@ -120,9 +93,7 @@ class SmithWatermanHIP(SmithWaterman):
         return hitlist
 
     def _align_block(self, queries, targets, hitlist):
-        qstr = [str(r.seq) for r in queries]
-        tstr = [str(r.seq) for r in targets]
-        nq, nt = len(qstr), len(tstr)
+        nq, nt = len(queries), len(targets)
         qp, tp, pq, pt, mss, same = job_arrays(queries, targets, self.score.highest_score, self.filter_factor)
         res = self.dev.align_pairs(qp, tp, pq, pt, mss, same, float(self.settings.minimum_score),
                                    self.maximum_number_starting_points, self.mode)
@@ -131,31 +102,32 @@ class SmithWatermanHIP(SmithWaterman):
         if len(res.starting_points) >= self.maximum_number_starting_points * nq * nt:
             self.logger.warning("Too many hits returned. Skipping the rest. Please set lower_limit_score higher in "
                                 "config.")
-        self._collect_hits(res, queries, targets, qstr, tstr, hitlist)
+        self._collect_hits(res, queries, targets, hitlist)
 
-    def _collect_hits(self, res, queries, targets, qstr, tstr, hitlist):
+    def _collect_hits(self, res, queries, targets, hitlist):
         hits = res.hits
         if len(hits) == 0:
             return
+        # the records as Python values in one conversion, and the library's
+        # alignment rows (SmithWaterman.py:777-830, start-to-end order, 3 chars
+        # per op) as one string
+        recs = hits[['shown', 'query', 'target', 's_start', 's_end', 't_start', 't_end', 'matches', 'mismatches',
+                     'length', 'score', 'ops_offset']].tolist()
+        rows = bytes(res.aligned).decode('latin-1')
         # starting points are processed in ascending score order, stable (SmithWaterman.py:699)
-        order = np.argsort(hits['score'], kind='stable')
+        order = np.argsort(hits['score'], kind='stable').tolist()
         for k in order:
-            h = hits[k]
-            if not h['shown']:
+            shown, qi, ti, s0, s1, t0, t1, matches, mismatches, n, score, off = recs[k]
+            if not shown:
                 continue
-            qi, ti = int(h['query']), int(h['target'])
             qrec, trec = queries[qi], targets[ti]
-            s_start = int(h['s_start']) + qrec.start_position
-            s_end = int(h['s_end']) + qrec.start_position
-            t_start = int(h['t_start']) + trec.start_position
-            t_end = int(h['t_end']) + trec.start_position
-            hit = Hit(self.logger, qrec, trec, (s_start, s_end), (t_start, t_end))
-            sm, al, tm = decode_alignment(res.ops, int(h['ops_offset']), int(h['length']), qstr[qi], tstr[ti],
-                                          int(h['s_start']), int(h['t_start']))
-            hit.set_sequence_match(sm)
-            hit.set_alignment(al)
-            hit.set_target_match(tm)
-            hit.set_scores(float(h['score']), int(h['matches']), int(h['mismatches']))
+            hit = Hit(self.logger, qrec, trec, (s0 + qrec.start_position, s1 + qrec.start_position),
+                      (t0 + trec.start_position, t1 + trec.start_position))
+            a0 = 3 * off
+            hit.set_sequence_match(rows[a0:a0 + n])
+            hit.set_alignment(rows[a0 + n:a0 + 2 * n])
+            hit.set_target_match(rows[a0 + 2 * n:a0 + 3 * n])
+            hit.set_scores(score, matches, mismatches)
             if self._filter_hit(hit):
                 hitlist.append(hit)
             else:

@@ -58,7 +58,8 @@ class ResultRec(ctypes.Structure):
                 ("ms_directions", ctypes.c_double), ("ms_traceback", ctypes.c_double), ("ms_walk", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("score_launches", ctypes.c_int64),
                 ("ms_readback", ctypes.c_double), ("ms_tail", ctypes.c_double), ("ms_assemble", ctypes.c_double),
-                ("ms_plan", ctypes.c_double), ("ms_upload", ctypes.c_double)]
+                ("ms_plan", ctypes.c_double), ("ms_upload", ctypes.c_double),
+                ("aligned", ctypes.POINTER(ctypes.c_char))]
 
 
 _lib = None
@@ -165,6 +166,7 @@ class AlignResult:
     ms_assemble: float = 0.0
     ms_plan: float = 0.0
     ms_upload: float = 0.0
+    aligned: memoryview = memoryview(b"")  # 3 bytes per op: query, alignment and target rows of each shown hit
 
 
 SP_DTYPE = np.dtype([("sequence", "<u4"), ("target", "<u4"), ("blockX", "<u4"), ("blockY", "<u4"),
@@ -283,12 +285,16 @@ class Device:
             buf = (ctypes.c_char * n_ops).from_address(ctypes.cast(r.ops, ctypes.c_void_p).value)
             buf._hold = hold
             ops = memoryview(buf).cast("B")
+            abuf = (ctypes.c_char * (3 * n_ops)).from_address(ctypes.cast(r.aligned, ctypes.c_void_p).value)
+            abuf._hold = hold
+            aligned = memoryview(abuf).cast("B")
         else:
             ops = memoryview(b"")
+            aligned = memoryview(b"")
         res = AlignResult(maxima, sps, hits, ops, int(r.n_traceback_pairs), int(r.sp_truncated),
                           float(r.cells), float(r.ms_score), float(r.ms_directions), float(r.ms_traceback),
                           float(r.ms_walk), float(r.ms_total), int(r.score_launches), float(r.ms_readback),
-                          float(r.ms_tail), float(r.ms_assemble), float(r.ms_plan), float(r.ms_upload))
+                          float(r.ms_tail), float(r.ms_assemble), float(r.ms_plan), float(r.ms_upload), aligned)
         # with no view left holding it (an empty result), the record goes back here
         del hold
         return res

@@ -19,16 +19,22 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <cctype>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pypaswas_hip.h"
@@ -158,6 +164,7 @@ struct HostResult {
     std::vector<pw_starting_point> sps;
     std::vector<pw_hit> hits;
     std::vector<char> ops;
+    std::vector<char> aligned;  // 3 chars per op: the three alignment rows of each shown hit
     int64_t n_sps = 0, n_ops = 0;
 };
 
@@ -205,6 +212,7 @@ namespace {
 
 struct Encoded {
     int K = 0, KP = 8, lkp = 5;
+    std::vector<uint8_t> chars;     // code -> the residue character it stands for
     std::vector<uint8_t> codes;     // all residues, queries then targets
     std::vector<SeqRef> refs;       // nq + nt
     std::vector<float> tables;      // 2 * KP * KP
@@ -649,41 +657,148 @@ extern "C" int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_shar
 
 namespace {
 
+// A pool of host threads for the planner's parallel loops, created on first
+// use and kept for the life of the process (its threads wait on a condition
+// variable between runs; thread creation would cost tens of microseconds per
+// thread and loop).  run(n, f) calls f(0) on the caller and f(1..n-1) on
+// pool threads, and returns when all have returned; runs from different
+// callers take turns.  f must not start a run itself.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool *pool = new HostPool();  // never destroyed: its threads outlive static destruction
+        return *pool;
+    }
+    int size() const { return nthreads_; }
+    void run(int n, const std::function<void(int)> &f) {
+        n = std::max(1, std::min(n, nthreads_));
+        std::lock_guard<std::mutex> turn(run_mutex_);
+        if (n > 1) {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &f;
+            njobs_ = n;
+            remaining_ = n - 1;
+            ++gen_;
+            cv_.notify_all();
+        }
+        f(0);
+        if (n > 1) {
+            std::unique_lock<std::mutex> lk(m_);
+            done_cv_.wait(lk, [this] { return remaining_ == 0; });
+            job_ = nullptr;
+        }
+    }
+
+  private:
+    HostPool() {
+        nthreads_ = (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+        for (int k = 1; k < nthreads_; ++k) std::thread([this, k] { loop(k); }).detach();
+    }
+    void loop(int k) {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)> *fn = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (k >= njobs_) continue;
+                fn = job_;
+            }
+            (*fn)(k);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--remaining_ == 0) done_cv_.notify_one();
+        }
+    }
+    int nthreads_ = 1;
+    std::mutex run_mutex_, m_;
+    std::condition_variable cv_, done_cv_;
+    uint64_t gen_ = 0;
+    int njobs_ = 0, remaining_ = 0;
+    const std::function<void(int)> *job_ = nullptr;
+};
+
+// Host threads for n elements: at most the pool's, and one per min_per_thread elements.
+int host_threads(int64_t n, int64_t min_per_thread) {
+    const int64_t hw = HostPool::get().size();
+    return (int)std::max<int64_t>(1, std::min<int64_t>(hw, n / std::max<int64_t>(1, min_per_thread)));
+}
+
+// Runs f(lo, hi, k) over [0, n) in contiguous ranges, thread k of the
+// host_threads(n, min_per_thread) threads taking the k-th range.
+template <class F>
+void parallel_ranges(int64_t n, int64_t min_per_thread, F f) {
+    const int nth = host_threads(n, min_per_thread);
+    if (nth <= 1) {
+        f((int64_t)0, n, 0);
+        return;
+    }
+    HostPool::get().run(nth, [&](int k) { f(n * k / nth, n * (k + 1) / nth, k); });
+}
+
+// Residues into the compact alphabet: the characters present get codes in
+// the order of their first appearance (queries, then targets).  Host threads
+// find each character's first position in their share of the sequences, then
+// encode their share.
 int encode(std::string &err, const Scoring &sc, const uint8_t *qres, const int64_t *qoff, const int32_t *qlen,
            int32_t nq, const uint8_t *tres, const int64_t *toff, const int32_t *tlen, int32_t nt, Encoded &E) {
+    const int64_t ns = (int64_t)nq + nt;
+    int64_t total = 0;
+    for (int64_t i = 0; i < ns; ++i) {
+        const int32_t len = i < nq ? qlen[i] : tlen[i - nq];
+        const int64_t off = i < nq ? qoff[i] : toff[i - nq];
+        if (len < 0 || off < 0) {
+            err = "negative sequence length or offset";
+            return PW_EINVAL;
+        }
+        total += len;
+    }
+    auto seq = [&](int64_t i, const uint8_t *&p, int32_t &len) {
+        if (i < nq) {
+            p = qres + qoff[i];
+            len = qlen[i];
+        } else {
+            p = tres + toff[i - nq];
+            len = tlen[i - nq];
+        }
+    };
+    // first appearance of every byte value, as (sequence << 32) + position
+    constexpr int64_t NONE = INT64_MAX;
+    const int64_t per = std::max<int64_t>(1, ns * 4096 / std::max<int64_t>(1, total));  // ~4096 residues per element
+    std::vector<std::array<int64_t, 256>> first(16);
+    for (auto &f : first) f.fill(NONE);
+    parallel_ranges(ns, per * 16, [&](int64_t lo, int64_t hi, int k) {
+        std::array<int64_t, 256> &f = first[k];
+        for (int64_t i = lo; i < hi; ++i) {
+            const uint8_t *p;
+            int32_t len;
+            seq(i, p, len);
+            for (int32_t j = 0; j < len; ++j)
+                if (f[p[j]] == NONE) f[p[j]] = (i << 32) + j;
+        }
+    });
+    std::array<int64_t, 256> pos;
+    pos.fill(NONE);
+    for (const auto &f : first)
+        for (int c = 0; c < 256; ++c) pos[c] = std::min(pos[c], f[c]);
+    std::vector<int> used;
+    for (int c = 0; c < 256; ++c)
+        if (pos[c] != NONE) used.push_back(c);
+    std::sort(used.begin(), used.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+    for (int c : used) {
+        const int idx = c - sc.char_offset;
+        if (idx < 0 || idx >= sc.dim) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "character %d ('%c') is outside the scoring alphabet (offset %d, dimension %d)", c,
+                     (c >= 32 && c < 127) ? c : '?', sc.char_offset, sc.dim);
+            err = buf;
+            return PW_EINVAL;
+        }
+    }
     int map[256];
     for (int &m : map) m = -1;
-    std::vector<int> used;
-    auto scan = [&](const uint8_t *res, const int64_t *off, const int32_t *len, int32_t n) -> int {
-        for (int32_t i = 0; i < n; ++i) {
-            if (len[i] < 0 || off[i] < 0) {
-                err = "negative sequence length or offset";
-                return PW_EINVAL;
-            }
-            const uint8_t *p = res + off[i];
-            for (int32_t k = 0; k < len[i]; ++k) {
-                const int c = p[k];
-                if (map[c] < 0) {
-                    const int idx = c - sc.char_offset;
-                    if (idx < 0 || idx >= sc.dim) {
-                        char buf[160];
-                        snprintf(buf, sizeof buf,
-                                 "character %d ('%c') is outside the scoring alphabet (offset %d, dimension %d)", c,
-                                 (c >= 32 && c < 127) ? c : '?', sc.char_offset, sc.dim);
-                        err = buf;
-                        return PW_EINVAL;
-                    }
-                    map[c] = (int)used.size();
-                    used.push_back(c);
-                }
-            }
-        }
-        return PW_OK;
-    };
-    int rc = scan(qres, qoff, qlen, nq);
-    if (rc) return rc;
-    rc = scan(tres, toff, tlen, nt);
-    if (rc) return rc;
+    for (size_t k = 0; k < used.size(); ++k) map[used[k]] = (int)k;
+    E.chars.assign(used.begin(), used.end());
     E.K = (int)used.size();
     if (E.K > KPMAX - 1) {
         err = "more than 63 distinct residues in one call";
@@ -703,22 +818,26 @@ int encode(std::string &err, const Scoring &sc, const uint8_t *qres, const int64
             // table 1: A = target -> entry [b][a] = S(query b, target a)
             E.tables[KP * KP + b * KP + a] = sc.subst[(size_t)ib * sc.dim + ia];
         }
-    int64_t total = 0;
-    for (int32_t i = 0; i < nq; ++i) total += qlen[i];
-    for (int32_t i = 0; i < nt; ++i) total += tlen[i];
     E.codes.resize((size_t)total + 64);
-    E.refs.resize((size_t)nq + nt);
-    int64_t pos = 0;
-    for (int32_t i = 0; i < nq; ++i) {
-        E.refs[i] = SeqRef{pos, qlen[i], 0};
-        for (int32_t k = 0; k < qlen[i]; ++k) E.codes[pos++] = (uint8_t)map[qres[qoff[i] + k]];
-        E.max_len_q = std::max<int64_t>(E.max_len_q, qlen[i]);
+    E.refs.resize((size_t)ns);
+    E.max_len_q = E.max_len_t = 0;
+    int64_t run = 0;
+    for (int64_t i = 0; i < ns; ++i) {
+        const int32_t len = i < nq ? qlen[i] : tlen[i - nq];
+        E.refs[i] = SeqRef{run, len, 0};
+        run += len;
+        if (i < nq) E.max_len_q = std::max<int64_t>(E.max_len_q, len);
+        else E.max_len_t = std::max<int64_t>(E.max_len_t, len);
     }
-    for (int32_t i = 0; i < nt; ++i) {
-        E.refs[nq + i] = SeqRef{pos, tlen[i], 0};
-        for (int32_t k = 0; k < tlen[i]; ++k) E.codes[pos++] = (uint8_t)map[tres[toff[i] + k]];
-        E.max_len_t = std::max<int64_t>(E.max_len_t, tlen[i]);
-    }
+    parallel_ranges(ns, per * 16, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const uint8_t *p;
+            int32_t len;
+            seq(i, p, len);
+            uint8_t *dst = E.codes.data() + E.refs[i].off;
+            for (int32_t j = 0; j < len; ++j) dst[j] = (uint8_t)map[p[j]];
+        }
+    });
     return PW_OK;
 }
 
@@ -817,11 +936,20 @@ int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, con
         err = "n_pairs out of range";
         return PW_EINVAL;
     }
-    for (int64_t i = 0; i < n_pairs; ++i)
-        if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
-            err = "pair " + std::to_string(i) + " indexes outside the sequence lists";
-            return PW_EINVAL;
-        }
+    // the first pair out of range, over host threads
+    std::vector<int64_t> bad(16, INT64_MAX);
+    parallel_ranges(n_pairs, 1 << 18, [&](int64_t lo, int64_t hi, int k) {
+        for (int64_t i = lo; i < hi; ++i)
+            if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
+                bad[k] = i;
+                break;
+            }
+    });
+    const int64_t first = *std::min_element(bad.begin(), bad.end());
+    if (first != INT64_MAX) {
+        err = "pair " + std::to_string(first) + " indexes outside the sequence lists";
+        return PW_EINVAL;
+    }
     return PW_OK;
 }
 
@@ -846,27 +974,6 @@ void free_plan(pw_ctx *ctx, pw_plan *P) {
     delete P;
 }
 
-// Host threads for n elements: up to 16, at most one per min_per_thread elements.
-int host_threads(int64_t n, int64_t min_per_thread) {
-    const int64_t hw = std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)std::thread::hardware_concurrency()));
-    return (int)std::max<int64_t>(1, std::min<int64_t>(hw, n / std::max<int64_t>(1, min_per_thread)));
-}
-
-// Runs f(lo, hi, k) over [0, n) in contiguous ranges, thread k of the
-// host_threads(n, min_per_thread) threads taking the k-th range.
-template <class F>
-void parallel_ranges(int64_t n, int64_t min_per_thread, F f) {
-    const int nth = host_threads(n, min_per_thread);
-    if (nth <= 1) {
-        f((int64_t)0, n, 0);
-        return;
-    }
-    std::vector<std::thread> th;
-    th.reserve((size_t)nth);
-    for (int k = 0; k < nth; ++k) th.emplace_back([&f, n, nth, k] { f(n * k / nth, n * (k + 1) / nth, k); });
-    for (std::thread &t : th) t.join();
-}
-
 // Host half of pw_prepare: encodes the residues and builds the scoring work
 // list.  No device call.
 int plan_worklist(const PlanOpts &opt, pw_plan *P);
@@ -889,14 +996,21 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
     P->nt = nt;
     P->n_pairs = n_pairs;
     P->mode = mode;
-    P->pq.assign(pq, pq + n_pairs);
-    P->pt.assign(pt, pt + n_pairs);
+    // the caller's per-pair arrays, copied on host threads (a reused plan keeps their capacity)
+    auto copy_in = [n_pairs](auto &v, const auto *src) {
+        using T = typename std::decay<decltype(v)>::type::value_type;
+        v.resize((size_t)n_pairs);
+        parallel_ranges(n_pairs, 1 << 18, [&](int64_t lo, int64_t hi, int) {
+            if (src) std::memcpy(v.data() + lo, src + lo, (size_t)(hi - lo) * sizeof(T));
+            else std::memset((void *)(v.data() + lo), 0, (size_t)(hi - lo) * sizeof(T));
+        });
+    };
+    copy_in(P->pq, pq);
+    copy_in(P->pt, pt);
     P->qlen.assign(qlen, qlen + nq);
     P->tlen.assign(tlen, tlen + nt);
-    if (mss) P->mss.assign(mss, mss + n_pairs);
-    else P->mss.assign((size_t)n_pairs, 0.0f);
-    if (same_id) P->same.assign(same_id, same_id + n_pairs);
-    else P->same.assign((size_t)n_pairs, 0);
+    copy_in(P->mss, mss);
+    copy_in(P->same, same_id);
     // per-pair sizes of the traceback stages for each cap of the direction
     // pass's residues per lane (2, 4, 8), in one pass over the pairs on
     // several host threads; the cap is then the largest of 8, 4, 2 for which
@@ -973,10 +1087,14 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
 // order), the kept pairs (ngrp[c], 64 per job) and, at the position of each
 // kept job's first pair, whether it is split.  Columns are independent and
 // run on several host threads.
+struct PairRef {
+    int32_t pair;  // pair index (n_pairs <= INT32_MAX)
+    int32_t row;   // its row sequence: the query (o = 0) or the target (o = 1)
+};
 struct OrientPlan {
     double cost = 0;
     std::vector<int64_t> off, ntile, ngrp;
-    std::vector<int32_t> by_col, tile, grp;  // pair indices (n_pairs <= INT32_MAX)
+    std::vector<PairRef> by_col, tile, grp;
     std::vector<uint8_t> split;
     std::vector<int> nb;
 };
@@ -1005,9 +1123,10 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
     const int32_t *pq = P->pq.data(), *pt = P->pt.data(), *qlen = P->qlen.data(), *tlen = P->tlen.data();
     const int32_t ncol = o ? nq : nt;
     auto col_of = [&](int64_t i) { return o ? pq[i] : pt[i]; };
-    auto row_len = [&](int64_t i) { return o ? tlen[pt[i]] : qlen[pq[i]]; };
-    // pairs grouped by column, in pair order: per-thread column counts, then
-    // each thread fills its own slots
+    const int32_t *rlen = o ? tlen : qlen;  // row length of a row sequence
+    // pairs grouped by column, in pair order, each with its row sequence (read
+    // here, in pair order, instead of gathered later): per-thread column
+    // counts, then each thread fills its own slots
     const int nth = host_threads(n, 1 << 16);  // the thread count parallel_ranges(n, 1 << 16, ...) uses
     std::vector<int64_t> cnt((size_t)nth * ncol, 0);
     parallel_ranges(n, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
@@ -1028,7 +1147,7 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
     R.by_col.resize((size_t)n);
     parallel_ranges(n, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
         int64_t *ps = pos.data() + (size_t)k * ncol;
-        for (int64_t i = lo; i < hi; ++i) R.by_col[ps[col_of(i)]++] = (int32_t)i;
+        for (int64_t i = lo; i < hi; ++i) R.by_col[ps[col_of(i)]++] = PairRef{(int32_t)i, o ? pt[i] : pq[i]};
     });
     R.tile.resize((size_t)n);
     R.grp.resize((size_t)n);
@@ -1040,32 +1159,32 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
     // columns in blocks, dynamically over the threads
     std::atomic<int32_t> next_col{0};
     const int cth = host_threads(n, 1 << 15);
-    std::vector<std::thread> th;
+    const int32_t step = std::max<int32_t>(1, std::min<int32_t>(16, ncol / (8 * cth)));
     auto work = [&]() {
-        std::vector<int32_t> cand, sorted;
+        std::vector<PairRef> cand, sorted;
         std::vector<int64_t> bucket;
         for (;;) {
-            const int32_t c0 = next_col.fetch_add(16);
+            const int32_t c0 = next_col.fetch_add(step);
             if (c0 >= ncol) break;
-            for (int32_t cs = c0; cs < std::min(ncol, c0 + 16); ++cs) {
+            for (int32_t cs = c0; cs < std::min(ncol, c0 + step); ++cs) {
                 const int64_t base = R.off[cs], m = R.off[cs + 1] - base;
                 if (m == 0) continue;
-                const int32_t *L = R.by_col.data() + base;
+                const PairRef *L = R.by_col.data() + base;
                 const int cl = o ? qlen[cs] : tlen[cs];
                 double cost = 0;
                 int64_t nti = 0, ngr = 0;
                 cand.clear();
                 int lmin = INT32_MAX, lmax = 0;
                 for (int64_t u = 0; u < m; ++u) {
-                    const int64_t i = L[u];
-                    const int rl = row_len(i);
+                    const PairRef pr = L[u];
+                    const int rl = rlen[pr.row];
                     if (!opt.force_tile && rl <= GROUP_MAX_ROWS) {
-                        cand.push_back((int32_t)i);
+                        cand.push_back(pr);
                         lmin = std::min(lmin, rl);
                         lmax = std::max(lmax, rl);
                     } else {
                         cost += TILE_COST_FACTOR * (double)rl * cl;
-                        R.tile[base + nti++] = (int32_t)i;
+                        R.tile[base + nti++] = pr;
                     }
                 }
                 // stable sort by row length, longest first: counting sort over
@@ -1075,14 +1194,14 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
                     const int64_t range = (int64_t)lmax - lmin + 1;
                     if (range <= 4 * nc + 1024) {
                         bucket.assign((size_t)range + 1, 0);
-                        for (int32_t i : cand) ++bucket[(size_t)(lmax - row_len(i)) + 1];
+                        for (const PairRef &pr : cand) ++bucket[(size_t)(lmax - rlen[pr.row]) + 1];
                         for (int64_t b = 1; b <= range; ++b) bucket[b] += bucket[b - 1];
                         sorted.resize((size_t)nc);
-                        for (int32_t i : cand) sorted[bucket[(size_t)(lmax - row_len(i))]++] = i;
+                        for (const PairRef &pr : cand) sorted[bucket[(size_t)(lmax - rlen[pr.row])]++] = pr;
                         cand.swap(sorted);
                     } else {
                         std::stable_sort(cand.begin(), cand.end(),
-                                         [&](int32_t x, int32_t y) { return row_len(x) > row_len(y); });
+                                         [&](const PairRef &x, const PairRef &y) { return rlen[x.row] > rlen[y.row]; });
                     }
                 }
                 // keep a 64-lane job only if its lanes are busy for most of its
@@ -1098,8 +1217,8 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
                 for (int64_t g0 = 0; g0 < nc; g0 += 64) {
                     const int64_t g1 = std::min(nc, g0 + 64);
                     double sum = 0;
-                    for (int64_t u = g0; u < g1; ++u) sum += row_len(cand[u]);
-                    const int top = std::max(1, row_len(cand[g0]));
+                    for (int64_t u = g0; u < g1; ++u) sum += rlen[cand[u].row];
+                    const int top = std::max(1, rlen[cand[g0].row]);
                     const double fill = sum / (64.0 * top);
                     const double job_steps = (double)top * cl;
                     bool keep =
@@ -1124,9 +1243,7 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
             }
         }
     };
-    for (int k = 1; k < cth; ++k) th.emplace_back(work);
-    work();
-    for (std::thread &t : th) t.join();
+    HostPool::get().run(cth, [&](int) { work(); });
     // lane slots of the jobs (64 x longest row x columns) plus the tile
     // kernel's cells at its lower rate per cell, column by column
     R.cost = 0;
@@ -1221,7 +1338,7 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
             // threads at offsets from prefix sums over the columns
             const int o = orient;
             const int32_t ncol = o ? nq : nt;
-            auto row_len = [&](int64_t i) { return o ? tlen[pt[i]] : qlen[pq[i]]; };
+            const int32_t *rlen = o ? tlen : qlen;
             std::vector<int64_t> job0((size_t)ncol + 1, 0), ent0((size_t)ncol + 1, 0), tile0((size_t)ncol + 1, 0);
             for (int32_t cs = 0; cs < ncol; ++cs) {
                 job0[cs + 1] = job0[cs] + (R.ngrp[cs] + 63) / 64;
@@ -1241,8 +1358,7 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                 int max_nb = 16;
                 for (int64_t cs = c_lo; cs < c_hi; ++cs) {
                     const int64_t base = R.off[cs];
-                    std::copy(R.tile.begin() + base, R.tile.begin() + base + R.ntile[cs],
-                              tile_pairs.begin() + tile0[cs]);
+                    for (int64_t u = 0; u < R.ntile[cs]; ++u) tile_pairs[tile0[cs] + u] = R.tile[base + u].pair;
                     const int cl = o ? qlen[cs] : tlen[cs];
                     const int nb = R.nb[cs];
                     const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
@@ -1250,12 +1366,12 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                     int64_t j = job0[cs], e = ent0[cs];
                     for (int64_t g0 = 0; g0 < ng; g0 += 64, ++j) {
                         const int cnt = (int)std::min<int64_t>(64, ng - g0);
-                        const int32_t *grp = R.grp.data() + base + g0;
+                        const PairRef *grp = R.grp.data() + base + g0;
                         GroupItem gi;
                         gi.c_seq = o ? (int32_t)cs : nq + (int32_t)cs;
                         gi.entry_off = (int32_t)e;
                         gi.count = cnt;
-                        gi.rows = row_len(grp[0]);
+                        gi.rows = rlen[grp[0].row];
                         gi.nb = nb;
                         gi.trans = o;
                         gi.k0 = 0;
@@ -1265,9 +1381,9 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                         gi.split = R.split[base + g0];
                         gi.pad = 0;
                         for (int u = 0; u < cnt; ++u) {
-                            const int64_t i = grp[u];
-                            gents[e++] = GroupEntry{o ? nq + pt[i] : pq[i], (int32_t)i};
-                            cells += (int64_t)row_len(i) * cl;
+                            const PairRef pr = grp[u];
+                            gents[e++] = GroupEntry{o ? nq + pr.row : pr.row, pr.pair};
+                            cells += (int64_t)rlen[pr.row] * cl;
                         }
                         gitems[j] = gi;
                         // critical path: the whole job in one wave, or the row pipeline of a split one
@@ -2121,7 +2237,7 @@ void sort_by_pair(std::vector<PairOut> &outs) {
     outs.swap(out);
 }
 
-void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostResult *R) {
+void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const pw_plan *P, HostResult *R) {
     sort_by_pair(outs);
     int64_t nsp = 0, nops = 0;
     for (const PairOut &o : outs) {
@@ -2131,11 +2247,14 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostR
     if ((int64_t)R->sps.size() < nsp) R->sps.resize((size_t)nsp);
     if ((int64_t)R->hits.size() < nsp) R->hits.resize((size_t)nsp);
     if ((int64_t)R->ops.size() < nops) R->ops.resize((size_t)nops);
+    if ((int64_t)R->aligned.size() < 3 * nops) R->aligned.resize((size_t)(3 * nops));
     R->n_sps = nsp;
     R->n_ops = nops;
     static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
+    const Encoded &E = P->E;
+    const int32_t nq = P->nq;
     // records of pairs [b, e) of outs, starting at sp0 / op0 in the result
-    auto fill = [R, &outs](size_t b, size_t e, int64_t sp0, int64_t op0) {
+    auto fill = [R, &outs, &E, nq](size_t b, size_t e, int64_t sp0, int64_t op0) {
         for (size_t i = b; i < e; ++i) {
             const PairOut &o = outs[i];
             if (o.nsp > 0) std::memcpy(&R->sps[sp0], o.sps, (size_t)o.nsp * sizeof(SpRec));
@@ -2156,6 +2275,32 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, HostR
                 h.score = hr.score;
                 h.sp_index = (int32_t)(sp0 + hr.sp_index);
                 h.ops_offset = op0 + hr.ops_offset;
+                // the three rows of SmithWaterman.py:777-830 (query, alignment,
+                // target), start to end: the op codes run from the end back
+                if (hr.shown && hr.length > 0) {
+                    const SeqRef qr = E.refs[hr.query], tr = E.refs[nq + hr.target];
+                    const uint8_t *qc = E.codes.data() + qr.off, *tc = E.codes.data() + tr.off;
+                    const char *op = o.ops + hr.ops_offset;
+                    char *qrow = R->aligned.data() + 3 * h.ops_offset, *arow = qrow + hr.length, *trow = arow + hr.length;
+                    int x = hr.s_start, y = hr.t_start;
+                    for (int c = 0; c < hr.length; ++c) {
+                        const char v = op[hr.length - 1 - c];
+                        if (v == 'D') {
+                            const char a = (char)E.chars[qc[x++]], t = (char)E.chars[tc[y++]];
+                            qrow[c] = a;
+                            trow[c] = t;
+                            arow[c] = std::tolower((unsigned char)a) == std::tolower((unsigned char)t) ? '|' : '.';
+                        } else if (v == 'L') {
+                            qrow[c] = (char)E.chars[qc[x++]];
+                            trow[c] = '-';
+                            arow[c] = ' ';
+                        } else {
+                            qrow[c] = '-';
+                            trow[c] = (char)E.chars[tc[y++]];
+                            arow[c] = ' ';
+                        }
+                    }
+                }
             }
             sp0 += o.nsp;
             op0 += o.nops;
@@ -2383,6 +2528,7 @@ void fill_public(HostResult *R, const float *maxima, int64_t n_pairs, double cel
     O.hits = R->hits.data();
     O.n_ops = R->n_ops;
     O.ops = R->ops.data();
+    O.aligned = R->aligned.data();
     O.n_traceback_pairs = ng;
     O.cells = cells;
     O.ms_score = F.ms_score;
@@ -2409,7 +2555,7 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
     rc = execute_finish(ctx, P, W, L, X, F);
     if (rc) return rc;
     const auto t0 = std::chrono::steady_clock::now();
-    assemble(F.outs, sp_cap, P->n_pairs, R);
+    assemble(F.outs, sp_cap, P->n_pairs, P, R);
     F.ms_assemble += ms_since(t0);
     fill_public(R, R->maxima, P->n_pairs, P->cells, F.ng, F, X);
     return PW_OK;

@@ -225,7 +225,7 @@ void split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j,
         ++cursor;
     }
     for (PairOut &o : mine) o.pair -= (int32_t)j->p0;
-    assemble(mine, j->sp_cap, (int64_t)j->pq.size(), R);
+    assemble(mine, j->sp_cap, (int64_t)j->pq.size(), ln.P, R);
     for (int64_t k = 0; k < R->n_sps; ++k) {
         pw_starting_point &sp = R->sps[(size_t)k];
         sp.sequence -= (uint32_t)j->q0;

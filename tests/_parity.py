@@ -115,6 +115,7 @@ def compare(O, queries, targets, p, res, qids=None, tids=None):
         if shown:
             sm, al, tm = decode_ops(res.ops, int(h["ops_offset"]), int(h["length"]), queries[qi], targets[ti],
                                     int(h["s_start"]), int(h["t_start"]))
+            assert library_rows(res, h) == (sm, al, tm), "library alignment rows differ from the op codes"
         h_w.setdefault((qi, ti), []).append(
             (shown, int(h["s_start"]), int(h["s_end"]), int(h["t_start"]), int(h["t_end"]), int(h["matches"]),
              int(h["mismatches"]), np.float32(h["score"]).view(np.uint32), sm, al, tm))
@@ -122,6 +123,13 @@ def compare(O, queries, targets, p, res, qids=None, tids=None):
     for k in o_w:
         assert o_w[k] == h_w[k], "walk differs for pair {}:\n{}\n{}".format(k, o_w[k][:3], h_w[k][:3])
     return br
+
+
+def library_rows(res, h):
+    """The query, alignment and target rows the library built for hit h (pw_result.aligned)."""
+    n, a0 = int(h["length"]), 3 * int(h["ops_offset"])
+    rows = bytes(res.aligned[a0:a0 + 3 * n]).decode("latin-1")
+    return rows[:n], rows[n:2 * n], rows[2 * n:]
 
 
 def decode_ops(ops: bytes, off: int, n: int, q: str, t: str, s_start: int, t_start: int):
@@ -183,6 +191,7 @@ def compare_pairs(O, queries, targets, pq, pt, p, res, qids=None, tids=None, thr
         if shown:
             sm, al, tm = decode_ops(res.ops, int(h["ops_offset"]), int(h["length"]), queries[qi], targets[ti],
                                     int(h["s_start"]), int(h["t_start"]))
+            assert library_rows(res, h) == (sm, al, tm), "library alignment rows differ from the op codes"
         hip_w.setdefault((qi, ti), []).append(
             (shown, int(h["s_start"]), int(h["s_end"]), int(h["t_start"]), int(h["t_end"]), int(h["matches"]),
              int(h["mismatches"]), np.float32(h["score"]).view(np.uint32), sm, al, tm))

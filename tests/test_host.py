@@ -292,7 +292,7 @@ def _fake_result(O, br, queries, targets, qids, tids, minimum):
     per_pair = {}
     for w in ws:
         per_pair.setdefault((w.qi, w.ti), []).append(w)
-    hits, ops = [], bytearray()
+    hits, ops, aligned = [], bytearray(), bytearray()
     for key in sorted(per_pair):
         for w in per_pair[key]:
             codes = []
@@ -301,12 +301,14 @@ def _fake_result(O, br, queries, targets, qids, tids, minimum):
                     codes.append("L" if b == "-" else ("U" if a == "-" else "D"))
             off = len(ops)
             ops += "".join(reversed(codes)).encode()
+            # the three rows at 3 * ops_offset, as pw_result.aligned holds them
+            aligned += (w.sequence_match + w.alignment + w.target_match).encode() if w.shown else b""
             hits.append((w.qi, w.ti, int(w.shown), w.s_start, w.s_end, w.t_start, w.t_end, w.matches,
                          w.mismatches, len(codes), w.score, 0, off))
     N, T = br.maxima.shape
     return AlignResult(br.maxima.reshape(-1),
                        np.zeros(0, SP_DTYPE), np.array(hits, dtype=HIT_DTYPE), bytes(ops), 0, 0, 0.0, 0, 0, 0, 0, 0,
-                       0)
+                       0, aligned=memoryview(bytes(aligned)))
 
 
 def test_collect_hits_reproduces_genbank_fixture(oracle):
@@ -337,7 +339,7 @@ def test_collect_hits_reproduces_genbank_fixture(oracle):
     qrecs = [SWSeqRecord(Seq(s), i) for i, s in rq]
     trecs = [SWSeqRecord(Seq(s), i) for i, s in rt]
     hl = HitList(LOG)
-    eng._collect_hits(res, qrecs, trecs, [s for _, s in rq], [s for _, s in rt], hl)
+    eng._collect_hits(res, qrecs, trecs, hl)
     lines = DefaultFormatter(LOG, hl, None).format_lines()
     want = open(os.path.join(GOLDEN, "aligner_basic_genbank_output.txt")).read().splitlines()
     got = "\n".join(lines).splitlines()
@@ -368,8 +370,7 @@ def test_collect_hits_matches_oracle_records(oracle):
     eng = SmithWatermanHIP.__new__(SmithWatermanHIP)
     eng.logger, eng.settings = LOG, S()
     hl = HitList(LOG)
-    eng._collect_hits(res, [SWSeqRecord(Seq(s), i) for i, s in rq], [SWSeqRecord(Seq(s), i) for i, s in rt], qs, ts,
-                      hl)
+    eng._collect_hits(res, [SWSeqRecord(Seq(s), i) for i, s in rq], [SWSeqRecord(Seq(s), i) for i, s in rt], hl)
     got = "".join(l + "\n" for l in DefaultFormatter(LOG, hl, None).format_lines())
     assert got == O.txt_records(hl_o)
     assert len(hl.real_hits) > 3

@@ -239,15 +239,17 @@ def _fake_result(O, qs, ts, p):
     for k, s in enumerate(br.starting_points):
         sps[k] = (s.sequence, s.target, s.blockX, s.blockY, s.valueX, s.valueY, s.score, s.maxScore, s.posScore)
     hits = np.zeros(len(ws), HIT_DTYPE)
-    ops = bytearray()
+    ops, aligned = bytearray(), bytearray()
     for k, w in enumerate(ws):
         off = len(ops)
         if w.shown:
             for a, b in reversed(list(zip(w.sequence_match, w.target_match))):
                 ops += b"L" if b == "-" else (b"U" if a == "-" else b"D")
+            aligned += (w.sequence_match + w.alignment + w.target_match).encode()
         hits[k] = (w.qi, w.ti, int(w.shown), w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches,
                    len(w.alignment) if w.shown else 0, w.score, 0, off)
-    return SimpleNamespace(maxima=br.maxima.reshape(-1), starting_points=sps, hits=hits, ops=bytes(ops))
+    return SimpleNamespace(maxima=br.maxima.reshape(-1), starting_points=sps, hits=hits, ops=bytes(ops),
+                           aligned=bytes(aligned))
 
 
 @pytest.mark.parametrize("affine", [False, True])
