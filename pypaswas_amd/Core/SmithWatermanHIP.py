@@ -99,9 +99,13 @@ class SmithWatermanHIP(SmithWaterman):
                                    self.maximum_number_starting_points, self.mode)
         self.last_result = res
         self.logger.debug('Number of starting points is: {0}.'.format(len(res.starting_points)))
-        if len(res.starting_points) >= self.maximum_number_starting_points * nq * nt:
-            self.logger.warning("Too many hits returned. Skipping the rest. Please set lower_limit_score higher in "
-                                "config.")
+        cap = self.maximum_number_starting_points * nq * nt
+        if len(res.starting_points) >= cap:
+            # SmithWaterman.py:683-685 drops the starting points beyond this count of its
+            # memory-sized batch; here every one is walked and reported
+            self.logger.warning("{0} starting points, at or above maximum_number_starting_points x pairs ({1}); "
+                                "all are reported. Set lower_limit_score higher in the config to find fewer."
+                                .format(len(res.starting_points), cap))
         self._collect_hits(res, queries, targets, hitlist)
 
     def _collect_hits(self, res, queries, targets, hitlist):

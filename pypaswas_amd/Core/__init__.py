@@ -10,8 +10,3 @@ NO_DIRECTION = 2 ** 5
 STOP_DIRECTION = 2 ** 6
 IN_ALIGNMENT = 2 ** 5 + 2 ** 6
 
-
-def read_file(filename):
-    """Reads a file and returns its contents as a single string."""
-    with open(filename) as contents:
-        return ''.join(contents.readlines())
