@@ -65,7 +65,10 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-batch launch")
-    ap.add_argument("--cpu-reads", type=int, default=800, help="reads in the CPU-baseline sample")
+    ap.add_argument("--cpu-reads", type=int, default=1600, help="reads in the CPU-baseline sample")
+    ap.add_argument("--cpu-one-core-reads", type=int, default=120, help="reads of the one-thread CPU run")
+    ap.add_argument("--cpu-device-gib", type=float, default=8.0,
+                    help="MAX_MEM_ALLOC_SIZE of the CPU device the reference's batching is sized for")
     ap.add_argument("--no-protein", action="store_true", help="skip the uneven-length protein comparison")
     ap.add_argument("--protein", default="200x2000", help="queries x targets of the protein sample")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
@@ -196,7 +199,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(q, t, settings, score, dev, args.cpu_reads)
+        cpu = cpu_baseline(q, t, settings, score, dev, args.cpu_reads, args.cpu_one_core_reads,
+                           args.cpu_device_gib)
 
     serving = None
     if not args.no_serving:
@@ -393,10 +397,59 @@ def protein_comparison(dev, nq, nt, tune=True):
     return out
 
 
-def cpu_baseline(q, t, settings, score, dev, n_reads):
-    """Oracle restatement of the reference CPU schedule on the first n_reads
-    reads (reader order) x all targets, in sub-batches of 8 reads (the
-    reference also batches by memory); hits compared with the GPU's."""
+# The reference's own cpu-OpenCL run on its benchmark (8 ankyrin queries of 4,624 residues x the 340-target file of
+# 281,798 residues, BLOSUM80, i7-2600K, 8 threads): data/desktop/timings/canisLupusAnkyrinPRED.fasta_340.fasta.CPU.log.txt
+# logs "Processing 8- vs 340-sequences" and 45 "Time spent on Smith-Waterman" lines summing to 106.42 s.  Its published
+# 0.82 GCUPS divides data/cellUpdates.csv:35 (80,113,762,410 cells) by that time, but calcCellUpdates.py multiplies the
+# target residues by the 284,295 residues of the 346-sequence canisLupusAnkyrinPRED.fasta, not by the 4,624 query
+# residues the run aligned: the cells of that run are 4,624 x 281,798 = 1.303e9.
+REFERENCE_CPU_LOGGED = {"cells": 4624 * 281798, "seconds": 106.42, "gcups": round(4624 * 281798 / 106.42 / 1e9, 4),
+                        "published_gcups": 0.82, "threads": 8, "cpu": "i7-2600K (4C/8T, 4.5 GHz)"}
+
+
+def reference_batches(lengths, target_len, n_targets, mem_bytes, fill, affine):
+    """Read batches of SmithWaterman.align_sequences (SmithWaterman.py:441-486,
+    recompile=T): from read `index`, padded length = that read's length rounded up
+    to 8 (reads are sorted longest first), and as many reads as
+    _get_max_number_sequences (:202-221) lets the score matrices of a batch fill
+    mem_bytes * fill: (len x tlen x (1 + 4 or 12 B) + len x tlen x 4 / 64 B) x targets
+    per read.  Returns (first read, count) per batch."""
+    basic = 1 + (3 if affine else 1) * 4
+    tlen = -(-target_len // 8) * 8
+    out, index = [], 0
+    while index < len(lengths):
+        length = -(-lengths[index] // 8) * 8
+        per_read = (length * tlen * basic + length * tlen * 4.0 / 64) * n_targets
+        count = max(1, int(mem_bytes * fill // per_read))
+        count = min(count, len(lengths) - index)
+        out.append((index, count))
+        index += count
+    return out
+
+
+def cpu_identity():
+    """nproc (the CPUs this process may run on) and the CPU model string."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return len(os.sched_getaffinity(0)), model
+
+
+def cpu_baseline(q, t, settings, score, dev, n_reads, one_core_reads, device_gib):
+    """The oracle's restatement of the reference OpenCL-CPU schedule (full FP32
+    matrices per pair, 8x8 blocks, one parallel sweep per block anti-diagonal,
+    the reverse traceback sweep and the host walk) on a stride sample of the
+    reads x all targets, batched as the reference batches for a CPU device whose
+    MAX_MEM_ALLOC_SIZE is device_gib (SmithWatermanOcl.py:131-134), on every CPU
+    this process may use; then the first one_core_reads reads of the sample as
+    one batch on one thread.  The hits of the full sample are compared with the
+    GPU's on the same pairs."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
     from pypaswas_amd.Core.SmithWatermanHIP import job_arrays
@@ -405,39 +458,62 @@ def cpu_baseline(q, t, settings, score, dev, n_reads):
                  filter_factor=float(settings.filter_factor), highest=float(score.highest_score))
     stride = max(1, len(q) // n_reads)
     sample = q[::stride][:n_reads]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    cells = float(sum(len(s) for _, s in sample)) * float(sum(len(s) for _, s in t))
+    nproc, model = cpu_identity()
+    omp_env = os.environ.get("OMP_NUM_THREADS")
+    threads = nproc
     targets = [s for _, s in t]
     tids = [i for i, _ in t]
-    o_hits = {}
-    t0 = time.perf_counter()
-    for b in range(0, len(sample), 8):
-        chunk = sample[b:b + 8]
-        qs = [s for _, s in chunk]
-        br = O.sched_batch(qs, targets, p, nthreads=threads)
-        ws = O.walk_c(br.directions, br.starting_points, qs, targets, [i for i, _ in chunk], tids, p.minimum)
-        for w in ws:
-            o_hits.setdefault((b + w.qi, w.ti), []).append(
-                (w.shown, w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches, w.score))
-    el = time.perf_counter() - t0
+    tcells = float(sum(len(s) for s in targets))
+    batches = reference_batches([len(s) for _, s in sample], max(len(s) for s in targets), len(targets),
+                                device_gib * 2.0 ** 30, float(settings.maximum_memory_usage),
+                                score.gap_extension is not None)
+
+    def run(reads, plan, nthreads, collect):
+        hits = {}
+        t0 = time.perf_counter()
+        for b, n in plan:
+            chunk = reads[b:b + n]
+            qs = [s for _, s in chunk]
+            br = O.sched_batch(qs, targets, p, nthreads=nthreads)
+            ws = O.walk_c(br.directions, br.starting_points, qs, targets, [i for i, _ in chunk], tids, p.minimum)
+            del br
+            if collect:
+                for w in ws:
+                    hits.setdefault((b + w.qi, w.ti), []).append(
+                        (bool(w.shown), w.s_start, w.s_end, w.t_start, w.t_end, w.matches, w.mismatches, w.score))
+        return time.perf_counter() - t0, hits
+
+    cells = float(sum(len(s) for _, s in sample)) * tcells
+    el, o_hits = run(sample, batches, threads, True)
+    one = sample[:one_core_reads]
+    cells1 = float(sum(len(s) for _, s in one)) * tcells
+    el1, _ = run(one, [(0, len(one))], 1, False)
     # the same sample on the GPU
     qp, tp, pq, pt, mss, same = job_arrays(sample, t, score.highest_score, float(settings.filter_factor))
     r = dev.align_pairs(qp, tp, pq, pt, mss, same, float(settings.minimum_score), mode=1)
     g_hits = {}
-    for h in r.hits:
-        g_hits.setdefault((int(h["query"]), int(h["target"])), []).append(
-            (bool(h["shown"]), int(h["s_start"]), int(h["s_end"]), int(h["t_start"]), int(h["t_end"]),
-             int(h["matches"]), int(h["mismatches"]), float(h["score"])))
-    o_norm = {k: [(bool(a),) + tuple(v[1:]) for v in vs for a in [v[0]]] for k, vs in o_hits.items()}
-    parity = "bit-exact" if o_norm == g_hits else "MISMATCH"
-    log("cpu baseline: %.2e cells in %.1fs on %d threads = %.3f GCUPS; GPU parity on the sample: %s (%d walks)"
-        % (cells, el, threads, cells / el / 1e9, parity, sum(len(v) for v in o_hits.values())))
+    for h in r.hits.tolist():
+        rec = dict(zip(r.hits.dtype.names, h))
+        g_hits.setdefault((rec["query"], rec["target"]), []).append(
+            (bool(rec["shown"]), rec["s_start"], rec["s_end"], rec["t_start"], rec["t_end"], rec["matches"],
+             rec["mismatches"], float(rec["score"])))
+    parity = "bit-exact" if o_hits == g_hits else "MISMATCH"
+    log("cpu baseline: %.2e cells in %.1fs on %d threads (%d batches) = %.3f GCUPS; 1 thread: %.2e cells in %.1fs "
+        "= %.4f GCUPS; %s; GPU parity on the sample: %s (%d walks)"
+        % (cells, el, threads, len(batches), cells / el / 1e9, cells1, el1, cells1 / el1 / 1e9, model, parity,
+           sum(len(v) for v in o_hits.values())))
     return {"value": round(cells / el / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
-            "sample": "%d reads (every %d-th in reader order) x all %d targets of the same workload, "
-                      "%.3g cells, oracle restatement of the reference OpenCL-CPU schedule (8x8 blocks, 2x2 "
-                      "work-items of 4x4 cells, full FP32 matrices) + host walk, OpenMP"
-                      % (len(sample), stride, len(t), cells),
-            "seconds": round(el, 2), "gpu_parity_on_sample": parity}
+            "sample": "%d reads (every %d-th in reader order) x all %d targets of the same workload, %.3g cells, in "
+                      "%d batches sized as the reference sizes them for a %g GiB CPU device at "
+                      "maximum_memory_usage %s; oracle restatement of the reference OpenCL-CPU schedule (8x8 "
+                      "blocks, 2x2 work-items of 4x4 cells, full FP32 matrices) + host walk, OpenMP"
+                      % (len(sample), stride, len(t), cells, len(batches), device_gib,
+                         settings.maximum_memory_usage),
+            "seconds": round(el, 2), "nproc": nproc, "cpu_model": model, "omp_num_threads_env": omp_env,
+            "one_core": {"value": round(cells1 / el1 / 1e9, 4), "cells": cells1, "seconds": round(el1, 2),
+                         "sample": "the first %d reads of the sample x all targets, one batch" % len(one)},
+            "gpu_parity_on_sample": parity,
+            "reference_logged": REFERENCE_CPU_LOGGED}
 
 
 if __name__ == "__main__":

@@ -392,10 +392,14 @@ long oracle_sched_batch(const char *seqs, int N, int X, const char *tgts, int T,
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
-    /* SmithWatermanOcl.py:371-383: matrix filled with 0, I/J with -1e10 */
-    for (size_t i = 0; i < npair * S.mstride; i++) S.H[i] = 0.0f;
-    if (p->affine)
-        for (size_t i = 0; i < npair * S.mstride; i++) { S.I[i] = -1E10f; S.J[i] = -1E10f; }
+    /* SmithWatermanOcl.py:371-383: matrix filled with 0, I/J with -1E10 (in parallel,
+     * as the runtime's buffer fill is) */
+    const long total_cells = (long)(npair * S.mstride);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < total_cells; i++) {
+        S.H[i] = 0.0f;
+        if (p->affine) { S.I[i] = -1E10f; S.J[i] = -1E10f; }
+    }
 
     /* SmithWaterman._calculate_score, SmithWaterman.py:592-618 */
     int xb = S.XB, yb = S.YB;
