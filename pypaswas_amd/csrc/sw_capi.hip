@@ -2253,8 +2253,15 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const
     static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
     const Encoded &E = P->E;
     const int32_t nq = P->nq;
+    // residue characters by code, and their lower-case forms for the match test
+    // of the alignment row (SmithWaterman.py:794-799 compares lower-cased)
+    char chr[256], low[256];
+    for (int c = 0; c < 256; ++c) {
+        chr[c] = c < (int)E.chars.size() ? (char)E.chars[c] : '?';
+        low[c] = (char)std::tolower((unsigned char)chr[c]);
+    }
     // records of pairs [b, e) of outs, starting at sp0 / op0 in the result
-    auto fill = [R, &outs, &E, nq](size_t b, size_t e, int64_t sp0, int64_t op0) {
+    auto fill = [R, &outs, &E, nq, &chr, &low](size_t b, size_t e, int64_t sp0, int64_t op0) {
         for (size_t i = b; i < e; ++i) {
             const PairOut &o = outs[i];
             if (o.nsp > 0) std::memcpy(&R->sps[sp0], o.sps, (size_t)o.nsp * sizeof(SpRec));
@@ -2280,23 +2287,23 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const
                 if (hr.shown && hr.length > 0) {
                     const SeqRef qr = E.refs[hr.query], tr = E.refs[nq + hr.target];
                     const uint8_t *qc = E.codes.data() + qr.off, *tc = E.codes.data() + tr.off;
-                    const char *op = o.ops + hr.ops_offset;
+                    const char *op = o.ops + hr.ops_offset + hr.length - 1;
                     char *qrow = R->aligned.data() + 3 * h.ops_offset, *arow = qrow + hr.length, *trow = arow + hr.length;
                     int x = hr.s_start, y = hr.t_start;
                     for (int c = 0; c < hr.length; ++c) {
-                        const char v = op[hr.length - 1 - c];
+                        const char v = op[-c];
                         if (v == 'D') {
-                            const char a = (char)E.chars[qc[x++]], t = (char)E.chars[tc[y++]];
-                            qrow[c] = a;
-                            trow[c] = t;
-                            arow[c] = std::tolower((unsigned char)a) == std::tolower((unsigned char)t) ? '|' : '.';
+                            const uint8_t a = qc[x++], t = tc[y++];
+                            qrow[c] = chr[a];
+                            trow[c] = chr[t];
+                            arow[c] = low[a] == low[t] ? '|' : '.';
                         } else if (v == 'L') {
-                            qrow[c] = (char)E.chars[qc[x++]];
+                            qrow[c] = chr[qc[x++]];
                             trow[c] = '-';
                             arow[c] = ' ';
                         } else {
                             qrow[c] = '-';
-                            trow[c] = (char)E.chars[tc[y++]];
+                            trow[c] = chr[tc[y++]];
                             arow[c] = ' ';
                         }
                     }
@@ -2306,39 +2313,31 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const
             op0 += o.nops;
         }
     };
-    // large results are filled by a few host threads, each a contiguous range
-    // of pairs at its precomputed offsets (the same records as one thread);
-    // the fill costs ~30 ns per pair plus ~0.1 ns per byte, a thread start
-    // ~20 us
-    const double bytes = (double)nops + (double)nsp * (sizeof(SpRec) + sizeof(pw_hit));
-    const double est_us = 0.03 * (double)outs.size() + 1e-4 * bytes;
-    const int nth = est_us >= 200.0 ? 4 : (est_us >= 80.0 ? 2 : 1);
+    // large results are filled by the host pool, each thread a contiguous range
+    // of pairs of about equal cost at its precomputed offsets (the same records
+    // as one thread); the fill costs ~30 ns per pair plus ~0.1 ns per record
+    // byte and ~1 ns per alignment column
+    const size_t n = outs.size();
+    std::vector<double> cost(n + 1, 0.0);
+    std::vector<int64_t> sp_at(n + 1, 0), op_at(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        const PairOut &o = outs[i];
+        cost[i + 1] = cost[i] + 300.0 + 11.0 * (double)o.nops + (double)o.nsp * (sizeof(SpRec) + sizeof(pw_hit));
+        sp_at[i + 1] = sp_at[i] + o.nsp;
+        op_at[i + 1] = op_at[i] + o.nops;
+    }
+    const double total = cost[n];  // units of 0.1 ns
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(HostPool::get().size(), (int64_t)(total / 5e5)));
     if (nth == 1) {
-        fill(0, outs.size(), 0, 0);
+        fill(0, n, 0, 0);
     } else {
-        std::vector<std::thread> th;
-        size_t b = 0;
-        int64_t sp0 = 0, op0 = 0;
-        // ranges of equal estimated cost, in units of 0.1 ns (a pair: 300)
-        const double total = 300.0 * (double)outs.size() + bytes;
-        double acc = 0;
-        for (int t = 0; t < nth; ++t) {
-            size_t e = b;
-            int64_t sp1 = sp0, op1 = op0;
-            const double goal = total * (t + 1) / nth;
-            while (e < outs.size() && (t == nth - 1 || acc < goal)) {
-                acc += 300.0 + (double)outs[e].nops + (double)outs[e].nsp * (sizeof(SpRec) + sizeof(pw_hit));
-                sp1 += outs[e].nsp;
-                op1 += outs[e].nops;
-                ++e;
-            }
-            if (t == nth - 1) th.emplace_back(fill, b, e, sp0, op0);
-            else if (e > b) th.emplace_back(fill, b, e, sp0, op0);
-            b = e;
-            sp0 = sp1;
-            op0 = op1;
-        }
-        for (std::thread &x : th) x.join();
+        auto cut = [&](int k) {
+            return (size_t)(std::lower_bound(cost.begin(), cost.end(), total * k / nth) - cost.begin());
+        };
+        HostPool::get().run(nth, [&](int k) {
+            const size_t b = k == 0 ? 0 : cut(k), e = k == nth - 1 ? n : cut(k + 1);
+            if (e > b) fill(b, e, sp_at[b], op_at[b]);
+        });
     }
     // the reference keeps at most sp_cap starting points per pair of a batch in
     // total (SmithWaterman.py:683-685); report how many a cap would drop

@@ -42,6 +42,20 @@ __device__ __forceinline__ float shr1(float old_lane0, float v) {
         __builtin_amdgcn_update_dpp(__float_as_int(old_lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+__device__ __forceinline__ int shr1i(int old_lane0, int v) {
+    return __builtin_amdgcn_update_dpp(old_lane0, v, 0x138, 0xf, 0xf, false);
+}
+
+// DPP wave_shl:1 — lane k receives lane k+1's v; lane 63 keeps old_lane63.
+__device__ __forceinline__ float shl1(float old_lane63, float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old_lane63), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ int shl1i(int old_lane63, int v) {
+    return __builtin_amdgcn_update_dpp(old_lane63, v, 0x130, 0xf, 0xf, false);
+}
+
 __device__ __forceinline__ float fmax3(float a, float b, float c) { return __builtin_fmaxf(__builtin_fmaxf(a, b), c); }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -147,10 +161,9 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     if (s_last > nstripes) s_last = nstripes;
     // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
     const float thr = __builtin_fmaxf(t1, mp);
-    // LDS staging of the stripe boundary rows, HANDOFF columns at a time: the row
-    // above (read by lane 0, one column per step) and this stripe's bottom row
-    // (written by lane 63, one column per step, stored HANDOFF at a time)
-    float4 *stg_in = stg, *stg_out = stg + WAVE;
+    // LDS staging of the row above, HANDOFF columns at a time (lane 0 takes one
+    // column per step, as a broadcast read)
+    float4 *stg_in = stg;
 
     // window of the swept residues for steps [t0, t0 + BWIN): every load is
     // issued before the first LDS store, so the wave waits for memory once
@@ -239,15 +252,22 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         }
         float dgT = 0.0f, dgM = 0.0f;
 
-        // substitution scores of step 0; each step reads those of the next
-        // one, so that the LDS latency overlaps a step of arithmetic
+        // Table offsets of the swept residues: lane k sweeps column t - k at
+        // step t, so its column at step t+1 is lane k-1's at step t (a DPP
+        // shift), and only lane 0 takes a new one, column t+1, from fB.  fB
+        // holds the offsets of 64 consecutive columns (lane l: column base + l,
+        // read from the LDS window every 64 steps) and moves down one lane per
+        // step.  Substitution scores are read one step ahead, so that the LDS
+        // latency overlaps a step of arithmetic.
         load_window(0);
+        int fB = ((int)bwin[(WAVE - 1) + lane]) << lkp;  // columns 0..63
+        int boff = lane == 0 ? fB : (fill << lkp);       // column -lane < 0 for lanes 1..63
         float scn[R];
-        {
-            const int boff = ((int)bwin[(WAVE - 1) - lane]) << lkp;
 #pragma unroll
-            for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
-        }
+        for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+        // lane 63's bottom-row values of the latest steps, the newest in lane 63
+        // (lane L holds the one of the step 63 - L steps back)
+        float oT = 0.0f, oJ = 0.0f, oM = 0.0f;
 
         for (int t = 0; t < nsteps; ++t) {
             const int j = t - lane;  // this lane's column
@@ -255,15 +275,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
 #pragma unroll
             for (int r = 0; r < R; ++r) sc[r] = scn[r];
             const int tn = t + 1;
-            if (tn < nsteps) {
-                if ((tn & (BWIN - 1)) == 0) load_window(tn);
-                const int boff = ((int)bwin[(tn & (BWIN - 1)) + (WAVE - 1) - lane]) << lkp;
-#pragma unroll
-                for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
-            }
-            // input of step t+1 for lane 0: column tn of the row above
+            // input of step t+1 for lane 0: column tn of the row above (read
+            // before the score reads below, so that the wait for it at the end
+            // of the step leaves those in flight)
             float4 inj = make_float4(0.0f, bJ, 0.0f, 0.0f);
-            if (above && tn < nsteps) {
+            if (above) {
                 if ((tn & (HANDOFF - 1)) == 0) {
                     if (links && tn < blen_it)
                         wait_progress(prog + (s - 1), (unsigned)min(tn + HANDOFF, blen_it), abort, wlimit);
@@ -271,6 +287,16 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                 }
                 inj = stg_in[tn & (HANDOFF - 1)];
             }
+            // scores of step t+1 (also at t+1 == nsteps: offsets always lie in the table)
+            if ((tn & (WAVE - 1)) == 0) {
+                if ((tn & (BWIN - 1)) == 0) load_window(tn);
+                fB = ((int)bwin[(tn & (BWIN - 1)) + (WAVE - 1) + lane]) << lkp;
+            } else {
+                fB = shl1i(fB, fB);
+            }
+            boff = shr1i(fB, boff);
+#pragma unroll
+            for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
 
             float tup = upT, jin = upJ;
             uint32_t bits[R];
@@ -424,20 +450,21 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             const float botJ = AFF ? jin : 0.0f;
             const float botM = DIRS ? Dp[R - 1] : 0.0f;  // d of the bottom row: the next stripe's STOP test
             if (below) {
+                // lane 63 at step t finishes column t - 63 of the bottom row; the
+                // values move down one lane per step, so at the end of a
+                // hand-over group lane L holds column c63 - 63 + L
+                oT = shl1(botT, oT);
+                if (AFF) oJ = shl1(botJ, oJ);
+                if (DIRS && AFF) oM = shl1(botM, oM);
                 const int c63 = t - (WAVE - 1);
-                if (c63 >= 0 && c63 < blen_it) {
-                    if (lane == WAVE - 1) stg_out[c63 & (HANDOFF - 1)] = make_float4(botT, botJ, botM, 0.0f);
-                    if ((c63 & (HANDOFF - 1)) == HANDOFF - 1 || c63 == blen_it - 1) {
-                        __syncthreads();
-                        const float4 v = stg_out[lane];
-                        const int c = (c63 & ~(HANDOFF - 1)) + lane;
-                        if (lane < HANDOFF && c <= c63) {
-                            store_scr(&sout[3 * c], v.x);
-                            if (AFF) store_scr(&sout[3 * c + 1], v.y);
-                            if (DIRS && AFF) store_scr(&sout[3 * c + 2], v.z);
-                        }
-                        if (links) publish_progress(prog + s, (unsigned)(c63 + 1));
+                if (c63 >= 0 && ((c63 & (HANDOFF - 1)) == HANDOFF - 1 || c63 == blen_it - 1)) {
+                    const int c = c63 - (WAVE - 1) + lane;
+                    if (c >= (c63 & ~(HANDOFF - 1))) {
+                        store_scr(&sout[3 * c], oT);
+                        if (AFF) store_scr(&sout[3 * c + 1], oJ);
+                        if (DIRS && AFF) store_scr(&sout[3 * c + 2], oM);
                     }
+                    if (links) publish_progress(prog + s, (unsigned)(c63 + 1));
                 }
             }
             // inputs of step t+1
