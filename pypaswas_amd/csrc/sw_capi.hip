@@ -1667,6 +1667,14 @@ int64_t dir_limit(pw_ctx *ctx) {
     return ctx->lim.dir_max > 0 ? ctx->lim.dir_max : (int64_t)(ctx->total_mem / 4);
 }
 
+// Candidate slots an execution may hold: their five per-slot records
+// (candidate, starting point twice, sort key, hit) within a quarter of the
+// direction limit.
+int64_t cand_slot_limit(pw_ctx *ctx) {
+    const int64_t slot_bytes = (int64_t)(sizeof(Cand) + 2 * sizeof(SpRec) + sizeof(uint64_t) + sizeof(HitRec));
+    return std::max<int64_t>(1 << 20, dir_limit(ctx) / 4 / slot_bytes);
+}
+
 // Sizes the work areas of W for an execution of P; every allocation happens
 // here, before the first launch.
 int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
@@ -1983,6 +1991,35 @@ struct TailStore {
     std::vector<std::vector<SpRec>> sps;
     std::vector<std::vector<HitRec>> hits;
     std::vector<std::vector<char>> ops;
+    // copies the records of outs[from, end) out of the staging buffers, one
+    // block per kind, and points those outs at the copies
+    void keep(std::vector<PairOut> &outs, size_t from) {
+        size_t nsp = 0, nops = 0;
+        for (size_t i = from; i < outs.size(); ++i) {
+            nsp += (size_t)outs[i].nsp;
+            nops += (size_t)outs[i].nops;
+        }
+        sps.emplace_back(nsp);
+        hits.emplace_back(nsp);
+        ops.emplace_back(nops);
+        SpRec *ps = sps.back().data();
+        HitRec *ph = hits.back().data();
+        char *po = ops.back().data();
+        for (size_t i = from; i < outs.size(); ++i) {
+            PairOut &o = outs[i];
+            if (o.nsp > 0) {
+                std::memcpy(ps, o.sps, (size_t)o.nsp * sizeof(SpRec));
+                std::memcpy(ph, o.hits, (size_t)o.nsp * sizeof(HitRec));
+            }
+            if (o.nops > 0) std::memcpy(po, o.ops, (size_t)o.nops);
+            o.sps = ps;
+            o.hits = ph;
+            o.ops = po;
+            ps += o.nsp;
+            ph += o.nsp;
+            po += o.nops;
+        }
+    }
 };
 
 int take_maxima_buffer(pw_ctx *ctx, int64_t n, HostResult *R) {
@@ -2058,34 +2095,60 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
 // Tail rounds: pairs that did not fit the work areas (or overflowed their
 // candidate or ops ranges) run again, host-driven, with exact sizes.
 int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum,
-                std::vector<PairTask> pending, std::vector<PairOut> &outs, TailStore &store, double *ms_dir,
+                std::vector<PairTask> &pending, std::vector<PairOut> &outs, TailStore &store, double *ms_dir,
                 double *ms_tb, double *ms_walk) {
-    int rounds = 0;
-    while (!pending.empty()) {
-        if (++rounds > 64) {
+    // Pairs are taken in order from `pending` (a cursor, no copies of the rest),
+    // after the pairs of the previous round that overflowed (`retry`).  Every
+    // round takes at least its first pair (the areas are sized for it); a pair
+    // goes round again only after an overflow, then first and with its exact
+    // candidate count, so the round after completes it.  Two rounds in a row
+    // that complete no pair therefore mean a sizing mistake and end the call.
+    int idle_rounds = 0;
+    const int64_t cand_limit = cand_slot_limit(ctx);
+    size_t cursor = 0;
+    std::vector<PairTask> retry;
+    auto area_of = [&](const PairTask &t) { return dir_area(P->qlen[P->pq[t.pair]], P->tlen[P->pt[t.pair]], t.R); };
+    // exact candidate capacity where the count is known (an overflowed pair)
+    auto exact_cap = [](PairTask &t) {
+        if (t.status == TASK_CAND_OVERFLOW || t.status == TASK_OPS_OVERFLOW) t.cand_cap = std::max(t.ncand, 1);
+        else t.cand_cap = std::max(t.cand_cap, 1);
+    };
+    for (PairTask &t : pending) exact_cap(t);
+    while (cursor < pending.size() || !retry.empty()) {
+        if (idle_rounds >= 2) {
             ctx->err = "traceback tail rounds did not converge";
             return PW_EDEVICE;
         }
-        // exact candidate capacity where the count is known
-        for (PairTask &t : pending) {
-            if (t.status == TASK_CAND_OVERFLOW || t.status == TASK_OPS_OVERFLOW) t.cand_cap = std::max(t.ncand, 1);
-            else t.cand_cap = std::max(t.cand_cap, 1);
-        }
-        // areas large enough for the first pair; then take pairs while they fit
-        const PairTask &f = pending.front();
-        const int64_t farea = dir_area(P->qlen[P->pq[f.pair]], P->tlen[P->pt[f.pair]], f.R);
+        for (PairTask &t : retry) exact_cap(t);
+        const size_t nret = retry.size();
+        auto at = [&](size_t k) -> PairTask & { return k < nret ? retry[k] : pending[cursor + (k - nret)]; };
+        const size_t avail = nret + (pending.size() - cursor);
+        // areas large enough for the first pair, and grown toward what the next
+        // pairs need (up to the limits), so that a large deferred set runs in
+        // few rounds; then take pairs in order while they fit
+        const PairTask &f = at(0);
+        const int64_t farea = area_of(f);
         if (W.dir_ext && farea > W.dir_cap) {
             // a lent area too small for this pair: continue in an own one
             W.dir_ext = nullptr;
             W.dir_ext_cap = 0;
             W.dir_cap = 0;
         }
-        if (!W.dir_ext) {
-            ENSURE(ctx, W.dirs, (size_t)std::max<int64_t>(farea, W.dir_cap));
-            W.dir_cap = (int64_t)W.dirs.cap;
+        int64_t dneed = 0, cneed = 0;
+        for (size_t k = 0; k < avail && (dneed < dir_limit(ctx) || cneed < cand_limit); ++k) {
+            dneed += area_of(at(k));
+            cneed += at(k).cand_cap;
         }
-        if (f.cand_cap > W.cand_cap) {
-            const size_t c = (size_t)f.cand_cap;
+        if (!W.dir_ext) {
+            const int64_t want = std::max<int64_t>({farea, W.dir_cap, std::min(dneed, dir_limit(ctx))});
+            if (want > W.dir_cap) {
+                ENSURE(ctx, W.dirs, (size_t)want);
+                W.dir_cap = (int64_t)W.dirs.cap;
+            }
+        }
+        const int64_t cwant = std::max<int64_t>((int64_t)f.cand_cap, std::min(cneed, cand_limit));
+        if (cwant > W.cand_cap) {
+            const size_t c = (size_t)cwant;
             ENSURE(ctx, W.cands, c * sizeof(Cand));
             ENSURE(ctx, W.spslot, c * sizeof(SpRec));
             ENSURE(ctx, W.keys, c * sizeof(uint64_t));
@@ -2097,34 +2160,38 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
                                             (int64_t)(W.out_sps.cap / sizeof(SpRec)),
                                             (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
         }
-        std::vector<PairTask> chunk, rest;
+        std::vector<PairTask> chunk;
         int64_t dused = 0, cused = 0;
-        for (PairTask t : pending) {
-            const int64_t a = dir_area(P->qlen[P->pq[t.pair]], P->tlen[P->pt[t.pair]], t.R);
-            if (rest.empty() && dused + a <= W.dir_cap && cused + t.cand_cap <= W.cand_cap &&
-                (int64_t)chunk.size() < P->n_pairs) {
-                t.dir_off = dused;
-                t.cand_off = cused;
-                t.status = TASK_OK;
-                t.ncand = 0;
-                t.nsp = 0;
-                t.ops_off = 0;
-                t.nops = 0;
-                t.out_off = 0;
-                dused += a;
-                cused += t.cand_cap;
-                chunk.push_back(t);
-            } else {
-                rest.push_back(t);
-            }
+        const int64_t max_take = std::min<int64_t>(P->n_pairs, W.units_cap);
+        size_t k = 0;
+        for (; k < avail && (int64_t)chunk.size() < max_take; ++k) {
+            PairTask t = at(k);
+            const int64_t a = area_of(t);
+            if (dused + a > W.dir_cap || cused + t.cand_cap > W.cand_cap) break;
+            t.dir_off = dused;
+            t.cand_off = cused;
+            t.status = TASK_OK;
+            t.ncand = 0;
+            t.nsp = 0;
+            t.ops_off = 0;
+            t.nops = 0;
+            t.out_off = 0;
+            dused += a;
+            cused += t.cand_cap;
+            chunk.push_back(t);
         }
-        const int nc = (int)std::min<int64_t>((int64_t)chunk.size(), W.units_cap);
-        for (int k = nc; k < (int)chunk.size(); ++k) rest.push_back(chunk[k]);
-        chunk.resize((size_t)nc);
+        // the taken pairs leave retry and pending
+        if (k >= nret) {
+            cursor += k - nret;
+            retry.clear();
+        } else {
+            retry.erase(retry.begin(), retry.begin() + (std::ptrdiff_t)k);
+        }
+        const int nc = (int)chunk.size();
         std::vector<DirUnit> units((size_t)nc);
-        for (int k = 0; k < nc; ++k) {
-            chunk[k].split = 0;  // tail rounds run each pair's stripes in one unit
-            units[k] = DirUnit{k, -1};
+        for (int u = 0; u < nc; ++u) {
+            chunk[u].split = 0;  // tail rounds run each pair's stripes in one unit
+            units[u] = DirUnit{u, -1};
         }
         Ctl c;
         std::memset(&c, 0, sizeof c);
@@ -2183,18 +2250,14 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         std::vector<PairTask> again;
         int rc = read_tasks(ctx, W, L, nc, got, again);
         if (rc) return rc;
+        idle_rounds = got.empty() ? idle_rounds + 1 : 0;
         // copy this round's records out of the staging buffers the next round reuses
-        for (PairOut &o : got) {
-            store.sps.emplace_back(o.sps, o.sps + o.nsp);
-            store.hits.emplace_back(o.hits, o.hits + o.nsp);
-            store.ops.emplace_back(o.ops, o.ops + o.nops);
-            o.sps = store.sps.back().data();
-            o.hits = store.hits.back().data();
-            o.ops = store.ops.back().data();
-            outs.push_back(o);
-        }
-        for (PairTask &t : again) rest.push_back(t);
-        pending.swap(rest);
+        const size_t first = outs.size();
+        outs.insert(outs.end(), got.begin(), got.end());
+        store.keep(outs, first);
+        // overflowed pairs first: the next round takes them with exact sizes
+        again.insert(again.end(), retry.begin(), retry.end());
+        retry.swap(again);
     }
     return PW_OK;
 }
@@ -2207,7 +2270,8 @@ void adapt_workspace(pw_ctx *ctx, Workspace &W, const Ctl &c, const std::vector<
     for (const PairTask &t : pending)
         if (t.status == TASK_CAND_OVERFLOW) maxc = std::max(maxc, t.ncand);
     W.cand0 = std::min<int32_t>(maxc, 1024);
-    W.want_cand = std::max<int64_t>({W.want_cand, (int64_t)c.cand_used, (int64_t)c.ngated * W.cand0});
+    W.want_cand = std::min(std::max<int64_t>({W.want_cand, (int64_t)c.cand_used, (int64_t)c.ngated * W.cand0}),
+                           cand_slot_limit(ctx));
     W.want_ops = std::max<int64_t>(W.want_ops, (int64_t)c.ops_used);
 }
 
@@ -2497,14 +2561,7 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
     t0 = std::chrono::steady_clock::now();
     if (!pending.empty()) {
         // the tail rounds reuse the staging buffers: keep copies of what they hold
-        for (PairOut &o : F.outs) {
-            F.store.sps.emplace_back(o.sps, o.sps + o.nsp);
-            F.store.hits.emplace_back(o.hits, o.hits + o.nsp);
-            F.store.ops.emplace_back(o.ops, o.ops + o.nops);
-            o.sps = F.store.sps.back().data();
-            o.hits = F.store.hits.back().data();
-            o.ops = F.store.ops.back().data();
-        }
+        F.store.keep(F.outs, 0);
         adapt_workspace(ctx, W, c, pending);
         rc = tail_rounds(ctx, P, W, L, X.g, X.minimum, pending, F.outs, F.store, &F.ms_dir, &F.ms_tb, &F.ms_walk);
         if (rc) return rc;

@@ -54,6 +54,22 @@ def _mutate_indel(rng, s, identity, indel_rate, alphabet, p=None):
     return "".join(out)
 
 
+def _mutate_indel_np(rng, s, identity, indel_rate, alphabet, p=None):
+    """_mutate_indel's per-position rule in array form: each residue is deleted
+    with probability indel_rate / 2, gets a random residue inserted before it
+    with probability indel_rate / 2, and (when kept) is replaced by a random one
+    with probability 1 - identity."""
+    a = np.frombuffer(s.encode(), dtype=np.uint8)
+    n = len(a)
+    alph = np.frombuffer("".join(alphabet).encode(), dtype=np.uint8)
+    r = rng.random(n)
+    keep = r >= indel_rate / 2
+    ins = keep & (r < indel_rate)
+    cur = np.where(rng.random(n) < identity, a, alph[rng.choice(len(alph), size=n, p=p)])
+    pieces = np.stack([alph[rng.choice(len(alph), size=n, p=p)], cur], axis=1)
+    return pieces[np.stack([ins, keep], axis=1)].tobytes().decode()
+
+
 def config1(seed: int = 1):
     """64 reads of 150 nt (uniform ACGT), each with a 30-nt piece of one of 16
     adapters (30-60 nt) written over a uniform offset (the read stays 150 nt),
@@ -104,7 +120,7 @@ def config3(seed: int = 3, n_queries: int = 2000, n_targets: int = 20000, famili
 
     def member(prefix, i):
         f = int(rng.integers(0, families))
-        s = _mutate_indel(rng, ancestors[f], ident, 0.02, AMINO, AMINO_FREQ)
+        s = _mutate_indel_np(rng, ancestors[f], ident, 0.02, AMINO, AMINO_FREQ)
         s = s[:5000] if len(s) >= 30 else s + "".join(rng.choice(AMINO, size=30 - len(s), p=AMINO_FREQ))
         return ("%s%05d_f%03d" % (prefix, i, f), s)
 
