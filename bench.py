@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in shape cost model")
     ap.add_argument("--serving-jobs", type=int, default=3000, help="requests in the serving run")
     ap.add_argument("--serving-qps", type=float, default=2000.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serving-sweep", default="8000,20000",
+                    help="further Poisson rates of the serving sweep (comma-separated; saturation is added)")
     ap.add_argument("--serving-lanes", type=int, default=4, help="stream lanes of the server")
     ap.add_argument("--serving-arena-mib", type=int, default=16384, help="direction arena of the server")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving measurement")
@@ -102,8 +104,9 @@ def main():
 
     if args.serving_only:
         dev = Device(0)
-        print(json.dumps(serving_run(dev, args.serving_jobs, args.serving_qps, args.serving_lanes,
-                                     args.serving_arena_mib)), flush=True)
+        runs = [serving_run(dev, args.serving_jobs, qps, args.serving_lanes, args.serving_arena_mib)
+                for qps in [args.serving_qps] + [float(v) for v in args.serving_sweep.split(",") if v] + [None]]
+        print(json.dumps(runs), flush=True)
         return
 
     t0 = time.time()
@@ -204,8 +207,14 @@ def main():
                            args.cpu_device_gib)
 
     serving = None
+    sweep = None
     if not args.no_serving:
         serving = serving_run(dev, args.serving_jobs, args.serving_qps, args.serving_lanes, args.serving_arena_mib)
+        # offered load from the headline rate up to saturation (every request at once)
+        sweep = []
+        for qps in [float(v) for v in args.serving_sweep.split(",") if v] + [None]:
+            r = serving_run(dev, args.serving_jobs, qps, args.serving_lanes, args.serving_arena_mib)
+            sweep.append({k: r[k] for k in r if k not in ("cells", "hits")})
 
     protein = None
     if not args.no_protein:  # last: it changes the device's scoring constants
@@ -230,6 +239,7 @@ def main():
         "latency_ms": ({"p50": serving["p50_ms"], "p99": serving["p99_ms"], "workload": "serving mix"}
                        if serving else None),
         "serving": serving,
+        "serving_sweep": sweep,
         "traceback_pairs": last.n_traceback_pairs,
         "hits": int(last.hits["shown"].sum()) if len(last.hits) else 0,
         "autotune": tuning,
@@ -284,35 +294,47 @@ def engine_window(dev, q, t, settings, score, qp, tp, pq, pt, mss, same, minimum
     return out
 
 
+_SERVING_JOBS = {}
+
+
 def serving_run(dev, n_jobs, qps, lanes, arena_mib, seed=5):
     """Serving mix through the continuous-batching server at a fixed Poisson
-    rate: 70% reads (100-300 nt) x 8-adapter panels (+RC), 25% protein pairs
-    (100-2,000, BLOSUM62 -11/-1), 5% long DNA pairs (5,000-20,000 nt, +RC).
-    Latency = completion poll - arrival.  A short untimed burst first sizes
-    the lanes' work areas."""
+    rate (qps None: every request at time 0, the saturation point): 70% reads
+    (100-300 nt) x 8-adapter panels (+RC), 25% protein pairs (100-2,000,
+    BLOSUM62 -11/-1), 5% long DNA pairs (5,000-20,000 nt, +RC).  Latency =
+    completion poll - arrival.  A short untimed burst first sizes the lanes'
+    work areas."""
     from pypaswas_amd import workloads as W
     from pypaswas_amd.serving import Job, Server, poisson_arrivals, replay
     t0 = time.time()
     schemes = W.serving_schemes()
-    raw = W.serving_jobs(seed, n_jobs)
-    jobs = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t) for k, q, t in raw]
-    warm = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t)
-            for k, q, t in W.serving_jobs(seed + 1000, 400)]
-    log("serving: %d jobs built in %.1fs" % (n_jobs, time.time() - t0))
+    key = (seed, n_jobs)
+    if key not in _SERVING_JOBS:  # the same requests for every rate of a sweep
+        raw = W.serving_jobs(seed, n_jobs)
+        jobs = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t)
+                for k, q, t in raw]
+        warm = [Job.all_vs_all(W.SERVING_KINDS.index(k), schemes[W.SERVING_KINDS.index(k)], q, t)
+                for k, q, t in W.serving_jobs(seed + 1000, 400)]
+        _SERVING_JOBS[key] = (raw, jobs, warm)
+        log("serving: %d jobs built in %.1fs" % (n_jobs, time.time() - t0))
+    raw, jobs, warm = _SERVING_JOBS[key]
     srv = Server(dev, lanes=lanes, direction_bytes=int(arena_mib) << 20)
     try:
         for i, sc in enumerate(schemes):
             srv.set_scheme(i, sc)
         replay(srv, warm)
         st0 = srv.stats()
-        arrivals = poisson_arrivals(n_jobs, qps, seed)
-        results, lat, wall = replay(srv, jobs, arrivals)
+        arrivals = poisson_arrivals(n_jobs, qps, seed) if qps else None
+        watch = {}
+        results, lat, wall = replay(srv, jobs, arrivals, watch)
         st = srv.stats()
     finally:
         srv.close()
     cells = float(sum(j.cells for j in jobs))
     kinds = np.array([k for k, _, _ in raw])
-    out = {"jobs": n_jobs, "qps": qps, "lanes": lanes, "arena_mib": arena_mib, "wall_s": round(wall, 3),
+    out = {"jobs": n_jobs, "qps": qps if qps else "all at once", "lanes": lanes, "arena_mib": arena_mib,
+           "wall_s": round(wall, 3), "jobs_per_s": round(n_jobs / wall, 1),
+           "peak_queued": watch.get("peak_queued", 0), "peak_running": watch.get("peak_running", 0),
            "gcups": round(cells / wall / 1e9, 2), "cells": cells,
            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
@@ -330,8 +352,9 @@ def serving_run(dev, n_jobs, qps, lanes, arena_mib, seed=5):
         if m.any():
             out["p50_ms_" + k] = round(float(np.percentile(lat[m], 50)) * 1e3, 3)
             out["p99_ms_" + k] = round(float(np.percentile(lat[m], 99)) * 1e3, 3)
-    log("serving: %d jobs at %.0f/s in %.2fs, %.1f GCUPS, p50 %.2f ms, p99 %.2f ms, %d batches" %
-        (n_jobs, qps, wall, out["gcups"], out["p50_ms"], out["p99_ms"], out["batches"]))
+    log("serving: %d jobs at %s/s in %.2fs, %.1f GCUPS, p50 %.2f ms, p99 %.2f ms, %d batches, %.2f jobs per "
+        "batch, peak queue %d" % (n_jobs, qps if qps else "inf", wall, out["gcups"], out["p50_ms"], out["p99_ms"],
+                                  out["batches"], out["jobs_per_batch"], out["peak_queued"]))
     return out
 
 

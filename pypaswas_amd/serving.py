@@ -131,11 +131,14 @@ def poisson_arrivals(n: int, qps: float, seed: int = 5) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / qps, size=n))
 
 
-def replay(server: Server, jobs: Sequence[Job], arrivals: Optional[np.ndarray] = None):
+def replay(server: Server, jobs: Sequence[Job], arrivals: Optional[np.ndarray] = None,
+           watch: Optional[Dict[str, int]] = None):
     """Submits each job at its arrival time (all at once when arrivals is None),
     polls until every job completed.  Returns (results, latencies_s, wall_s)
     with results and latencies in job order; a job's latency runs from its
-    arrival time to the poll that returned it."""
+    arrival time to the poll that returned it.  With a dict `watch`, the
+    largest queued and running job counts the server reported while the jobs
+    ran are kept in it ("peak_queued", "peak_running")."""
     n = len(jobs)
     arr = np.zeros(n) if arrivals is None else np.asarray(arrivals, dtype=np.float64)
     results: List[Optional[AlignResult]] = [None] * n
@@ -150,6 +153,10 @@ def replay(server: Server, jobs: Sequence[Job], arrivals: Optional[np.ndarray] =
             i += 1
         done = server.poll(block=(i >= n))
         t = time.perf_counter() - t0
+        if watch is not None:
+            st = server.stats()
+            watch["peak_queued"] = max(watch.get("peak_queued", 0), st["queued"])
+            watch["peak_running"] = max(watch.get("peak_running", 0), st["running"])
         for jid in done:
             k = pending.pop(jid)
             lat[k] = t - arr[k]
