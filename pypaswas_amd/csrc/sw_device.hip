@@ -268,12 +268,15 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         // lane 63's bottom-row values of the latest steps, the newest in lane 63
         // (lane L holds the one of the step 63 - L steps back)
         float oT = 0.0f, oJ = 0.0f, oM = 0.0f;
+        // the column after which lane 63's next hand-over group is complete: the
+        // last of each HANDOFF columns, and the row's last column (none for an
+        // empty row)
+        int next_ho = blen_it > 0 ? min(HANDOFF - 1, blen_it - 1) : INT32_MIN;
 
-        for (int t = 0; t < nsteps; ++t) {
+        // one step, scores in sc, the next step's read into nx; two steps per
+        // loop iteration alternate the two register sets
+        auto step = [&](const int t, float (&sc)[R], float (&nx)[R]) {
             const int j = t - lane;  // this lane's column
-            float sc[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) sc[r] = scn[r];
             const int tn = t + 1;
             // input of step t+1 for lane 0: column tn of the row above (read
             // before the score reads below, so that the wait for it at the end
@@ -296,7 +299,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             }
             boff = shr1i(fB, boff);
 #pragma unroll
-            for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+            for (int r = 0; r < R; ++r) nx[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
 
             float tup = upT, jin = upJ;
             uint32_t bits[R];
@@ -457,7 +460,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                 if (AFF) oJ = shl1(botJ, oJ);
                 if (DIRS && AFF) oM = shl1(botM, oM);
                 const int c63 = t - (WAVE - 1);
-                if (c63 >= 0 && ((c63 & (HANDOFF - 1)) == HANDOFF - 1 || c63 == blen_it - 1)) {
+                if (c63 == next_ho) {
                     const int c = c63 - (WAVE - 1) + lane;
                     if (c >= (c63 & ~(HANDOFF - 1))) {
                         store_scr(&sout[3 * c], oT);
@@ -465,6 +468,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         if (DIRS && AFF) store_scr(&sout[3 * c + 2], oM);
                     }
                     if (links) publish_progress(prog + s, (unsigned)(c63 + 1));
+                    next_ho = min(next_ho + HANDOFF, blen_it - 1);
                 }
             }
             // inputs of step t+1
@@ -473,8 +477,25 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             upT = shr1(inj.x, botT);
             if (AFF) upJ = shr1(inj.y, botJ);
             if (DIRS && AFF) upM = shr1(inj.z, botM);
+        };
+        float scm[R];
+        if constexpr (DIRS) {
+            // (two steps per iteration here raise the direction pass's register
+            // use past 200 VGPRs, so its steps copy the next scores instead)
+            for (int t = 0; t < nsteps; ++t) {
+                step(t, scn, scm);
+#pragma unroll
+                for (int r = 0; r < R; ++r) scn[r] = scm[r];
+            }
+            flush_cands();
+        } else {
+            int t = 0;
+            for (; t + 1 < nsteps; t += 2) {
+                step(t, scn, scm);
+                step(t + 1, scm, scn);
+            }
+            if (t < nsteps) step(t, scn, scm);
         }
-        if (DIRS) flush_cands();
     }
     return rmax;
 }
