@@ -303,7 +303,6 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
 
             float tup = upT, jin = upJ;
             uint32_t bits[R];
-            bool cand[R];  // direction pass: the cell is a candidate start
             if (!DIRS) {
                 // diagonal sums T[i-1][j-1] + s of all of this step's cells
                 // first, so that each row's T is replaced in place
@@ -351,7 +350,6 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         const float dG = d + G;
                         const float I = __builtin_fmaxf(Ip[r] + E, Mp[r]);  // Mp holds d + G
                         const float T = fmax3(d, I, jin);
-                        rmax = __builtin_fmaxf(rmax, T);
                         // default_main_cpu.cl:283-345: source = M, I, J in that
                         // priority among equal positive values, none when 0; a
                         // main-matrix cell stops on a zero diagonal predecessor
@@ -365,8 +363,6 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         asm volatile("" : "+v"(main), "+v"(other));
                         const bool isM = T == d, pos = T > 0.0f;
                         bits[r] = pos ? (isM ? main : other) : 0u;
-                        // M >= 0, so "M >= thr or M <= -thr" is M >= thr
-                        cand[r] = pos && isM && d >= thr;
                         const float jout = fmax3(jin + E, dG, 0.0f);
                         td = Tp[r];
                         md = Dp[r];
@@ -395,8 +391,6 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         const uint32_t b = isl ? 3u : (isu ? 2u : dgc);
                         const bool pos = cur > 0.0f;
                         bits[r] = pos ? b : 0u;
-                        cand[r] = pos && !isl && !isu && cur >= thr;  // cur >= 0 (as M above)
-                        rmax = __builtin_fmaxf(rmax, cur);
                         td = Tp[r];
                         Tp[r] = cur;
                         tup = cur;
@@ -422,14 +416,21 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                         reinterpret_cast<uint32_t *>(dst)[w] = word;
                     }
                 }
-                bool any = false;
+                // A candidate start is a main-matrix (diagonal) cell whose M
+                // reaches the threshold (M >= 0, so "M >= thr or M <= -thr" is
+                // M >= thr; for the linear form M is the cell value); then the
+                // step's largest T reaches it too, which is the cheap test of
+                // every step; the cells are tested only when it passes.
+                float smax = Tp[0];
 #pragma unroll
-                for (int r = 0; r < R; ++r) any |= cand[r];
-                if (__builtin_amdgcn_ballot_w64(any && j >= 0 && j < B.len) != 0) {
+                for (int r = 1; r < R; ++r) smax = __builtin_fmaxf(smax, Tp[r]);
+                rmax = __builtin_fmaxf(rmax, smax);
+                if (__builtin_amdgcn_ballot_w64(smax >= thr && j >= 0 && j < B.len) != 0) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int x = s * rows + lane * R + r;
-                        const bool c = cand[r] && x < A.len && j >= 0 && j < B.len;
+                        const float m = AFF ? Dp[r] : Tp[r];
+                        const bool c = (bits[r] & 3u) == DB_MAIN && m >= thr && x < A.len && j >= 0 && j < B.len;
                         const uint64_t mask = __builtin_amdgcn_ballot_w64(c);
                         if (mask == 0) continue;
                         if (nbuf > CAND_BUF - WAVE) flush_cands();
