@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-batch launch")
     ap.add_argument("--cpu-reads", type=int, default=1600, help="reads in the CPU-baseline sample")
-    ap.add_argument("--cpu-one-core-reads", type=int, default=120, help="reads of the one-thread CPU run")
+    ap.add_argument("--cpu-one-core-reads", type=int, default=60, help="reads of the one-thread CPU run")
     ap.add_argument("--cpu-device-gib", type=float, default=8.0,
                     help="MAX_MEM_ALLOC_SIZE of the CPU device the reference's batching is sized for")
     ap.add_argument("--no-protein", action="store_true", help="skip the uneven-length protein comparison")
