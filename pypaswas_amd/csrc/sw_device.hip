@@ -398,23 +398,29 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                 }
             }
             if (DIRS) {
-                // one R-byte word per lane and step, tile-major
-                uint8_t *dst = dir + ((int64_t)(s * (int64_t)nsteps + t) * WAVE + lane) * R;
+                // R nibbles per lane and step, tile-major: nibble
+                // ((s * nsteps + t) * 64 + lane) * R + r, two per byte (nibble r
+                // at bit 4r of the lane's word)
+                // (byte address: 32R bytes per step, lane * R / 2 into it; for
+                // R = 1 the even lane's byte)
+                uint8_t *const dbyte = dir + (int64_t)(s * (int64_t)nsteps + t) * (WAVE / 2 * R) + ((lane * R) >> 1);
+                uint32_t word = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) word |= bits[r] << (4 * r);
                 if (R == 1) {
-                    dst[0] = (uint8_t)bits[0];
+                    // lanes 2k and 2k+1 share a byte: the even lane stores both
+                    const uint32_t other = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)word, 0xB1, 0xf, 0xf, false);
+                    if ((lane & 1) == 0) *dbyte = (uint8_t)(word | (other << 4));
                 } else if (R == 2) {
-                    *reinterpret_cast<uint16_t *>(dst) = (uint16_t)(bits[0] | (bits[1 % R] << 8));
+                    *dbyte = (uint8_t)word;
                 } else if (R == 6) {
-#pragma unroll
-                    for (int w = 0; w < 3; ++w)
-                        reinterpret_cast<uint16_t *>(dst)[w] = (uint16_t)(bits[(2 * w) % R] | (bits[(2 * w + 1) % R] << 8));
+                    dbyte[0] = (uint8_t)word;
+                    dbyte[1] = (uint8_t)(word >> 8);
+                    dbyte[2] = (uint8_t)(word >> 16);
+                } else if (R == 4) {
+                    *reinterpret_cast<uint16_t *>(dbyte) = (uint16_t)word;
                 } else {
-#pragma unroll
-                    for (int w = 0; w < R / 4; ++w) {
-                        const uint32_t word = bits[4 * w] | (bits[(4 * w + 1) % R] << 8) |
-                                              (bits[(4 * w + 2) % R] << 16) | (bits[(4 * w + 3) % R] << 24);
-                        reinterpret_cast<uint32_t *>(dst)[w] = word;
-                    }
+                    *reinterpret_cast<uint32_t *>(dbyte) = word;
                 }
                 // A candidate start is a main-matrix (diagonal) cell whose M
                 // reaches the threshold (M >= 0, so "M >= thr or M <= -thr" is
@@ -883,7 +889,8 @@ __device__ __host__ __forceinline__ int dir_residues_bound(int qlen) {
 __device__ __host__ __forceinline__ int64_t dir_area(int qlen, int tlen, int R) {
     const int rows = WAVE * R;
     const int64_t stripes = qlen > 0 ? (qlen + rows - 1) / rows : 1;
-    return (stripes * (int64_t)(tlen + WAVE - 1) * WAVE * R + 15) & ~(int64_t)15;
+    // one nibble per cell (64R, an even count, per step), 16-byte multiples
+    return (stripes * (int64_t)(tlen + WAVE - 1) * WAVE * R / 2 + 15) & ~(int64_t)15;
 }
 
 // A pair is split into one work unit per stripe when it has several stripes
@@ -1050,11 +1057,22 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
 // pair for pairs whose paths may be long.
 // ---------------------------------------------------------------------------
 
+// Nibble index (in the whole direction area, two per byte) of cell (x, y) of a
+// task, and reading / or-ing / replacing it.
 __device__ __forceinline__ int64_t cell_offset(const PairTask &it, int tlen, int x, int y) {
     const int R = it.R, rows = WAVE * R;
     const int s = x / rows, rem = x - s * rows, k = rem / R, r = rem - k * R;
     const int64_t nsteps = (int64_t)tlen + WAVE - 1;
-    return it.dir_off + (((int64_t)s * nsteps + (y + k)) * WAVE + k) * R + r;
+    return 2 * it.dir_off + (((int64_t)s * nsteps + (y + k)) * WAVE + k) * R + r;
+}
+
+__device__ __forceinline__ uint32_t nib_get(const uint8_t *p, int64_t n) {
+    return ((uint32_t)p[n >> 1] >> ((uint32_t)(n & 1) << 2)) & 15u;
+}
+
+__device__ __forceinline__ void nib_put(uint8_t *p, int64_t n, uint32_t v) {
+    const uint32_t sh = (uint32_t)(n & 1) << 2;
+    p[n >> 1] = (uint8_t)(((uint32_t)p[n >> 1] & ~(15u << sh)) | ((v & 15u) << sh));
 }
 
 // Canonical emission order (larger key = emitted first): the serial reading
@@ -1114,7 +1132,7 @@ __device__ __forceinline__ bool long_path_pair(const SeqRef *__restrict__ seqs, 
     return long_path_lengths(seqs[it.q_seq].len, seqs[it.t_seq].len);
 }
 constexpr int WALK_QWIN = WAVE * 8;                      // 64R rows, R <= 8
-constexpr int WALK_TWIN = WALK_WIN / WAVE + WAVE;        // steps of an R = 1 window plus the skew
+constexpr int WALK_TWIN = 2 * WALK_WIN / WAVE + WAVE;    // steps of an R = 1 window (2 cells per byte) plus the skew
 constexpr int WALK_LDS = WALK_WIN + WALK_QWIN + WALK_TWIN;
 
 struct PathWindow {
@@ -1148,14 +1166,14 @@ __device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restr
     const int t = y + k;
     if (s != w.s || t < w.t0 || t >= w.t0 + w.nt) {
         const int lane = threadIdx.x & 63;
-        const int B = WALK_WIN / rows;
+        const int B = 2 * WALK_WIN / rows;  // steps of rows nibbles
         const int t0 = max(0, t - B + 1);
         const int nt = (int)min((int64_t)B, w.nsteps - t0);
         // lane 0's earlier stores (marks) reach memory before the wave reads
         // it back, and every lane's reads of the old window are done
         __syncthreads();
-        const uint8_t *src = dirs + w.dir_off + ((int64_t)s * w.nsteps + t0) * rows;
-        const int bytes = nt * rows;
+        const uint8_t *src = dirs + w.dir_off + ((int64_t)s * w.nsteps + t0) * rows / 2;
+        const int bytes = nt * rows / 2;
         if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
             const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
             uint4 *d4 = reinterpret_cast<uint4 *>(w.blk);
@@ -1181,20 +1199,26 @@ __device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restr
     return ((t - w.t0) * WAVE + k) * R + r;
 }
 
-// The direction byte of cell (x, y), wave-uniform.
+// The direction nibble of cell (x, y), wave-uniform; idx is its nibble index
+// in the window.
 template <bool RES>
 __device__ __forceinline__ uint32_t window_read(PathWindow &w, const uint8_t *dirs, const uint8_t *res, int x, int y,
                                                 int &idx) {
     idx = window_cell<RES>(w, dirs, res, x, y);
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx]);
+    const uint32_t byte = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx >> 1]);
+    return (byte >> ((uint32_t)(idx & 1) << 2)) & 15u;
 }
 
-// Stores byte b for the cell at LDS index idx (its global offset derived from
-// the window's stripe and steps): LDS by every lane, memory by lane 0.
+// Replaces the nibble at window index idx with b (its byte's global offset
+// derived from the window's stripe and steps): LDS by every lane, memory by
+// lane 0.
 __device__ __forceinline__ void window_write(PathWindow &w, uint8_t *__restrict__ dirs, int idx, uint32_t b) {
-    w.blk[idx] = (uint8_t)b;
+    const uint32_t sh = (uint32_t)(idx & 1) << 2;
+    const uint32_t old = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx >> 1]);
+    const uint8_t nb = (uint8_t)((old & ~(15u << sh)) | ((b & 15u) << sh));
+    w.blk[idx >> 1] = nb;
     if ((threadIdx.x & 63) == 0)
-        dirs[w.dir_off + ((int64_t)w.s * w.nsteps + w.t0) * (WAVE * w.R) + idx] = (uint8_t)b;
+        dirs[w.dir_off + ((int64_t)w.s * w.nsteps + w.t0) * (WAVE * w.R) / 2 + (idx >> 1)] = nb;
 }
 
 // default_main_cpu.cl:389-604 restated per pair: candidates in canonical
@@ -1230,7 +1254,7 @@ __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs,
             int x = c[i].x & ~(CAND_PLAIN | CAND_MARKED), y = c[i].y;
             const float v = c[i].v;
             const int64_t off = cell_offset(it, tlen, x, y);
-            const uint8_t b = dirs[off];
+            const uint32_t b = nib_get(dirs, off);
             float score;
             bool emit = false;
             if (!(b & DB_MARK)) {
@@ -1241,15 +1265,15 @@ __global__ void __launch_bounds__(64) tb_kernel(const SeqRef *__restrict__ seqs,
                     int cx = x, cy = y;
                     for (;;) {
                         const int64_t o = cell_offset(it, tlen, cx, cy);
-                        const uint8_t cb = dirs[o];
-                        dirs[o] = cb | DB_MARK;
+                        const uint32_t cb = nib_get(dirs, o);
+                        nib_put(dirs, o, cb | DB_MARK);
                         const uint32_t code = cb & 3u;
                         if (cb & DB_STOP) break;
                         if (code == 0u) break;
                         if (AFF && code != 1u) break;
                         const int px = cx - (code != 2u ? 1 : 0), py = cy - (code != 3u ? 1 : 0);
                         if (px < 0 || py < 0) break;
-                        if (dirs[cell_offset(it, tlen, px, py)] & DB_MARK) break;
+                        if (nib_get(dirs, cell_offset(it, tlen, px, py)) & DB_MARK) break;
                         cx = px;
                         cy = py;
                     }
@@ -1370,8 +1394,8 @@ __device__ __forceinline__ uint32_t float_order(float f) {
 // the matrix: STOP where a marked cell's predecessor is 0, otherwise the
 // code (linear none = 32; affine keeps direction & 3, so none = 0).
 template <bool AFF>
-__device__ __forceinline__ int eff_dir(uint8_t b) {
-    if (b & DB_WALK) return 96;
+__device__ __forceinline__ int eff_dir(uint32_t b) {
+    if (is_walked(b)) return 96;
     const int code = b & 3;
     if ((b & DB_MARK) && (b & DB_STOP) && (AFF ? code == 1 : code != 0)) return 64;
     if (code == 0) return AFF ? 0 : 32;
@@ -1429,7 +1453,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
             const int s_end = x, t_end = y;
             int s_start = s_end + 1, t_start = t_end + 1;
             int matches = 0, mismatches = 0, len = 0;
-            int direction = eff_dir<AFF>(dirs[cell_offset(it, t.len, x, y)]);
+            int direction = eff_dir<AFF>(nib_get(dirs, cell_offset(it, t.len, x, y)));
             bool show = true;
             if (it.same_id) {
                 direction = 64;
@@ -1440,9 +1464,9 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
             bool term = false;
             while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
                 const int64_t o = cell_offset(it, t.len, x, y);
-                const uint8_t b = dirs[o];
+                const uint32_t b = nib_get(dirs, o);
                 direction = eff_dir<AFF>(b);
-                dirs[o] = b | DB_WALK;
+                nib_put(dirs, o, nib_walked(b));
                 if (direction == 96) {
                     show = false;
                 } else if (direction == 1 || direction == 64) {
@@ -1548,7 +1572,7 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             int s_start = s_end + 1, t_start = t_end + 1;
             int matches = 0, mismatches = 0, len = 0;
             int idx;
-            int direction = eff_dir<AFF>((uint8_t)window_read<true>(w, dirs, res, x, y, idx));
+            int direction = eff_dir<AFF>(window_read<true>(w, dirs, res, x, y, idx));
             bool show = true;
             if (it.same_id) {
                 direction = 64;
@@ -1560,8 +1584,8 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
                 int o;
                 const uint32_t b = window_read<true>(w, dirs, res, x, y, o);
-                direction = eff_dir<AFF>((uint8_t)b);
-                window_write(w, dirs, o, b | DB_WALK);
+                direction = eff_dir<AFF>(b);
+                window_write(w, dirs, o, nib_walked(b));
                 if (direction == 96) {
                     show = false;
                 } else if (direction == 1 || direction == 64) {

@@ -166,12 +166,16 @@ struct HitRec {
     int64_t ops_offset;
 };
 
-// direction byte of the direction pass
+// Direction nibble of the direction pass, two cells per byte (the cell of even
+// index in the low nibble; the candidate flags travel in the candidate record):
 //   bits 1:0  code: 0 none, 1 upper-left / main matrix, 2 upper / I matrix, 3 left / J matrix
 //   bit 2     the predecessor along the code has value 0 (a mark here becomes STOP)
-//   bits 4:3  unused (the candidate flags travel in the candidate record)
-//   bit 5     marked by the traceback
-//   bit 6     consumed by the walk (IN_ALIGNMENT)
-constexpr uint8_t DB_MAIN = 1, DB_STOP = 4, DB_MARK = 32, DB_WALK = 64;
+//   bit 3     marked by the traceback
+// A marked cell always has a code (the marking never reaches a cell of value
+// 0), so "marked with code 0" is free and means consumed by the walk
+// (IN_ALIGNMENT): the walk reads a cell's code before it clears it.
+constexpr uint8_t DB_MAIN = 1, DB_STOP = 4, DB_MARK = 8;
+__host__ __device__ inline uint32_t nib_walked(uint32_t b) { return (b & ~3u) | DB_MARK; }
+__host__ __device__ inline bool is_walked(uint32_t b) { return (b & DB_MARK) && (b & 3u) == 0; }
 
 }  // namespace pw
