@@ -1021,23 +1021,41 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
         int64_t units[3] = {}, dir_worst[3] = {}, dir_largest[3] = {}, dlinks[3] = {}, dprog[3] = {};
     };
     const int caps[3] = {2, 4, 8};
+    // per query and cap: residues per lane and stripes (the divisions of
+    // dir_residues, dir_area and split_stripes, once per query instead of per pair)
+    struct QShape {
+        int32_t R[3], S[3];
+    };
+    std::vector<QShape> qs((size_t)nq);
+    parallel_ranges(nq, 1 << 12, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t q = lo; q < hi; ++q)
+            for (int c = 0; c < 3; ++c) {
+                const int R = dir_residues(qlen[q], caps[c]), rows = WAVE * R;
+                qs[q].R[c] = R;
+                qs[q].S[c] = qlen[q] > 0 ? (qlen[q] + rows - 1) / rows : 1;
+            }
+    });
     std::vector<Acc> acc(16);  // per thread (parallel_ranges runs at most 16)
     parallel_ranges(n_pairs, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
         Acc a;
         for (int64_t i = lo; i < hi; ++i) {
+            const QShape &qh = qs[pq[i]];
             const int ql = qlen[pq[i]], tl = tlen[pt[i]];
             a.cells += (int64_t)ql * tl;
             a.max_t = std::max<int64_t>(a.max_t, tl);
+            const int64_t steps = (int64_t)tl + WAVE - 1;
+            const bool long_cols = tl >= SPLIT_MIN_COLS;
             for (int c = 0; c < 3; ++c) {
-                const int R = dir_residues(ql, caps[c]);
-                const int64_t area = dir_area(ql, tl, R);
+                const int64_t S = qh.S[c];
+                // dir_area(ql, tl, R) and split_stripes(ql, tl, R)
+                const int64_t area = (S * steps * (WAVE / 2) * qh.R[c] + 15) & ~(int64_t)15;
                 a.dir_worst[c] += area;
                 a.dir_largest[c] = std::max(a.dir_largest[c], area);
-                const int ns = split_stripes(ql, tl, R);
-                a.units[c] += ns > 0 ? ns : 1;
-                if (ns > 0) {
-                    a.dlinks[c] += (int64_t)(ns - 1) * link_row_floats(tl);
-                    a.dprog[c] += ns;
+                const bool split = S >= 2 && long_cols;
+                a.units[c] += split ? S : 1;
+                if (split) {
+                    a.dlinks[c] += (S - 1) * link_row_floats(tl);
+                    a.dprog[c] += S;
                 }
             }
         }
