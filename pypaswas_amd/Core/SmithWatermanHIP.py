@@ -28,6 +28,29 @@ def min_start_scores(highest, filter_factor, qlens, tlens):
     return ((float(highest) * shortest) * float(filter_factor)).astype(np.float32)
 
 
+def gap_open(score, settings):
+    """The gap-open constant the kernels take.
+
+    ``kernel_semantics`` CPU (the default) is the OpenCL-CPU kernels' recurrence,
+    which opens an affine gap with G (default_main_cpu.cl:304, 319).  GPU is
+    the scoring recurrence of the GPU-OpenCL and CUDA kernels, which open it
+    with G + E (default_main_gpu.cl:298, 305; default_main.cu:487, 502), the
+    sum formed in FP32 before it is added, as those kernels do.  Their other
+    scoring differences change no value: zeroed fill cells are padding, and a
+    linear cell whose best candidate ties at 0 has value 0 (it takes a
+    direction there, default_main_gpu.cl:140-150, which matters only for a
+    starting point of score 0, possible only with lower_limit_score 0 and
+    filter_factor 0).  Start selection, marking and the walk are the OpenCL-CPU
+    ones in both settings.  Linear gaps are scored the same in both."""
+    semantics = str(getattr(settings, 'kernel_semantics', 'CPU') or 'CPU').upper()
+    if semantics not in ('CPU', 'GPU'):
+        from pypaswas_amd.Core.Exceptions import InvalidOptionException
+        raise InvalidOptionException('kernel_semantics must be CPU or GPU, not {0}'.format(semantics))
+    if semantics == 'GPU' and score.gap_extension is not None:
+        return float(np.float32(score.gap_score) + np.float32(score.gap_extension))
+    return float(score.gap_score)
+
+
 def job_arrays(queries, targets, highest, filter_factor):
     """All-vs-all pair arrays for (id, sequence) records or SWSeqRecords.
 
@@ -64,7 +87,7 @@ class SmithWatermanHIP(SmithWaterman):
         self.mode = mode
         self.dev = device if device is not None else Device(self.device)
         self.last_result = None
-        self.dev.set_scoring(score.to_float32(), ord(score.char_offset), score.gap_score,
+        self.dev.set_scoring(score.to_float32(), ord(score.char_offset), gap_open(score, settings),
                              score.gap_extension if score.gap_extension is not None else 0.0,
                              score.lower_limit_score, score.minimum_score)
 

@@ -76,6 +76,9 @@ _OPTIONS = [
     ]),
     ('Framework', [
         (('--framework',), 'framework', 'Framework', 'language', 'HIP'),
+        (('--kernel_semantics',), 'kernel_semantics', 'Framework', 'kernel_semantics',
+         'CPU: the OpenCL-CPU kernels (default); GPU: the recurrence of the GPU-OpenCL and CUDA kernels '
+         '(affine gap open G + E)'),
     ]),
     ('OpenCL', [
         (('--device_type',), 'device_type', 'OpenCL', 'device_type', 'CPU, GPU or ACCELERATOR'),

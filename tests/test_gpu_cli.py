@@ -5,7 +5,8 @@ reverse complements) through `pypaswas_amd.pypaswasall.Pypaswas` — the
 reference's pypaswas.py flow (readers, engine dispatch, hit filtering,
 formatter) — compared byte for byte with the oracle's records of the same
 reader-ordered inputs, in every output format (TXT, SAM, trimmedFasta,
-FASTA) and through both plugins (aligner, trimmer).
+FASTA), through both plugins (aligner, trimmer) and under both kernel
+semantics (the OpenCL-CPU recurrence and the GPU-OpenCL / CUDA one).
 """
 import pytest
 
@@ -78,3 +79,43 @@ def test_trimmer_program_trimmed_fasta(tmp_path, oracle):
     got = run_program(tmp_path, reads, adapters, ["-g", "0", "-p", "trimmer", "--outputformat", "trimmedFasta"])
     assert got == oracle.trimmed_records(hl)
     assert len(hl.real_hits) >= 32
+
+
+def indel_set(seed=9):
+    """Reads carrying a 40-nt adapter with one or two residues inserted (gapped
+    best alignments), plus uniform reads."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dna = np.array(list("ACGT"))
+    adapters = [("adapter%d" % i, "".join(rng.choice(dna, size=40))) for i in range(4)]
+    reads = []
+    for i in range(24):
+        a = adapters[i % 4][1]
+        cut = int(rng.integers(12, 28))
+        ins = "".join(rng.choice(dna, size=1 + i % 2))
+        piece = a[:cut] + ins + a[cut:]
+        left, right = "".join(rng.choice(dna, size=30)), "".join(rng.choice(dna, size=30))
+        reads.append(("read%02d" % i, left + piece + right))
+    return reads, adapters
+
+
+@pytest.mark.parametrize("semantics", ["CPU", "GPU"])
+def test_aligner_kernel_semantics_match_oracle(tmp_path, oracle, semantics):
+    """`--kernel_semantics` CPU (the OpenCL-CPU recurrence, gap open G) and GPU
+    (the GPU-OpenCL / CUDA recurrence, gap open G + E) on reads with gapped
+    adapter hits equal the oracle run with gap open -5 or -5 + -1; the two
+    settings give different records on this data."""
+    print("program indel set, kernel semantics", semantics, flush=True)
+    reads, adapters = indel_set()
+    got = run_program(tmp_path, reads, adapters, ["-g", "-1", "--kernel_semantics", semantics])
+    q, t = oracle.prepare_records(reads, adapters, True)
+    want = {}
+    for name, g in (("CPU", -5.0), ("GPU", -6.0)):
+        p = oracle.Params(matrix=oracle.dna_rna_matrix(1.0, -3.0, -1.0, 0.0), gap=g, gap_ext=-1.0, lower_limit=1.0,
+                          minimum=10.0, filter_factor=0.2, highest=1.0, relative_score=0.0, base_score=0.0,
+                          query_coverage=0.0, query_identity=0.0, dna=True)
+        hl, _, _ = oracle.align_all(q, t, p)
+        want[name] = oracle.txt_records(hl)
+    assert got == want[semantics]
+    assert want["CPU"] != want["GPU"]
+    assert got.count("\n") >= 24
