@@ -130,9 +130,11 @@ int pw_release_plan(pw_ctx *ctx, pw_plan *plan);
  * order, pairs, lane-per-pair items, wave-per-pair items, the orientation of
  * the lane-per-pair jobs (0: the target is the shared column sequence, 1: the
  * query), cells scored lane-per-pair, cells scored wave-per-pair, the widest
- * chunk of a lane-per-pair job, the direction pass's largest residues per lane
- * and the lane-per-pair items that belong to jobs split into one item per
- * chunk (at most 9 values). */
+ * chunk of a lane-per-pair job, the direction pass's largest residues per lane,
+ * the lane-per-pair items that belong to jobs split into one item per chunk
+ * and the checkpointed pairs (long linear-gap pairs whose scoring pass saves
+ * checkpoints, so that their direction pass recomputes only the blocks a path
+ * can reach) (at most 10 values). */
 int pw_plan_info(const pw_plan *plan, int64_t *info, int32_t n);
 
 /* pw_execute with the device work of the execution (scoring, gate, direction

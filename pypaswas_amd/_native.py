@@ -407,10 +407,11 @@ class Plan:
 
     def info(self) -> dict:
         """Routing of the plan (pw_plan_info)."""
-        v = np.zeros(9, dtype=np.int64)
-        self.dev._check(self.dev.lib.pw_plan_info(self.handle, v.ctypes.data, 9), "pw_plan_info")
+        v = np.zeros(10, dtype=np.int64)
+        self.dev._check(self.dev.lib.pw_plan_info(self.handle, v.ctypes.data, 10), "pw_plan_info")
         keys = ("pairs", "lane_per_pair_items", "wave_per_pair_items", "orientation", "lane_per_pair_cells",
-                "wave_per_pair_cells", "max_chunk_width", "direction_residues_per_lane", "split_job_items")
+                "wave_per_pair_cells", "max_chunk_width", "direction_residues_per_lane", "split_job_items",
+                "checkpointed_pairs")
         return {k: int(x) for k, x in zip(keys, v)}
 
     def release(self):

@@ -139,6 +139,7 @@ struct PlanOpts {
     int orient = -1;  // column sequence of lane-per-pair jobs: 0 target, 1 query, -1 the cheaper one
     int dir_rmax = 0;  // > 0: largest residues per lane of the direction pass (2, 4 or 8)
     double grid_share = 1.0;  // persistent grids cover at most this share of the device's resident waves
+    bool checkpoints = true;  // long linear pairs save checkpoints for a block-wise direction pass (CkPair)
     PlanScratch *scratch = nullptr;  // reusable host buffers of the planner (the context's; may be null)
 };
 
@@ -273,12 +274,17 @@ struct pw_plan {
     std::vector<std::pair<int64_t, int64_t>> batches;         // plain launch: item range of each batch
     std::vector<ScoreItem> items;
     std::vector<uint8_t> trans;
+    std::vector<CkPair> ckpairs;  // checkpointed pairs, by pair index
+    int64_t ck_floats = 0;        // their checkpoints and block maxima
+    int64_t ck_units = 0;         // their direction-pass blocks (beyond units_worst)
+    int64_t ck_dir_extra = 0;     // their direction bytes at CK_R beyond those at the plan's cap (in dir_worst)
     std::vector<GroupItem> gitems;
     std::vector<GroupEntry> gents;
     // per-plan arrays: views into one device allocation (blob) filled by one
     // copy from pinned staging
     DevBuf res, seqs, tables, d_items, d_trans, d_gitems, d_gents, d_mss, d_pairs, d_same;
-    DevBuf blob, d_links, d_prog;
+    DevBuf d_ckpairs;
+    DevBuf blob, d_links, d_prog, d_ck;
     PinBuf h_blob;
     Workspace own;
     PlanOpts opts;  // routing choices the work list was built with
@@ -954,7 +960,7 @@ int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, con
 }
 
 void free_plan_buffers(pw_ctx *ctx, pw_plan *P) {
-    DevBuf *bufs[] = {&P->blob, &P->d_links, &P->d_prog};
+    DevBuf *bufs[] = {&P->blob, &P->d_links, &P->d_prog, &P->d_ck};
     for (DevBuf *b : bufs) free_buf(ctx, *b);
     free_pinned(P->h_blob);
     free_workspace(ctx, P->own);
@@ -1280,6 +1286,10 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
     const int32_t *pq = P->pq.data(), *pt = P->pt.data(), *qlen = P->qlen.data(), *tlen = P->tlen.data();
     P->batches.clear();
     P->score_links = P->score_prog = 0;
+    P->ckpairs.clear();
+    P->ck_floats = 0;
+    P->ck_units = 0;
+    P->ck_dir_extra = 0;
     P->gitems.clear();
     P->gents.clear();
     P->n_groups = 0;
@@ -1307,7 +1317,7 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
         int bx = 0, by = 0;
         auto close = [&](int64_t end) {
             for (int64_t k = begin; k < end; ++k)
-                items[k] = ScoreItem{pq[k], nq + pt[k], (int32_t)k, 4, bx, by, 0, 1 << 30, -1, 0};
+                items[k] = ScoreItem{pq[k], nq + pt[k], (int32_t)k, 4, bx, by, 0, 1 << 30, -1, 0, -1, 0};
             P->batches.push_back({begin, end});
             max_b = std::max<int64_t>(max_b, by);
         };
@@ -1442,12 +1452,20 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
         std::vector<ScoreItem> titems((size_t)ntile);
         std::vector<uint8_t> ttrans((size_t)ntile, 0);
         std::vector<int> tsplit((size_t)ntile, 0);
+        std::vector<uint8_t> tck((size_t)ntile, 0);
         std::vector<std::pair<double, int64_t>> order((size_t)ntile);
+        // long linear pairs are checkpointed: query along the lanes, CK_R
+        // residues per lane (the direction pass's shape), split into stripes
+        const bool ck_on = opt.checkpoints && !sc.affine && opt.force_R <= 0;
         for (int64_t k = 0; k < ntile; ++k) {
             const int64_t i = tile_pairs[k];
             const int ql = qlen[pq[i]], tl = tlen[pt[i]];
             int tr, R;
-            if (opt.force_R > 0) {
+            if (ck_on && ql >= CK_MIN_LEN && tl >= CK_MIN_LEN) {
+                tr = 0;
+                R = CK_R;
+                tck[k] = 1;
+            } else if (opt.force_R > 0) {
                 tr = 0;
                 R = opt.force_R;
             } else {
@@ -1455,7 +1473,7 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
             }
             const int a = tr ? tl : ql, b = tr ? ql : tl;
             const int aseq = tr ? nq + pt[i] : pq[i], bseq = tr ? pq[i] : nq + pt[i];
-            titems[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b, 0, 1 << 30, -1, 0};
+            titems[k] = ScoreItem{aseq, bseq, (int32_t)i, R, a, b, 0, 1 << 30, -1, 0, -1, 0};
             ttrans[k] = (uint8_t)tr;
             const int rows = 64 * R;
             const int ns = (a + rows - 1) / rows;
@@ -1485,13 +1503,33 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
             it.prog_off = P->score_prog;
             P->score_links += (int64_t)(ns - 1) * link_row_floats(it.blen);
             P->score_prog += ns;
+            int64_t ck0 = -1, bm0 = 0;
+            int nb = 0;
+            if (tck[o]) {
+                // checkpoints [stripe][block][slot][lane], then block maxima [stripe][block]
+                nb = ck_blocks(it.blen);
+                ck0 = P->ck_floats;
+                bm0 = ck0 + (int64_t)ns * nb * CK_SLOTS * WAVE;
+                P->ck_floats = bm0 + (int64_t)ns * nb;
+                P->ckpairs.push_back(CkPair{it.out, nb, ck0, bm0, it.rows_off});
+                P->ck_units += (int64_t)ns * nb;
+                const int ql = qlen[pq[it.out]], tl = tlen[pt[it.out]];
+                P->ck_dir_extra += std::max<int64_t>(
+                    0, dir_area(ql, tl, CK_R) - dir_area(ql, tl, dir_residues(ql, P->dir_rmax)));
+            }
             for (int st = 0; st < ns; ++st) {
                 it.s0 = st;
                 it.s1 = st + 1;
+                if (ck0 >= 0) {
+                    it.ck_off = ck0 + (int64_t)st * nb * CK_SLOTS * WAVE;
+                    it.bmax_off = bm0 + (int64_t)st * nb;
+                }
                 items.push_back(it);
                 trans.push_back(ttrans[o]);
             }
         }
+        std::sort(P->ckpairs.begin(), P->ckpairs.end(),
+                  [](const CkPair &x, const CkPair &y) { return x.pair < y.pair; });
 
         // lane-per-pair jobs in order; a split job becomes one item per chunk,
         // consecutive in chunk order (the persistent workers dequeue in order,
@@ -1556,7 +1594,8 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
                     {&P->d_gents, P->gents.data(), P->gents.size() * sizeof(GroupEntry), 0},
                     {&P->d_mss, P->mss.data(), (size_t)n * sizeof(float), 0},
                     {&P->d_pairs, nullptr, (size_t)n * sizeof(int2), 0},
-                    {&P->d_same, P->same.data(), (size_t)n, 0}};
+                    {&P->d_same, P->same.data(), (size_t)n, 0},
+                    {&P->d_ckpairs, P->ckpairs.data(), P->ckpairs.size() * sizeof(CkPair), 0}};
     size_t total = 0;
     for (Part &q : parts) {
         q.off = total;
@@ -1590,6 +1629,7 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
     HIPCHK(ctx, hipMemcpyAsync(P->blob.p, stage.p, total, hipMemcpyHostToDevice, s));
     ENSURE(ctx, P->d_links, (size_t)P->score_links * sizeof(float));
     ENSURE(ctx, P->d_prog, (size_t)P->score_prog * sizeof(unsigned));
+    ENSURE(ctx, P->d_ck, (size_t)P->ck_floats * sizeof(float));
     if (sync) HIPCHK(ctx, hipStreamSynchronize(s));
     return PW_OK;
 }
@@ -1672,7 +1712,8 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
     // passing the gate could use (one per stripe of a split pair), up to the
     // resident limit; split stripes need their producer stripes running
     g.dir_grid = persistent_grid(ctx, (const void *)(aff ? dir_kernel<true> : dir_kernel<false>),
-                                 lds_bytes(P->E.KP, true), std::max<int64_t>(1, P->units_worst), P->opts.grid_share);
+                                 lds_bytes(P->E.KP, true), std::max<int64_t>(1, P->units_worst + P->ck_units),
+                                 P->opts.grid_share);
     g.dscr_stride = 3 * (P->max_t + 64);
     g.task_grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, (int64_t)ctx->num_cus * 4));
     g.wave_grid = long_path_lengths(P->E.max_len_q, P->E.max_len_t)
@@ -1711,8 +1752,9 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     } else {
         // the whole worst case when it is at most half the limit, else the
         // first size (grown after an execution that needed more)
-        const int64_t want = P->dir_worst <= dir_limit(ctx) / 2 ? std::max(W.want_dir, P->dir_worst) : W.want_dir;
-        const int64_t dcap = std::max<int64_t>(16, std::min(P->dir_worst, std::min(want, dir_limit(ctx))));
+        const int64_t worst = P->dir_worst + P->ck_dir_extra;
+        const int64_t want = worst <= dir_limit(ctx) / 2 ? std::max(W.want_dir, worst) : W.want_dir;
+        const int64_t dcap = std::max<int64_t>(16, std::min(worst, std::min(want, dir_limit(ctx))));
         ENSURE(ctx, W.dirs, (size_t)dcap);
         W.dir_cap = (int64_t)W.dirs.cap;
     }
@@ -1731,7 +1773,7 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     // direction-pass work units: sized for every pair passing the gate; the
     // split pairs' boundary rows up to a quarter of the direction limit (pairs
     // beyond it run unsplit)
-    ENSURE(ctx, W.units, (size_t)std::max<int64_t>(1, P->units_worst) * sizeof(DirUnit));
+    ENSURE(ctx, W.units, (size_t)std::max<int64_t>(1, P->units_worst + P->ck_units) * sizeof(DirUnit));
     W.units_cap = (int64_t)(W.units.cap / sizeof(DirUnit));
     const int64_t lk = std::min<int64_t>(P->dlinks_worst, dir_limit(ctx) / 4 / (int64_t)sizeof(float));
     ENSURE(ctx, W.dlinks, (size_t)std::max<int64_t>(1, lk) * sizeof(float));
@@ -1836,7 +1878,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
                 hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, L.s, res, seqs, items + b.first, (int)n,
                                    trans + b.first, tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
                                    (float *)W.tscratch.p, g.tscr_stride, (unsigned *)nullptr, mx, (float *)nullptr,
-                                   (unsigned *)nullptr, &ctl->abort_wait, ctx->wait_polls);
+                                   (unsigned *)nullptr, &ctl->abort_wait, ctx->wait_polls, (float *)P->d_ck.p);
                 HIPCHK(ctx, hipGetLastError());
                 if (launches) *launches += 1;
             }
@@ -1848,7 +1890,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
                                tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
                                ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p, &ctl->abort_wait,
-                               ctx->wait_polls);
+                               ctx->wait_polls, (float *)P->d_ck.p);
             HIPCHK(ctx, hipGetLastError());
             if (launches) *launches += 1;
         }
@@ -1922,7 +1964,8 @@ int launch_traceback(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geome
     hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP, true), L.s, res, seqs, tasks,
                        (const DirUnit *)W.units.p, ctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
                        P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
-                       (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap, ctx->wait_polls);
+                       (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap, ctx->wait_polls,
+                       (const CkPair *)P->d_ckpairs.p, (float *)P->d_ck.p, (float *)P->d_links.p);
     HIPCHK(ctx, hipGetLastError());
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
     int rc = launch_start_selection(ctx, P, W, L, g, ctl);
@@ -1981,7 +2024,8 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
                            (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
                            (const SeqRef *)P->seqs.p, P->nq, P->sc.ll, ctl, (const int *)W.gated.p,
                            (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
-                           W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax);
+                           W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax,
+                           (const CkPair *)P->d_ckpairs.p, (int)P->ckpairs.size(), (const float *)P->d_ck.p);
         HIPCHK(ctx, hipGetLastError());
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
@@ -2209,7 +2253,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         std::vector<DirUnit> units((size_t)nc);
         for (int u = 0; u < nc; ++u) {
             chunk[u].split = 0;  // tail rounds run each pair's stripes in one unit
-            units[u] = DirUnit{u, -1};
+            units[u] = DirUnit{u, -1, -1};
         }
         Ctl c;
         std::memset(&c, 0, sizeof c);
@@ -2234,7 +2278,8 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         hipLaunchKernelGGL(dk, dim3(g.dir_grid), dim3(64), lds_bytes(P->E.KP, true), L.s, res, seqs, tasks,
                            (const DirUnit *)W.units.p, dctl, (const float *)P->tables.p, P->E.KP, P->E.lkp, P->sc.gap,
                            P->sc.ext, (float *)W.dscratch.p, g.dscr_stride, dir_ptr(W), (Cand *)W.cands.p,
-                           (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap, ctx->wait_polls);
+                           (float *)W.dlinks.p, (unsigned *)W.dprog.p, W.units_cap, ctx->wait_polls,
+                           (const CkPair *)P->d_ckpairs.p, (float *)P->d_ck.p, (float *)P->d_links.p);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[3], L.s));
         int rc0 = launch_start_selection(ctx, P, W, L, g, dctl);
@@ -2440,7 +2485,7 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     const DevBuf *bufs[] = {&W.ctl,      &W.maxima,   &W.gated,    &W.tasks,    &W.touts,    &W.dirs,
                             &W.cands,    &W.spslot,   &W.keys,     &W.out_sps,  &W.out_hits, &W.ops,
                             &W.gscratch, &W.tscratch, &W.dscratch, &W.units,    &W.dlinks,   &W.dprog,
-                            &P->d_links, &P->d_prog};
+                            &P->d_links, &P->d_prog,   &P->d_ck,    &P->d_ckpairs};
     for (const DevBuf *b : bufs) {
         add((uint64_t)(uintptr_t)b->p);
         add(b->cap);
@@ -2648,6 +2693,13 @@ PlanOpts plan_opts(pw_ctx *ctx, const Scoring &sc) {
     return o;
 }
 
+// Calls that only score: no checkpoints for a direction pass.
+PlanOpts score_only_opts(pw_ctx *ctx, const Scoring &sc) {
+    PlanOpts o = plan_opts(ctx, sc);
+    o.checkpoints = false;
+    return o;
+}
+
 int ready(pw_ctx *ctx) {
     if (!ctx) return PW_EINVAL;
     if (!ctx->scoring_set) {
@@ -2700,7 +2752,9 @@ int time_synthetic(pw_ctx *ctx, const PlanOpts &opt, int nq, int A, int nt, int 
     for (int i = 0; i < nq; ++i) qoff[i] = (int64_t)i * A;
     for (int i = 0; i < nt; ++i) toff[i] = (int64_t)i * B;
     pw_plan P;
-    int rc = plan_host(ctx->err, sc, opt, &P, qres.data(), qoff.data(), qlen.data(), nq, tres.data(), toff.data(),
+    PlanOpts o = opt;
+    o.checkpoints = false;  // the tuned shapes are those of the scoring pass alone
+    int rc = plan_host(ctx->err, sc, o, &P, qres.data(), qoff.data(), qlen.data(), nq, tres.data(), toff.data(),
                        tlen.data(), nt, pq.data(), pt.data(), (int64_t)pq.size(), nullptr, nullptr, 1);
     if (!rc) rc = plan_upload(ctx, &P, ctx->lane.s, ctx->h_stage, true);
     Workspace &W = P.own;
@@ -2975,16 +3029,17 @@ extern "C" int pw_execute_graph(pw_ctx *ctx, pw_plan *plan, double minimum, int3
 
 extern "C" int pw_plan_info(const pw_plan *plan, int64_t *info, int32_t n) {
     if (!plan || !info || n < 0) return PW_EINVAL;
-    const int64_t v[9] = {plan->n_pairs,
-                          plan->n_groups,
-                          plan->n_tile,
-                          plan->orient,
-                          (int64_t)plan->group_cells,
-                          (int64_t)(plan->cells - plan->group_cells),
-                          plan->mode == 1 ? plan->max_nb : 0,
-                          plan->dir_rmax,
-                          plan->group_split_items};
-    for (int32_t i = 0; i < n && i < 9; ++i) info[i] = v[i];
+    const int64_t v[10] = {plan->n_pairs,
+                           plan->n_groups,
+                           plan->n_tile,
+                           plan->orient,
+                           (int64_t)plan->group_cells,
+                           (int64_t)(plan->cells - plan->group_cells),
+                           plan->mode == 1 ? plan->max_nb : 0,
+                           plan->dir_rmax,
+                           plan->group_split_items,
+                           (int64_t)plan->ckpairs.size()};
+    for (int32_t i = 0; i < n && i < 10; ++i) info[i] = v[i];
     return PW_OK;
 }
 
@@ -3027,8 +3082,8 @@ extern "C" int pw_score_pairs(pw_ctx *ctx, const uint8_t *qres, const int64_t *q
     }
     if (!ctx->oneshot) ctx->oneshot = new pw_plan();
     pw_plan *P = ctx->oneshot;
-    rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq, pt, n_pairs,
-                   nullptr, nullptr, mode);
+    rc = plan_host(ctx->err, ctx->sc, score_only_opts(ctx, ctx->sc), P, qres, qoff, qlen, nq, tres, toff, tlen, nt, pq,
+                   pt, n_pairs, nullptr, nullptr, mode);
     if (!rc) rc = plan_upload(ctx, P, ctx->lane.s, ctx->h_stage, true);
     Workspace &W = ctx->ws;
     Lane &L = ctx->lane;
@@ -3147,8 +3202,8 @@ extern "C" int pw_score_pairs_noalloc(pw_ctx *ctx, const uint8_t *qres, const in
     // the plan is built in host memory only; its device arrays are views of
     // the static scratch, never released to the context's buffer cache
     std::unique_ptr<pw_plan> P(new pw_plan());
-    rc = plan_host(ctx->err, ctx->sc, plan_opts(ctx, ctx->sc), P.get(), qres, qoff, qlen, nq, tres, toff, tlen, nt,
-                   pq, pt, n_pairs, nullptr, nullptr, mode);
+    rc = plan_host(ctx->err, ctx->sc, score_only_opts(ctx, ctx->sc), P.get(), qres, qoff, qlen, nq, tres, toff, tlen,
+                   nt, pq, pt, n_pairs, nullptr, nullptr, mode);
     if (rc) return rc;
     Workspace W;
     Geometry g = geometry(ctx, P.get());

@@ -128,6 +128,20 @@ __device__ __forceinline__ void wait_progress(const unsigned *p, unsigned need, 
 // pipeline lag per stripe is HANDOFF + 63 steps.
 constexpr int HANDOFF = 32;
 static_assert((HANDOFF & (HANDOFF - 1)) == 0 && HANDOFF <= WAVE, "HANDOFF: a power of two up to the wave size");
+static_assert(CKB % WAVE == 0 && CKB % HANDOFF == 0 && BWIN % CKB == 0, "a block starts on a hand-over chunk, "
+              "a residue-offset reload and inside one residue window");
+
+// Block-wise work of a checkpointed pair (CkPair).  Scoring pass (CK): ck is
+// this stripe's checkpoint array, written at every block start after the
+// first, and bmax its block maxima.  Direction pass (t1 >= 0): the unit runs
+// steps [t0, t1) of one stripe, starting from the checkpoint at ck (nullptr
+// for the first block), with the row above read from the scoring pass's
+// boundary row and no hand-over to the stripe below.
+struct CkArgs {
+    float *ck = nullptr;
+    float *bmax = nullptr;
+    int t0 = 0, t1 = -1;
+};
 
 // ---------------------------------------------------------------------------
 // One tile job: A along lanes (stripes of 64*R rows), B swept.
@@ -142,12 +156,13 @@ static_assert((HANDOFF & (HANDOFF - 1)) == 0 && HANDOFF <= WAVE, "HANDOFF: a pow
 // from boundary s-1 of links and the bottom row goes to boundary s, chunk by
 // chunk, synchronised through the progress counters prog.
 // ---------------------------------------------------------------------------
-template <bool AFF, int R, bool DIRS>
+template <bool AFF, int R, bool DIRS, bool CK = false>
 __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int alen_it, int blen_it,
                       const float *subT, int lkp, uint8_t *bwin, float4 *stg, float *__restrict__ scr, float G,
                       float E, int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
                       Cand *__restrict__ cands, int cand_cap, int s_first, int s_last, float *__restrict__ links,
-                      unsigned *__restrict__ prog, unsigned *abort, unsigned wlimit) {
+                      unsigned *__restrict__ prog, unsigned *abort, unsigned wlimit, CkArgs ckv = CkArgs()) {
+    static_assert(!CK || (!DIRS && !AFF && R == CK_R), "checkpoints: the linear scoring pass at CK_R residues per lane");
     const int lane = threadIdx.x & 63;
     const int rows = WAVE * R;
     const int nstripes = (alen_it + rows - 1) / rows;
@@ -161,6 +176,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     if (s_last > nstripes) s_last = nstripes;
     // a candidate start needs value >= t1 and >= mp (or -value for a marked cell)
     const float thr = __builtin_fmaxf(t1, mp);
+    // direction pass over one block of a checkpointed pair: steps [tb0, te)
+    const bool blk = DIRS && ckv.t1 >= 0;
+    const int tb0 = blk ? ckv.t0 : 0;
+    const int te = blk ? min(ckv.t1, nsteps) : nsteps;
+    float pm = 0.0f;  // checkpointing scoring pass: maximum of the blocks done
     // LDS staging of the row above, HANDOFF columns at a time (lane 0 takes one
     // column per step, as a broadcast read)
     float4 *stg_in = stg;
@@ -221,7 +241,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             Dp[r] = 0.0f;      // M = 0 there
         }
         const bool above = s > 0;
-        const bool below = s + 1 < nstripes;
+        const bool below = !blk && s + 1 < nstripes;
         const float *sin = links ? links + (int64_t)(s - 1) * row_floats : scr;  // row above (when above)
         float *sout = links ? links + (int64_t)s * row_floats : scr;             // bottom row (when below)
         // columns [c0, c0 + HANDOFF) of the row above into LDS (one column per lane)
@@ -238,8 +258,8 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             __syncthreads();
         };
         if (above) {
-            if (links && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(HANDOFF, blen_it), abort, wlimit);
-            stage_in(0);
+            if (prog && blen_it > 0) wait_progress(prog + (s - 1), (unsigned)min(HANDOFF, blen_it), abort, wlimit);
+            stage_in(tb0);
         }
         // inputs of step 0: lane 0 reads column 0 of the row above; the other
         // lanes start in the virtual columns left of the matrix
@@ -251,6 +271,16 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             upM = v.z;
         }
         float dgT = 0.0f, dgM = 0.0f;
+        if (blk && tb0 > 0) {
+            // the state the scoring pass saved before step tb0 (linear gaps:
+            // every lane's R values at its previous column, and the row-above
+            // inputs of step tb0)
+            const float *c = ckv.ck + lane;
+#pragma unroll
+            for (int r = 0; r < R; ++r) Tp[r] = c[r * WAVE];
+            upT = c[R * WAVE];
+            dgT = c[(R + 1) * WAVE];
+        }
 
         // Table offsets of the swept residues: lane k sweeps column t - k at
         // step t, so its column at step t+1 is lane k-1's at step t (a DPP
@@ -259,9 +289,11 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         // read from the LDS window every 64 steps) and moves down one lane per
         // step.  Substitution scores are read one step ahead, so that the LDS
         // latency overlaps a step of arithmetic.
-        load_window(0);
-        int fB = ((int)bwin[(WAVE - 1) + lane]) << lkp;  // columns 0..63
-        int boff = lane == 0 ? fB : (fill << lkp);       // column -lane < 0 for lanes 1..63
+        load_window(tb0 & ~(BWIN - 1));
+        const int w0 = tb0 & (BWIN - 1);
+        int fB = ((int)bwin[w0 + (WAVE - 1) + lane]) << lkp;  // columns tb0 .. tb0 + 63
+        // column tb0 - lane (< 0 for lanes 1..63 at step 0)
+        int boff = tb0 > 0 ? ((int)bwin[w0 + (WAVE - 1) - lane]) << lkp : (lane == 0 ? fB : (fill << lkp));
         float scn[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
@@ -284,7 +316,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             float4 inj = make_float4(0.0f, bJ, 0.0f, 0.0f);
             if (above) {
                 if ((tn & (HANDOFF - 1)) == 0) {
-                    if (links && tn < blen_it)
+                    if (prog && tn < blen_it)
                         wait_progress(prog + (s - 1), (unsigned)min(tn + HANDOFF, blen_it), abort, wlimit);
                     stage_in(tn);
                 }
@@ -489,22 +521,40 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         if constexpr (DIRS) {
             // (two steps per iteration here raise the direction pass's register
             // use past 200 VGPRs, so its steps copy the next scores instead)
-            for (int t = 0; t < nsteps; ++t) {
+            for (int t = tb0; t < te; ++t) {
                 step(t, scn, scm);
 #pragma unroll
                 for (int r = 0; r < R; ++r) scn[r] = scm[r];
             }
             flush_cands();
         } else {
-            int t = 0;
-            for (; t + 1 < nsteps; t += 2) {
-                step(t, scn, scm);
-                step(t + 1, scm, scn);
+            // blocks of CKB steps when checkpointing (an even count, so that the
+            // two score register sets keep their roles), else one block
+            for (int b0 = 0; b0 < nsteps; b0 += (CK ? CKB : nsteps)) {
+                const int b1 = CK ? min(b0 + CKB, nsteps) : nsteps;
+                if (CK && b0 > 0) {
+                    float *c = ckv.ck + (int64_t)(b0 / CKB) * (CK_SLOTS * WAVE) + lane;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) c[r * WAVE] = Tp[r];
+                    c[R * WAVE] = upT;
+                    c[(R + 1) * WAVE] = dgT;
+                }
+                int t = b0;
+                for (; t + 1 < b1; t += 2) {
+                    step(t, scn, scm);
+                    step(t + 1, scm, scn);
+                }
+                if (t < b1) step(t, scn, scm);
+                if (CK) {
+                    const float bm = wave_max(rmax);
+                    if (lane == 0) ckv.bmax[b0 / CKB] = bm;
+                    pm = __builtin_fmaxf(pm, rmax);
+                    rmax = 0.0f;
+                }
             }
-            if (t < nsteps) step(t, scn, scm);
         }
     }
-    return rmax;
+    return CK ? __builtin_fmaxf(rmax, pm) : rmax;
 }
 
 // Loads the two transposed substitution tables (A = query, A = target) into LDS.
@@ -513,17 +563,23 @@ __device__ __forceinline__ void load_tables(float *lds, const float *__restrict_
     __syncthreads();
 }
 
-template <bool AFF, int R>
+template <bool AFF, int R, bool CK = false>
 __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *seqs, const ScoreItem &it,
                                             const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float4 *stg,
                                             float *scr,
                                             float G, float E, int fill, int trans, float *links, unsigned *prog,
-                                            unsigned *abort, unsigned wlimit) {
+                                            unsigned *abort, unsigned wlimit, float *ckarea = nullptr) {
     const float *tab = lds_tab + (trans ? kp2 : 0);
     float *lk = it.rows_off >= 0 ? links + it.rows_off : nullptr;
     unsigned *pg = it.rows_off >= 0 ? prog + it.prog_off : nullptr;
-    return tile<AFF, R, false>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, stg, scr, G, E,
-                               fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg, abort, wlimit);
+    CkArgs ckv;
+    if (CK) {
+        ckv.ck = ckarea + it.ck_off;
+        ckv.bmax = ckarea + it.bmax_off;
+    }
+    return tile<AFF, R, false, CK>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, stg, scr,
+                                   G, E, fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg, abort,
+                                   wlimit, ckv);
 }
 
 // Scoring pass.  queue != nullptr: persistent workers pull items from a
@@ -533,6 +589,8 @@ __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *se
 // items blockIdx.x, blockIdx.x + gridDim.x, ... (no split pairs then).  A
 // split pair's maximum is the maximum over its stripes, combined with an
 // integer atomic max on the (non-negative) float bits into a zeroed slot.
+// The stripes of a checkpointed pair also write their checkpoints and block
+// maxima into ckarea.
 template <bool AFF>
 __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
                                                    const ScoreItem *__restrict__ items, int nitems,
@@ -541,7 +599,7 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
                                                    float E, float *__restrict__ scratch, int64_t scr_stride,
                                                    unsigned *__restrict__ queue, float *__restrict__ maxima,
                                                    float *__restrict__ links, unsigned *__restrict__ prog,
-                                                   unsigned *abort, unsigned wlimit) {
+                                                   unsigned *abort, unsigned wlimit, float *__restrict__ ckarea) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
@@ -568,7 +626,16 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
         case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
         case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
         case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
-        default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+        default:
+            if constexpr (!AFF) {
+                if (it.ck_off >= 0) {
+                    m = score_item<AFF, CK_R, true>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links,
+                                                    prog, abort, wlimit, ckarea);
+                    break;
+                }
+            }
+            m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit);
+            break;
         }
         m = wave_max(m);
         if (threadIdx.x == 0) {
@@ -813,8 +880,9 @@ __device__ __forceinline__ int task_count(const Ctl *ctl) {
 }
 
 // Direction pass over the work units of the traceback tasks (one unit per
-// task, or one per stripe of a split task, consecutive in stripe order);
-// persistent workgroups pull units from a device counter up to the unit count.
+// task, or one per stripe of a split task, consecutive in stripe order, or
+// one per needed block of a checkpointed task, in any order); persistent
+// workgroups pull units from a device counter up to the unit count.
 template <bool AFF>
 __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
                                                  PairTask *__restrict__ tasks, const DirUnit *__restrict__ units,
@@ -822,7 +890,9 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
                                                  int lkp, float G, float E, float *__restrict__ scratch,
                                                  int64_t scr_stride, uint8_t *__restrict__ dirs,
                                                  Cand *__restrict__ cands, float *__restrict__ links,
-                                                 unsigned *__restrict__ prog, int64_t units_cap, unsigned wlimit) {
+                                                 unsigned *__restrict__ prog, int64_t units_cap, unsigned wlimit,
+                                                 const CkPair *__restrict__ ckpairs, float *__restrict__ ckarea,
+                                                 float *__restrict__ score_links) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
@@ -847,6 +917,23 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
         Cand *cl = cands + it.cand_off;
         const int s0 = du.stripe < 0 ? 0 : du.stripe;
         const int s1 = du.stripe < 0 ? (1 << 30) : du.stripe + 1;
+        if constexpr (!AFF) {
+            if (du.block >= 0 && it.ck >= 0 && it.R == CK_R) {
+                // one block of a checkpointed pair: the scoring pass's
+                // checkpoint and boundary row above, no waits
+                const CkPair cp = ckpairs[it.ck];
+                CkArgs ckv;
+                ckv.t0 = du.block * CKB;
+                ckv.t1 = ckv.t0 + CKB;
+                ckv.ck = du.block > 0 ? ckarea + cp.ck_off + ((int64_t)du.stripe * cp.nblocks + du.block) *
+                                                                 (CK_SLOTS * WAVE)
+                                      : nullptr;
+                tile<AFF, CK_R, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp,
+                                      cc, cl, it.cand_cap, s0, s1, score_links + cp.rows_off, nullptr, abort, wlimit,
+                                      ckv);
+                continue;
+            }
+        }
         float *lk = du.stripe < 0 ? nullptr : links + it.rows_off;
         unsigned *pg = du.stripe < 0 ? nullptr : prog + it.prog_off;
         switch (it.R) {
@@ -964,13 +1051,50 @@ __device__ __forceinline__ unsigned long long wave_alloc(unsigned long long *cou
     return b + off;
 }
 
+// Direction-pass blocks a checkpointed pair needs: a path starts at a candidate
+// (a cell whose value reaches thr, so its block's maximum does too) and moves
+// only up and left, so stripe s needs the columns up to ycol(s), the last
+// column of any block of stripe s or below whose maximum reaches thr, i.e.
+// its blocks up to the one that holds step ycol(s) + 63 (lane 63's cell of
+// that column).  Counts the units, and writes them to out when given.
+__device__ int ck_units(const CkPair &cp, const float *__restrict__ ckarea, int ns, float thr,
+                        DirUnit *__restrict__ out, int task) {
+    const float *bm = ckarea + cp.bmax_off;
+    const int nb = cp.nblocks;
+    int ycol = -1, n = 0;
+    for (int s = ns - 1; s >= 0; --s) {
+        for (int b = nb - 1; b >= 0 && (b + 1) * CKB - 1 > ycol; --b)
+            if (bm[(int64_t)s * nb + b] >= thr) {
+                ycol = (b + 1) * CKB - 1;
+                break;
+            }
+        if (ycol < 0) continue;
+        const int k = min(nb, (ycol + WAVE - 1) / CKB + 1);
+        if (out)
+            for (int b = 0; b < k; ++b) out[n + b] = DirUnit{task, s, b};
+        n += k;
+    }
+    return n;
+}
+
+// Index of pair i among the checkpointed pairs (sorted by pair), or -1.
+__device__ __forceinline__ int ck_find(const CkPair *__restrict__ ckp, int nck, int i) {
+    int lo = 0, hi = nck;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ckp[mid].pair < i) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < nck && ckp[lo].pair == i) ? lo : -1;
+}
+
 // Builds the task of every gated pair (slot = its position in `gated`), with
 // its direction and candidate areas taken from the bump allocators in ctl,
 // and its direction-pass work units (one per stripe for a split pair, whose
 // boundary rows and zeroed progress counters are allocated here too; without
-// room it runs unsplit).  A pair that does not fit is marked TASK_DEFERRED for
-// a later round.  Allocations are summed over the wave: one atomic per wave
-// and counter.
+// room it runs unsplit; for a checkpointed pair, one per block it needs).  A
+// pair that does not fit is marked TASK_DEFERRED for a later round.
+// Allocations are summed over the wave: one atomic per wave and counter.
 __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
                                                    const int2 *__restrict__ pairs,
                                                    const uint8_t *__restrict__ same_id,
@@ -979,7 +1103,9 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
                                                    PairTask *__restrict__ tasks, int64_t dir_cap,
                                                    int64_t cand_cap_total, int cand0, DirUnit *__restrict__ units,
                                                    int64_t units_cap, int64_t link_cap,
-                                                   unsigned *__restrict__ prog, int64_t prog_cap, int rmax) {
+                                                   unsigned *__restrict__ prog, int64_t prog_cap, int rmax,
+                                                   const CkPair *__restrict__ ckpairs, int nck,
+                                                   const float *__restrict__ ckarea) {
     const int ng = __hip_atomic_load(&ctl->ngated, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int stride = gridDim.x * blockDim.x;
     // whole waves iterate together (the scans need every lane)
@@ -987,7 +1113,7 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
         const int g = g0 + (threadIdx.x & 63);
         const bool on = g < ng;
         PairTask t;
-        int ql = 0, tl = 0, ns = 0;
+        int ql = 0, tl = 0, ns = 0, ck = -1, ckn = 0;
         int64_t area = 0;
         float m = 0.0f;
         if (on) {
@@ -999,9 +1125,10 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
             t.pair = i;
             t.q_seq = pr.x;
             t.t_seq = nq + pr.y;
-            t.R = dir_residues(ql, rmax);
+            if (nck > 0 && ql >= CK_MIN_LEN && tl >= CK_MIN_LEN) ck = ck_find(ckpairs, nck, i);
+            t.R = ck >= 0 ? CK_R : dir_residues(ql, rmax);
             area = dir_area(ql, tl, t.R);
-            ns = split_stripes(ql, tl, t.R);
+            ns = ck >= 0 ? 0 : split_stripes(ql, tl, t.R);
             t.ops_off = 0;
             t.nops = 0;
             t.t1 = ll * m;
@@ -1017,6 +1144,11 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
             t.split = 0;
             t.rows_off = 0;
             t.prog_off = 0;
+            t.ck = ck;
+            t.pad = 0;
+            if (ck >= 0)
+                ckn = ck_units(ckpairs[ck], ckarea, (ql + WAVE * CK_R - 1) / (WAVE * CK_R),
+                               __builtin_fmaxf(t.t1, t.mp), nullptr, g);
         }
         const unsigned long long doff = wave_alloc(&ctl->dir_used, on ? (uint64_t)area : 0ull);
         const unsigned long long coff = wave_alloc(&ctl->cand_used, on ? (uint64_t)cand0 : 0ull);
@@ -1025,7 +1157,7 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
         const unsigned long long lo = wave_alloc(&ctl->link_used, (uint64_t)rows);
         const unsigned long long po = wave_alloc(&ctl->prog_used, (ok && ns > 0) ? (uint64_t)ns : 0ull);
         const bool split = ok && ns > 0 && (int64_t)lo + rows <= link_cap && (int64_t)po + ns <= prog_cap;
-        const int cnt = ok ? (split ? ns : 1) : 0;
+        const int cnt = ok ? (ck >= 0 ? ckn : (split ? ns : 1)) : 0;
         const unsigned long long k0 = wave_alloc(reinterpret_cast<unsigned long long *>(&ctl->units_used),
                                                  (uint64_t)cnt);
         if (!on) continue;
@@ -1040,12 +1172,17 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
         }
         if (ok) {
             if ((int64_t)k0 + cnt <= units_cap) {
-                for (int k = 0; k < cnt; ++k) units[k0 + k] = DirUnit{g, split ? k : -1};
+                if (ck >= 0) {
+                    (void)ck_units(ckpairs[ck], ckarea, (ql + WAVE * CK_R - 1) / (WAVE * CK_R),
+                                   __builtin_fmaxf(t.t1, t.mp), units + k0, g);
+                } else {
+                    for (int k = 0; k < cnt; ++k) units[k0 + k] = DirUnit{g, split ? k : -1, -1};
+                }
             } else {
                 // no unit slots (the host sizes them for every pair): finish the
                 // pair in a tail round; its in-range slots become no-ops
                 t.status = TASK_DEFERRED;
-                for (int k = 0; k < cnt && (int64_t)k0 + k < units_cap; ++k) units[k0 + k] = DirUnit{g, -1};
+                for (int k = 0; k < cnt && (int64_t)k0 + k < units_cap; ++k) units[k0 + k] = DirUnit{g, -1, -1};
             }
         }
         tasks[g] = t;

@@ -34,7 +34,32 @@ struct ScoreItem {
     int32_t s0, s1;  // stripes of this item: all of the pair's, or one stripe of a split pair
     int64_t rows_off;  // split pair: its stripe-boundary rows in the link area (floats); -1 otherwise
     int64_t prog_off;  // split pair: its progress counters in the counter area
+    int64_t ck_off;    // checkpointed pair: this stripe's checkpoints in the checkpoint area (floats); -1 otherwise
+    int64_t bmax_off;  // checkpointed pair: this stripe's block maxima in the checkpoint area (floats)
 };
+
+// Checkpointed pairs (long linear-gap pairs of a batched plan).  Their
+// scoring pass runs with the query along the lanes, CK_R residues per lane and
+// one item per stripe, and every CKB steps of a stripe it saves the wave's
+// state (per lane: the R values of its rows at its previous column, then the
+// row-above inputs of the next step, upT and dgT) and the largest value of the
+// CKB steps before.  The direction pass of such a pair then recomputes only
+// the blocks of CKB steps that a path from a candidate start can reach (cells
+// up and left of the candidates), every block from its checkpoint and the
+// boundary row the scoring pass left above its stripe, with no hand-over
+// between waves.
+constexpr int CKB = 512;                 // steps per block (divides BWIN, a multiple of HANDOFF and of 64)
+constexpr int CK_R = 8;                  // residues per lane of a checkpointed pair
+constexpr int CK_SLOTS = CK_R + 2;       // checkpoint values per lane (linear gaps)
+constexpr int CK_MIN_LEN = 2 * 64 * CK_R;  // both sequences at least this long (two stripes, two blocks)
+struct CkPair {
+    int32_t pair;
+    int32_t nblocks;   // blocks per stripe: ceil((tlen + 63) / CKB)
+    int64_t ck_off;    // floats: checkpoints [stripe][block][slot][64 lanes]
+    int64_t bmax_off;  // floats: block maxima [stripe][block]
+    int64_t rows_off;  // floats: the scoring pass's stripe-boundary rows of the pair (link area)
+};
+__host__ __device__ inline int ck_blocks(int tlen) { return (tlen + 63 + CKB - 1) / CKB; }
 
 // Stripe-boundary rows of a split pair: boundary k (between stripes k and
 // k+1) holds LINK_VALS floats per column for blen + 64 columns; counter k
@@ -43,10 +68,12 @@ constexpr int LINK_VALS = 3;
 __host__ __device__ inline int64_t link_row_floats(int blen) { return (int64_t)LINK_VALS * (blen + 64); }
 
 // One work unit of the direction pass: a task, and the stripe of it this
-// unit runs (-1: all its stripes, one after the other).
+// unit runs (-1: all its stripes, one after the other); for a checkpointed
+// pair, the block of CKB steps of that stripe (-1: the whole stripe).
 struct DirUnit {
     int32_t task;
     int32_t stripe;
+    int32_t block;
 };
 
 // One job of the lane-per-pair scoring kernel: a column sequence shared by up
@@ -102,6 +129,8 @@ struct PairTask {
     int32_t split;       // 1: the direction pass runs each stripe as its own unit
     int64_t rows_off;    // split: stripe-boundary rows in the link area (floats)
     int64_t prog_off;    // split: progress counters in the counter area
+    int32_t ck;          // checkpointed pair: its CkPair index (-1: none)
+    int32_t pad;
 };
 
 // What the host reads back per task: where the pair's records sit (written
