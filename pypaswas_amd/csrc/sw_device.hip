@@ -289,7 +289,12 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         // read from the LDS window every 64 steps) and moves down one lane per
         // step.  Substitution scores are read one step ahead, so that the LDS
         // latency overlaps a step of arithmetic.
-        load_window(tb0 & ~(BWIN - 1));
+        // (an opaque window base: letting the compiler fold the block's first
+        // step into the window's addressing raised the kernel's register use
+        // from 112 to 166 VGPRs, one wave per SIMD fewer)
+        int wbase = tb0 & ~(BWIN - 1);
+        asm volatile("" : "+s"(wbase));
+        load_window(wbase);
         const int w0 = tb0 & (BWIN - 1);
         int fB = ((int)bwin[w0 + (WAVE - 1) + lane]) << lkp;  // columns tb0 .. tb0 + 63
         // column tb0 - lane (< 0 for lanes 1..63 at step 0)
@@ -917,31 +922,29 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
         Cand *cl = cands + it.cand_off;
         const int s0 = du.stripe < 0 ? 0 : du.stripe;
         const int s1 = du.stripe < 0 ? (1 << 30) : du.stripe + 1;
-        if constexpr (!AFF) {
-            if (du.block >= 0 && it.ck >= 0 && it.R == CK_R) {
-                // one block of a checkpointed pair: the scoring pass's
-                // checkpoint and boundary row above, no waits
-                const CkPair cp = ckpairs[it.ck];
-                CkArgs ckv;
-                ckv.t0 = du.block * CKB;
-                ckv.t1 = ckv.t0 + CKB;
-                ckv.ck = du.block > 0 ? ckarea + cp.ck_off + ((int64_t)du.stripe * cp.nblocks + du.block) *
-                                                                 (CK_SLOTS * WAVE)
-                                      : nullptr;
-                tile<AFF, CK_R, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp,
-                                      cc, cl, it.cand_cap, s0, s1, score_links + cp.rows_off, nullptr, abort, wlimit,
-                                      ckv);
-                continue;
-            }
-        }
         float *lk = du.stripe < 0 ? nullptr : links + it.rows_off;
         unsigned *pg = du.stripe < 0 ? nullptr : prog + it.prog_off;
+        CkArgs ckv;
+        if (!AFF && du.block >= 0 && it.ck >= 0 && it.R == CK_R) {
+            // one block of a checkpointed pair: the scoring pass's checkpoint
+            // and boundary row above, no waits (the same instance of the tile
+            // as the pair's whole-stripe form, so that the kernel keeps one
+            // register budget)
+            const CkPair cp = ckpairs[it.ck];
+            ckv.t0 = du.block * CKB;
+            ckv.t1 = ckv.t0 + CKB;
+            ckv.ck = du.block > 0
+                         ? ckarea + cp.ck_off + ((int64_t)du.stripe * cp.nblocks + du.block) * (CK_SLOTS * WAVE)
+                         : nullptr;
+            lk = score_links + cp.rows_off;
+            pg = nullptr;
+        }
         switch (it.R) {
         case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
         case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
         case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
         case 6: tile<AFF, 6, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
-        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
+        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit, ckv); break;
         }
     }
 }
