@@ -249,6 +249,7 @@ struct pw_plan {
     int32_t nq = 0, nt = 0;
     int64_t n_pairs = 0;
     int32_t mode = 1;
+    bool cross = false;  // the pairs are every query with every target, query-major (pair = q * nt + t)
     std::vector<int32_t> pq, pt, qlen, tlen;
     std::vector<float> mss;
     std::vector<uint8_t> same;
@@ -937,25 +938,43 @@ size_t lds_bytes(int KP, bool dirs = false) {
            (dirs ? CAND_BUF * sizeof(Cand) : 0);
 }
 
-int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs) {
+// Checks that every pair indexes the sequence lists; *cross (when given) tells
+// whether the pairs are every query with every target in query-major order
+// (pair q * nt + t), the all-vs-all batch of the engine, which the planner
+// handles without per-pair lists.
+int check_pairs(std::string &err, int32_t nq, int32_t nt, const int32_t *pq, const int32_t *pt, int64_t n_pairs,
+                bool *cross = nullptr) {
     if (n_pairs < 0 || n_pairs > (int64_t)INT32_MAX) {
         err = "n_pairs out of range";
         return PW_EINVAL;
     }
-    // the first pair out of range, over host threads
+    // the first pair out of range, and whether each range follows the
+    // query-major grid, over host threads
     std::vector<int64_t> bad(16, INT64_MAX);
+    std::vector<uint8_t> grid(16, 1);
+    const bool may_cross = nt > 0 && n_pairs > 0 && n_pairs == (int64_t)nq * nt;
     parallel_ranges(n_pairs, 1 << 18, [&](int64_t lo, int64_t hi, int k) {
-        for (int64_t i = lo; i < hi; ++i)
+        int32_t gq = may_cross ? (int32_t)(lo / nt) : 0, gt = may_cross ? (int32_t)(lo % nt) : 0;
+        bool g = may_cross;
+        for (int64_t i = lo; i < hi; ++i) {
             if (pq[i] < 0 || pq[i] >= nq || pt[i] < 0 || pt[i] >= nt) {
                 bad[k] = i;
                 break;
             }
+            g = g && pq[i] == gq && pt[i] == gt;
+            if (++gt == nt) {
+                gt = 0;
+                ++gq;
+            }
+        }
+        grid[k] = g ? 1 : 0;
     });
     const int64_t first = *std::min_element(bad.begin(), bad.end());
     if (first != INT64_MAX) {
         err = "pair " + std::to_string(first) + " indexes outside the sequence lists";
         return PW_EINVAL;
     }
+    if (cross) *cross = may_cross && std::all_of(grid.begin(), grid.end(), [](uint8_t v) { return v != 0; });
     return PW_OK;
 }
 
@@ -992,8 +1011,10 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
         err = "mode must be 0 (plain per-batch launch) or 1 (batched persistent)";
         return PW_EINVAL;
     }
-    int rc = check_pairs(err, nq, nt, pq, pt, n_pairs);
+    bool cross = false;
+    int rc = check_pairs(err, nq, nt, pq, pt, n_pairs, &cross);
     if (rc) return rc;
+    P->cross = cross;
     P->E = Encoded();
     rc = encode(err, sc, qres, qoff, qlen, nq, tres, toff, tlen, nt, P->E);
     if (rc) return rc;
@@ -1041,42 +1062,77 @@ int plan_host(std::string &err, const Scoring &sc, const PlanOpts &opt, pw_plan 
                 qs[q].S[c] = qlen[q] > 0 ? (qlen[q] + rows - 1) / rows : 1;
             }
     });
-    std::vector<Acc> acc(16);  // per thread (parallel_ranges runs at most 16)
-    parallel_ranges(n_pairs, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
-        Acc a;
-        for (int64_t i = lo; i < hi; ++i) {
-            const QShape &qh = qs[pq[i]];
-            const int ql = qlen[pq[i]], tl = tlen[pt[i]];
-            a.cells += (int64_t)ql * tl;
-            a.max_t = std::max<int64_t>(a.max_t, tl);
-            const int64_t steps = (int64_t)tl + WAVE - 1;
-            const bool long_cols = tl >= SPLIT_MIN_COLS;
+    Acc tot;
+    if (cross) {
+        // every query with every target: each sum over the pairs is a product
+        // of a sum over the queries and one over the targets (a pair's
+        // direction area S * (tlen + 63) * 32R needs no rounding)
+        int64_t sum_tl = 0, sum_steps = 0, max_steps = 0, nlong = 0, links_long = 0;
+        for (int32_t t = 0; t < nt; ++t) {
+            const int64_t tl = tlen[t];
+            sum_tl += tl;
+            sum_steps += tl + WAVE - 1;
+            max_steps = std::max<int64_t>(max_steps, tl + WAVE - 1);
+            tot.max_t = std::max<int64_t>(tot.max_t, tl);
+            if (tl >= SPLIT_MIN_COLS) {
+                ++nlong;
+                links_long += link_row_floats((int)tl);
+            }
+        }
+        int64_t sum_ql = 0;
+        for (int32_t q = 0; q < nq; ++q) {
+            sum_ql += qlen[q];
             for (int c = 0; c < 3; ++c) {
-                const int64_t S = qh.S[c];
-                // dir_area(ql, tl, R) and split_stripes(ql, tl, R)
-                const int64_t area = (S * steps * (WAVE / 2) * qh.R[c] + 15) & ~(int64_t)15;
-                a.dir_worst[c] += area;
-                a.dir_largest[c] = std::max(a.dir_largest[c], area);
-                const bool split = S >= 2 && long_cols;
-                a.units[c] += split ? S : 1;
-                if (split) {
-                    a.dlinks[c] += (S - 1) * link_row_floats(tl);
-                    a.dprog[c] += S;
+                const int64_t S = qs[q].S[c], per = S * (WAVE / 2) * qs[q].R[c];
+                tot.dir_worst[c] += per * sum_steps;
+                tot.dir_largest[c] = std::max(tot.dir_largest[c], per * max_steps);
+                if (S >= 2) {
+                    tot.units[c] += S * nlong + (nt - nlong);
+                    tot.dlinks[c] += (S - 1) * links_long;
+                    tot.dprog[c] += S * nlong;
+                } else {
+                    tot.units[c] += nt;
                 }
             }
         }
-        acc[k] = a;
-    });
-    Acc tot;
-    for (const Acc &a : acc) {
-        tot.cells += a.cells;
-        tot.max_t = std::max(tot.max_t, a.max_t);
-        for (int c = 0; c < 3; ++c) {
-            tot.units[c] += a.units[c];
-            tot.dir_worst[c] += a.dir_worst[c];
-            tot.dir_largest[c] = std::max(tot.dir_largest[c], a.dir_largest[c]);
-            tot.dlinks[c] += a.dlinks[c];
-            tot.dprog[c] += a.dprog[c];
+        tot.cells = sum_ql * sum_tl;
+    } else {
+        std::vector<Acc> acc(16);  // per thread (parallel_ranges runs at most 16)
+        parallel_ranges(n_pairs, 1 << 16, [&](int64_t lo, int64_t hi, int k) {
+            Acc a;
+            for (int64_t i = lo; i < hi; ++i) {
+                const QShape &qh = qs[pq[i]];
+                const int ql = qlen[pq[i]], tl = tlen[pt[i]];
+                a.cells += (int64_t)ql * tl;
+                a.max_t = std::max<int64_t>(a.max_t, tl);
+                const int64_t steps = (int64_t)tl + WAVE - 1;
+                const bool long_cols = tl >= SPLIT_MIN_COLS;
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t S = qh.S[c];
+                    // dir_area(ql, tl, R) and split_stripes(ql, tl, R)
+                    const int64_t area = (S * steps * (WAVE / 2) * qh.R[c] + 15) & ~(int64_t)15;
+                    a.dir_worst[c] += area;
+                    a.dir_largest[c] = std::max(a.dir_largest[c], area);
+                    const bool split = S >= 2 && long_cols;
+                    a.units[c] += split ? S : 1;
+                    if (split) {
+                        a.dlinks[c] += (S - 1) * link_row_floats(tl);
+                        a.dprog[c] += S;
+                    }
+                }
+            }
+            acc[k] = a;
+        });
+        for (const Acc &a : acc) {
+            tot.cells += a.cells;
+            tot.max_t = std::max(tot.max_t, a.max_t);
+            for (int c = 0; c < 3; ++c) {
+                tot.units[c] += a.units[c];
+                tot.dir_worst[c] += a.dir_worst[c];
+                tot.dir_largest[c] = std::max(tot.dir_largest[c], a.dir_largest[c]);
+                tot.dlinks[c] += a.dlinks[c];
+                tot.dprog[c] += a.dprog[c];
+            }
         }
     }
     int ci = 0;
@@ -1121,6 +1177,10 @@ struct OrientPlan {
     std::vector<PairRef> by_col, tile, grp;
     std::vector<uint8_t> split;
     std::vector<int> nb;
+    // the all-vs-all batch: per column and 64-lane group of the shared row
+    // order, 0 (tile kernel), 1 (one wave) or 2 (one item per chunk), in place
+    // of the tile, grp and split lists
+    std::vector<uint8_t> choice;
 };
 
 }  // namespace
@@ -1129,6 +1189,11 @@ struct OrientPlan {
 // pages are not faulted in again for every call.
 struct PlanScratch {
     OrientPlan op[2];
+    // work-list temporaries of plan_worklist (kept so that their pages are
+    // not faulted in and zeroed again for every plan)
+    std::vector<int64_t> tile_pairs, gorder;
+    std::vector<GroupItem> gitems, gjobs;
+    std::vector<double> gcost;
 };
 
 namespace {
@@ -1274,6 +1339,115 @@ void orient_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, 
     for (int32_t c = 0; c < ncol; ++c) R.cost += ccost[c];
 }
 
+// The all-vs-all batch (P->cross): every column holds every row sequence, in
+// row order, so the candidates' order (by row length, longest first, stable),
+// their 64-lane groups, length sums and longest rows are the same for every
+// column; only the choices that depend on the column's length differ.  The
+// functions below make exactly the choices and cost sums of orient_plan
+// without its per-pair lists.
+struct CrossRows {
+    std::vector<int32_t> cand;   // candidate rows, sorted
+    std::vector<int32_t> other;  // rows too long for a lane-per-pair job, in row order
+    std::vector<double> gsum;    // per 64-lane group: sum of its row lengths
+    std::vector<int32_t> gtop;   // per group: its longest row length (at least 1)
+};
+
+void cross_rows(const PlanOpts &opt, const pw_plan *P, int o, CrossRows &C) {
+    const int32_t nrows = o ? P->nt : P->nq;
+    const int32_t *rlen = o ? P->tlen.data() : P->qlen.data();
+    C.cand.clear();
+    C.other.clear();
+    for (int32_t r = 0; r < nrows; ++r) {
+        if (!opt.force_tile && rlen[r] <= GROUP_MAX_ROWS) C.cand.push_back(r);
+        else C.other.push_back(r);
+    }
+    std::stable_sort(C.cand.begin(), C.cand.end(), [&](int32_t x, int32_t y) { return rlen[x] > rlen[y]; });
+    const int64_t nc = (int64_t)C.cand.size();
+    C.gsum.clear();
+    C.gtop.clear();
+    for (int64_t g0 = 0; g0 < nc; g0 += 64) {
+        const int64_t g1 = std::min<int64_t>(nc, g0 + 64);
+        double sum = 0;
+        for (int64_t u = g0; u < g1; ++u) sum += rlen[C.cand[u]];
+        C.gsum.push_back(sum);
+        C.gtop.push_back(std::max(1, rlen[C.cand[g0]]));
+    }
+}
+
+// One column of length cl: its cost (summed in orient_plan's order), its
+// chunk width, and per group 0 (to the tile kernel), 1 (one wave) or 2 (one
+// item per chunk).
+double cross_column(const PlanOpts &opt, const CrossRows &C, const int32_t *rlen, int cl, double per_lane, int *nb_out,
+                    uint8_t *choice) {
+    double cost = 0;
+    for (int32_t r : C.other) cost += TILE_COST_FACTOR * (double)rlen[r] * cl;
+    const int nb = (opt.force_nb > 0 && cl > 96) ? opt.force_nb : group_nb(cl, opt.tune);
+    const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
+    for (size_t g = 0; g < C.gsum.size(); ++g) {
+        const double sum = C.gsum[g];
+        const int top = C.gtop[g];
+        const double fill = sum / (64.0 * top);
+        const double job_steps = (double)top * cl;
+        bool keep = opt.force_group || (fill >= GROUP_MIN_FILL && job_steps <= opt.group_max_share * per_lane);
+        const bool split = !keep && !opt.no_group_split && fill >= GROUP_MIN_FILL && nch > 1 &&
+                           (double)top * nb <= opt.group_max_share * per_lane &&
+                           (double)GROUP_PUB * (nch - 1) <= 2.0 * GROUP_SPLIT_MAX_WAIT * top;
+        keep = keep || split;
+        const double wait = split ? (double)GROUP_PUB * (nch - 1) / (2.0 * top) : 0.0;
+        cost += keep ? 64.0 * job_steps * (1.0 + wait) : TILE_COST_FACTOR * sum * cl;
+        if (choice) choice[g] = keep ? (split ? 2 : 1) : 0;
+    }
+    if (nb_out) *nb_out = nb;
+    return cost;
+}
+
+// The orientation's cost: per-column costs on host threads, summed in column order.
+double cross_cost(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, const CrossRows &C) {
+    const int32_t ncol = o ? P->nq : P->nt;
+    const int32_t *rlen = o ? P->tlen.data() : P->qlen.data();
+    const int32_t *clen = o ? P->qlen.data() : P->tlen.data();
+    std::vector<double> cc((size_t)ncol);
+    parallel_ranges(ncol, std::max<int64_t>(1, (int64_t)(1 << 14) / std::max<int64_t>(1, (int64_t)C.gsum.size())),
+                    [&](int64_t lo, int64_t hi, int) {
+                        for (int64_t c = lo; c < hi; ++c)
+                            cc[c] = cross_column(opt, C, rlen, clen[c], per_lane, nullptr, nullptr);
+                    });
+    double cost = 0;
+    for (double v : cc) cost += v;
+    return cost;
+}
+
+// orient_plan's choices for the all-vs-all batch: per column its chunk width,
+// its counts of tile-kernel and lane-per-pair pairs and the choice of every
+// group (the jobs are built from these and the shared row order).
+void cross_plan(const PlanOpts &opt, const pw_plan *P, int o, double per_lane, const CrossRows &C, OrientPlan &R) {
+    const int32_t ncol = o ? P->nq : P->nt;
+    const int32_t *rlen = o ? P->tlen.data() : P->qlen.data();
+    const int32_t *clen = o ? P->qlen.data() : P->tlen.data();
+    const int64_t ng = (int64_t)C.gsum.size(), nc = (int64_t)C.cand.size();
+    R.ntile.assign((size_t)ncol, 0);
+    R.ngrp.assign((size_t)ncol, 0);
+    R.nb.assign((size_t)ncol, 16);
+    R.choice.resize((size_t)ncol * ng);
+    std::vector<double> cc((size_t)ncol);
+    parallel_ranges(ncol, std::max<int64_t>(1, (int64_t)(1 << 14) / std::max<int64_t>(1, ng)),
+                    [&](int64_t lo, int64_t hi, int) {
+        for (int64_t cs = lo; cs < hi; ++cs) {
+            int nb = 16;
+            uint8_t *ch = R.choice.data() + cs * ng;
+            cc[cs] = cross_column(opt, C, rlen, clen[cs], per_lane, &nb, ch);
+            R.nb[cs] = nb;
+            int64_t kept = 0;
+            for (int64_t g = 0; g < ng; ++g)
+                if (ch[g]) kept += std::min<int64_t>(nc, g * 64 + 64) - g * 64;
+            R.ngrp[cs] = kept;
+            R.ntile[cs] = (int64_t)C.other.size() + nc - kept;
+        }
+    });
+    R.cost = 0;
+    for (double v : cc) R.cost += v;
+}
+
 // Builds the scoring work list of a plan whose inputs are in place (plain
 // batches, or lane-per-pair jobs and tile items under the routing choices of opt).
 int plan_worklist(const PlanOpts &opt, pw_plan *P) {
@@ -1341,19 +1515,31 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
         // target (orient 0, rows = queries) or the query (orient 1, rows =
         // targets); the orientation with the lower estimated cost is used.
         const double per_lane = P->cells / ((double)std::max(1, num_cus) * 4.0 * WAVE);
-        std::vector<int64_t> tile_pairs;
-        std::vector<GroupItem> gitems;
-        std::vector<GroupEntry> &gents = P->gents;
-        std::vector<double> gcost;
         std::unique_ptr<PlanScratch> own_scratch;
         PlanScratch *scr = opt.scratch;
         if (!scr) {
             own_scratch.reset(new PlanScratch());
             scr = own_scratch.get();
         }
+        std::vector<int64_t> &tile_pairs = scr->tile_pairs;
+        std::vector<GroupItem> &gitems = scr->gitems;
+        std::vector<GroupEntry> &gents = P->gents;
+        std::vector<double> &gcost = scr->gcost;
         OrientPlan *op = scr->op;
         int orient = opt.orient;
-        if (orient < 0) {
+        CrossRows cr[2];
+        if (P->cross) {
+            // the all-vs-all batch: both orientations costed from one sort of
+            // each side's lengths, then the chosen one's choices made
+            if (orient < 0) {
+                cross_rows(opt, P, 1, cr[1]);
+                cross_rows(opt, P, 0, cr[0]);
+                orient = cross_cost(opt, P, 1, per_lane, cr[1]) < cross_cost(opt, P, 0, per_lane, cr[0]) ? 1 : 0;
+            } else {
+                cross_rows(opt, P, orient, cr[orient]);
+            }
+            cross_plan(opt, P, orient, per_lane, cr[orient], op[orient]);
+        } else if (orient < 0) {
             orient_plan(opt, P, 1, per_lane, op[1]);
             orient_plan(opt, P, 0, per_lane, op[0]);
             orient = op[1].cost < op[0].cost ? 1 : 0;
@@ -1385,31 +1571,27 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                 int64_t cells = 0, rows = 0;
                 int max_nb = 16;
                 for (int64_t cs = c_lo; cs < c_hi; ++cs) {
-                    const int64_t base = R.off[cs];
-                    for (int64_t u = 0; u < R.ntile[cs]; ++u) tile_pairs[tile0[cs] + u] = R.tile[base + u].pair;
                     const int cl = o ? qlen[cs] : tlen[cs];
                     const int nb = R.nb[cs];
                     const int nch = cl > nb ? (cl + nb - 1) / nb : 1;
-                    const int64_t ng = R.ngrp[cs];
-                    int64_t j = job0[cs], e = ent0[cs];
-                    for (int64_t g0 = 0; g0 < ng; g0 += 64, ++j) {
-                        const int cnt = (int)std::min<int64_t>(64, ng - g0);
-                        const PairRef *grp = R.grp.data() + base + g0;
+                    int64_t j = job0[cs], e = ent0[cs], tp = tile0[cs];
+                    // one 64-lane job of cnt pairs, ref(u) the u-th
+                    auto job = [&](int cnt, int split, auto ref) {
                         GroupItem gi;
                         gi.c_seq = o ? (int32_t)cs : nq + (int32_t)cs;
                         gi.entry_off = (int32_t)e;
                         gi.count = cnt;
-                        gi.rows = rlen[grp[0].row];
+                        gi.rows = rlen[ref(0).row];
                         gi.nb = nb;
                         gi.trans = o;
                         gi.k0 = 0;
                         gi.k1 = nch;
                         gi.link_in = gi.link_out = -1;
                         gi.prog_in = gi.prog_out = -1;
-                        gi.split = R.split[base + g0];
+                        gi.split = split;
                         gi.pad = 0;
                         for (int u = 0; u < cnt; ++u) {
-                            const PairRef pr = grp[u];
+                            const PairRef pr = ref(u);
                             gents[e++] = GroupEntry{o ? nq + pr.row : pr.row, pr.pair};
                             cells += (int64_t)rlen[pr.row] * cl;
                         }
@@ -1418,6 +1600,35 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
                         gcost[j] = gi.split ? (double)(gi.rows + GROUP_PUB * (nch - 1)) * nb : (double)gi.rows * nch * nb;
                         if (nch > 1 && !gi.split) rows = std::max<int64_t>(rows, gi.rows);
                         max_nb = std::max(max_nb, nb);
+                        ++j;
+                    };
+                    if (P->cross) {
+                        // the shared row order and this column's group choices
+                        const CrossRows &C = cr[o];
+                        const int64_t ng = (int64_t)C.gsum.size(), nc = (int64_t)C.cand.size();
+                        const uint8_t *ch = R.choice.data() + cs * ng;
+                        auto pref = [&](int32_t row) {
+                            return PairRef{(int32_t)(o ? cs * nt + row : (int64_t)row * nt + cs), row};
+                        };
+                        for (int32_t r : C.other) tile_pairs[tp++] = pref(r).pair;
+                        for (int64_t g = 0; g < ng; ++g) {
+                            const int64_t g0 = g * 64, cnt = std::min<int64_t>(nc, g0 + 64) - g0;
+                            const int32_t *rows_g = C.cand.data() + g0;
+                            if (!ch[g]) {
+                                for (int64_t u = 0; u < cnt; ++u) tile_pairs[tp++] = pref(rows_g[u]).pair;
+                                continue;
+                            }
+                            job((int)cnt, ch[g] == 2 ? 1 : 0, [&](int u) { return pref(rows_g[u]); });
+                        }
+                    } else {
+                        const int64_t base = R.off[cs];
+                        for (int64_t u = 0; u < R.ntile[cs]; ++u) tile_pairs[tp++] = R.tile[base + u].pair;
+                        const int64_t ng = R.ngrp[cs];
+                        for (int64_t g0 = 0; g0 < ng; g0 += 64) {
+                            const PairRef *grp = R.grp.data() + base + g0;
+                            job((int)std::min<int64_t>(64, ng - g0), R.split[base + g0],
+                                [&](int u) { return grp[u]; });
+                        }
                     }
                 }
                 cells_t[k] = cells;
@@ -1434,13 +1645,15 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
         }
         P->orient = orient;
         // longest jobs first for the persistent workers
-        std::vector<int64_t> gorder(gitems.size());
+        std::vector<int64_t> &gorder = scr->gorder;
+        gorder.resize(gitems.size());
         std::iota(gorder.begin(), gorder.end(), 0);
         std::stable_sort(gorder.begin(), gorder.end(), [&](int64_t x, int64_t y) {
             if (gitems[x].nb != gitems[y].nb) return gitems[x].nb < gitems[y].nb;
             return gcost[x] > gcost[y];
         });
-        std::vector<GroupItem> gjobs(gitems.size());
+        std::vector<GroupItem> &gjobs = scr->gjobs;
+        gjobs.resize(gitems.size());
         for (size_t i = 0; i < gorder.size(); ++i) gjobs[i] = gitems[gorder[i]];
         // split jobs are expanded into their chunk items (below) after the
         // tile items have taken their link rows
@@ -1593,7 +1806,7 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
                     {&P->d_gitems, P->gitems.data(), P->gitems.size() * sizeof(GroupItem), 0},
                     {&P->d_gents, P->gents.data(), P->gents.size() * sizeof(GroupEntry), 0},
                     {&P->d_mss, P->mss.data(), (size_t)n * sizeof(float), 0},
-                    {&P->d_pairs, nullptr, (size_t)n * sizeof(int2), 0},
+                    {&P->d_pairs, nullptr, P->cross ? 0 : (size_t)n * sizeof(int2), 0},
                     {&P->d_same, P->same.data(), (size_t)n, 0},
                     {&P->d_ckpairs, P->ckpairs.data(), P->ckpairs.size() * sizeof(CkPair), 0}};
     size_t total = 0;
@@ -2025,7 +2238,8 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
                            (const SeqRef *)P->seqs.p, P->nq, P->sc.ll, ctl, (const int *)W.gated.p,
                            (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
                            W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax,
-                           (const CkPair *)P->d_ckpairs.p, (int)P->ckpairs.size(), (const float *)P->d_ck.p);
+                           (const CkPair *)P->d_ckpairs.p, (int)P->ckpairs.size(), (const float *)P->d_ck.p,
+                           P->cross ? (int)P->nt : 0);
         HIPCHK(ctx, hipGetLastError());
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));

@@ -142,6 +142,13 @@ struct CkArgs {
     float *bmax = nullptr;
     int t0 = 0, t1 = -1;
 };
+// The block arguments for the tile instance of R residues per lane: only the
+// CK_R instance runs checkpoint blocks.
+template <int R>
+__device__ __forceinline__ CkArgs ck_for(const CkArgs &c) {
+    if constexpr (R == CK_R) return c;
+    else return CkArgs();
+}
 
 // ---------------------------------------------------------------------------
 // One tile job: A along lanes (stripes of 64*R rows), B swept.
@@ -168,6 +175,10 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     const int nstripes = (alen_it + rows - 1) / rows;
     const int nsteps = blen_it + WAVE - 1;
     const char *subB = reinterpret_cast<const char *>(subT);
+    // The gap scores as vector registers: a VOP2 add whose operands are all
+    // VGPRs dual-issues (measured on MI355X: ~1.4 wave instructions per SIMD
+    // quad-cycle with 4 waves), one that reads an SGPR does not (~0.9)
+    asm volatile("" : "+v"(G), "+v"(E));
     float rmax = 0.0f;
     // J candidate entering row 1: max(-1e10 + E, 0 + G), which both passes
     // keep clamped at 0 (see the recurrences below)
@@ -626,21 +637,22 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
         if (item >= nitems) break;
         const ScoreItem it = items[item];
         const int tr = trans_flags ? trans_flags[item] : 0;
-        float m;
-        switch (it.R) {
-        case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
-        case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
-        case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
-        default:
-            if constexpr (!AFF) {
-                if (it.ck_off >= 0) {
-                    m = score_item<AFF, CK_R, true>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links,
-                                                    prog, abort, wlimit, ckarea);
-                    break;
-                }
+        float m = 0.0f;
+        bool done = false;
+        if constexpr (!AFF) {
+            if (it.ck_off >= 0) {
+                m = score_item<AFF, CK_R, true>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links,
+                                                prog, abort, wlimit, ckarea);
+                done = true;
             }
-            m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit);
-            break;
+        }
+        if (!done) {
+            switch (it.R) {
+            case 1: m = score_item<AFF, 1>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+            case 2: m = score_item<AFF, 2>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+            case 4: m = score_item<AFF, 4>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+            default: m = score_item<AFF, 8>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links, prog, abort, wlimit); break;
+            }
         }
         m = wave_max(m);
         if (threadIdx.x == 0) {
@@ -942,9 +954,9 @@ __global__ void __launch_bounds__(64) dir_kernel(const uint8_t *__restrict__ res
         switch (it.R) {
         case 1: tile<AFF, 1, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
         case 2: tile<AFF, 2, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
-        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
+        case 4: tile<AFF, 4, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit, ck_for<4>(ckv)); break;
         case 6: tile<AFF, 6, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit); break;
-        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit, ckv); break;
+        default: tile<AFF, 8, true>(res, q, t, q.len, t.len, lds, lkp, bwin, stg, scr, G, E, fill, d, it.t1, it.mp, cc, cl, it.cand_cap, s0, s1, lk, pg, abort, wlimit, ck_for<8>(ckv)); break;
         }
     }
 }
@@ -1096,7 +1108,9 @@ __device__ __forceinline__ int ck_find(const CkPair *__restrict__ ckp, int nck, 
 // and its direction-pass work units (one per stripe for a split pair, whose
 // boundary rows and zeroed progress counters are allocated here too; without
 // room it runs unsplit; for a checkpointed pair, one per block it needs).  A
-// pair that does not fit is marked TASK_DEFERRED for a later round.
+// pair that does not fit is marked TASK_DEFERRED for a later round.  With
+// cross_nt > 0 the pairs are every query with every target, query-major, and
+// pairs is not read.
 // Allocations are summed over the wave: one atomic per wave and counter.
 __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
                                                    const int2 *__restrict__ pairs,
@@ -1108,7 +1122,7 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
                                                    int64_t units_cap, int64_t link_cap,
                                                    unsigned *__restrict__ prog, int64_t prog_cap, int rmax,
                                                    const CkPair *__restrict__ ckpairs, int nck,
-                                                   const float *__restrict__ ckarea) {
+                                                   const float *__restrict__ ckarea, int cross_nt) {
     const int ng = __hip_atomic_load(&ctl->ngated, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int stride = gridDim.x * blockDim.x;
     // whole waves iterate together (the scans need every lane)
@@ -1121,7 +1135,8 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
         float m = 0.0f;
         if (on) {
             const int i = gated[g];
-            const int2 pr = pairs[i];
+            // the all-vs-all batch has no pair list: pair i is query i / nt, target i % nt
+            const int2 pr = cross_nt > 0 ? make_int2(i / cross_nt, i % cross_nt) : pairs[i];
             ql = seqs[pr.x].len;
             tl = seqs[nq + pr.y].len;
             m = maxima[i];

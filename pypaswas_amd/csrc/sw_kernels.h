@@ -49,9 +49,9 @@ struct ScoreItem {
 // boundary row the scoring pass left above its stripe, with no hand-over
 // between waves.
 constexpr int CKB = 512;                 // steps per block (divides BWIN, a multiple of HANDOFF and of 64)
-constexpr int CK_R = 8;                  // residues per lane of a checkpointed pair
+constexpr int CK_R = 8;                  // residues per lane of a checkpointed pair (4 or 8)
 constexpr int CK_SLOTS = CK_R + 2;       // checkpoint values per lane (linear gaps)
-constexpr int CK_MIN_LEN = 2 * 64 * CK_R;  // both sequences at least this long (two stripes, two blocks)
+constexpr int CK_MIN_LEN = 1024;         // both sequences at least this long (two stripes, two blocks)
 struct CkPair {
     int32_t pair;
     int32_t nblocks;   // blocks per stripe: ceil((tlen + 63) / CKB)
