@@ -1293,9 +1293,15 @@ constexpr int WALK_LDS = WALK_WIN + WALK_QWIN + WALK_TWIN;
 struct PathWindow {
     uint8_t *blk, *qw, *tw;  // LDS
     int64_t dir_off, nsteps;
-    int R, s, t0, nt;
+    int R, lgR, s, t0, nt;  // lgR: log2 R, or -1 for R = 6
     SeqRef q, t;
 };
+
+// v / R for v >= 0 and R in {1, 2, 4, 8} (a shift) or 6 (a multiply by the
+// reciprocal: 0xAAAAAAAB / 2^34 divides every 32-bit unsigned value by 6 exactly)
+__device__ __forceinline__ int div_r(int v, int lgR) {
+    return lgR >= 0 ? (v >> lgR) : (int)(((uint64_t)(uint32_t)v * 0xAAAAAAABull) >> 34);
+}
 
 __device__ __forceinline__ void window_init(PathWindow &w, uint8_t *lds, const PairTask &it, SeqRef q, SeqRef t) {
     w.blk = lds;
@@ -1304,6 +1310,7 @@ __device__ __forceinline__ void window_init(PathWindow &w, uint8_t *lds, const P
     w.dir_off = it.dir_off;
     w.nsteps = (int64_t)t.len + WAVE - 1;
     w.R = it.R;
+    w.lgR = it.R == 1 ? 0 : it.R == 2 ? 1 : it.R == 4 ? 2 : it.R == 8 ? 3 : -1;
     w.s = -1;
     w.t0 = 0;
     w.nt = 0;
@@ -1317,7 +1324,7 @@ template <bool RES>
 __device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restrict__ dirs,
                                            const uint8_t *__restrict__ res, int x, int y) {
     const int R = w.R, rows = WAVE * R;
-    const int s = x / rows, rem = x - s * rows, k = rem / R, r = rem - k * R;
+    const int s = div_r(x >> 6, w.lgR), rem = x - s * rows, k = div_r(rem, w.lgR), r = rem - k * R;
     const int t = y + k;
     if (s != w.s || t < w.t0 || t >= w.t0 + w.nt) {
         const int lane = threadIdx.x & 63;
@@ -1327,24 +1334,58 @@ __device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restr
         // lane 0's earlier stores (marks) reach memory before the wave reads
         // it back, and every lane's reads of the old window are done
         __syncthreads();
-        const uint8_t *src = dirs + w.dir_off + ((int64_t)s * w.nsteps + t0) * rows / 2;
-        const int bytes = nt * rows / 2;
-        if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
-            const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-            uint4 *d4 = reinterpret_cast<uint4 *>(w.blk);
-            for (int i = lane; i < bytes / 16; i += WAVE) d4[i] = s4[i];
-        } else {
-            for (int i = lane; i < bytes; i += WAVE) w.blk[i] = src[i];
+        // (a step holds 32R bytes and the area starts 16-byte aligned, so the
+        // window does too; loads go out eight at a time before their stores,
+        // one memory round trip per 8 KB instead of one per kilobyte)
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(dirs + w.dir_off + ((int64_t)s * w.nsteps + t0) * rows / 2);
+        uint4 *d4 = reinterpret_cast<uint4 *>(w.blk);
+        const int n16 = nt * rows / 32;
+        constexpr int BATCH = 8;
+        for (int i0 = 0; i0 < n16; i0 += BATCH * WAVE) {
+            // unconditional loads (indices clamped into the window), so that
+            // the batch stays in registers
+            uint4 v[BATCH];
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b) v[b] = s4[min(i0 + b * WAVE + lane, n16 - 1)];
+            // (every value taken here: otherwise the compiler sinks each load
+            // into its store's branch, one round trip per load again)
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b) asm volatile("" : "+v"(v[b].x), "+v"(v[b].y), "+v"(v[b].z), "+v"(v[b].w));
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b) {
+                const int i = i0 + b * WAVE + lane;
+                if (i < n16) d4[i] = v[b];
+            }
         }
         if (RES) {
-            for (int i = lane; i < rows; i += WAVE) {
-                const int xx = s * rows + i;
-                w.qw[i] = xx < w.q.len ? res[w.q.off + xx] : (uint8_t)0;
+            // the stripe's query residues (64R <= WALK_QWIN) and the window's
+            // target columns (nt + 63 <= WALK_TWIN), every load before any store
+            constexpr int QB = WALK_QWIN / WAVE, TB = WALK_TWIN / WAVE + 1;
+            int qv[QB], tv[TB];
+            // (clamped indices: every load is made, the value kept only in range)
+            const int qn = (int)w.q.len, tn = (int)w.t.len;
+#pragma unroll
+            for (int b = 0; b < QB; ++b) {
+                const int i = b * WAVE + lane, xx = s * rows + i;
+                const uint8_t c = res[w.q.off + min(max(xx, 0), max(qn - 1, 0))];
+                qv[b] = (i < rows && xx < qn) ? (int)c : 0;
             }
-            for (int i = lane; i < nt + WAVE - 1; i += WAVE) {
-                const int yy = t0 - (WAVE - 1) + i;
-                w.tw[i] = (yy >= 0 && yy < w.t.len) ? res[w.t.off + yy] : (uint8_t)0;
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                const int i = b * WAVE + lane, yy = t0 - (WAVE - 1) + i;
+                const uint8_t c = res[w.t.off + min(max(yy, 0), max(tn - 1, 0))];
+                tv[b] = (i < nt + WAVE - 1 && yy >= 0 && yy < tn) ? (int)c : 0;
             }
+#pragma unroll
+            for (int b = 0; b < QB; ++b) asm volatile("" : "+v"(qv[b]));
+#pragma unroll
+            for (int b = 0; b < TB; ++b) asm volatile("" : "+v"(tv[b]));
+#pragma unroll
+            for (int b = 0; b < QB; ++b)
+                if (b * WAVE + lane < rows) w.qw[b * WAVE + lane] = (uint8_t)qv[b];
+#pragma unroll
+            for (int b = 0; b < TB; ++b)
+                if (b * WAVE + lane < nt + WAVE - 1) w.tw[b * WAVE + lane] = (uint8_t)tv[b];
         }
         __syncthreads();
         w.s = s;
@@ -1358,18 +1399,18 @@ __device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restr
 // in the window.
 template <bool RES>
 __device__ __forceinline__ uint32_t window_read(PathWindow &w, const uint8_t *dirs, const uint8_t *res, int x, int y,
-                                                int &idx) {
+                                                int &idx, uint32_t &byte) {
     idx = window_cell<RES>(w, dirs, res, x, y);
-    const uint32_t byte = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx >> 1]);
+    byte = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx >> 1]);
     return (byte >> ((uint32_t)(idx & 1) << 2)) & 15u;
 }
 
-// Replaces the nibble at window index idx with b (its byte's global offset
-// derived from the window's stripe and steps): LDS by every lane, memory by
-// lane 0.
-__device__ __forceinline__ void window_write(PathWindow &w, uint8_t *__restrict__ dirs, int idx, uint32_t b) {
+// Replaces the nibble at window index idx with b, old being its byte as read
+// (its global offset derived from the window's stripe and steps): LDS by every
+// lane, memory by lane 0.
+__device__ __forceinline__ void window_write(PathWindow &w, uint8_t *__restrict__ dirs, int idx, uint32_t b,
+                                             uint32_t old) {
     const uint32_t sh = (uint32_t)(idx & 1) << 2;
-    const uint32_t old = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.blk[idx >> 1]);
     const uint8_t nb = (uint8_t)((old & ~(15u << sh)) | ((b & 15u) << sh));
     w.blk[idx >> 1] = nb;
     if ((threadIdx.x & 63) == 0)
@@ -1490,7 +1531,8 @@ __global__ void __launch_bounds__(64) tb_wave_kernel(const SeqRef *__restrict__ 
             const int y = __builtin_amdgcn_readfirstlane(ci.y);
             const float v = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ci.v)));
             int idx;
-            const uint32_t b = window_read<false>(w, dirs, nullptr, x, y, idx);
+            uint32_t byte;
+            const uint32_t b = window_read<false>(w, dirs, nullptr, x, y, idx, byte);
             float score = 0.0f;
             bool emit = false;
             if (!(b & DB_MARK)) {
@@ -1498,19 +1540,22 @@ __global__ void __launch_bounds__(64) tb_wave_kernel(const SeqRef *__restrict__ 
                     emit = true;
                     score = v;
                     // mark the path
-                    int cx = x, cy = y;
+                    // (the predecessor's nibble, read for its mark, is the next
+                    // cell's: read once; writes go to the cell just read, so the
+                    // window holds it when the next read needs it)
+                    int cx = x, cy = y, o = idx;
+                    uint32_t cb = b, cbyte = byte;
                     for (;;) {
-                        int o;
-                        const uint32_t cb = window_read<false>(w, dirs, nullptr, cx, cy, o);
-                        window_write(w, dirs, o, cb | DB_MARK);
+                        window_write(w, dirs, o, cb | DB_MARK, cbyte);
                         const uint32_t code = cb & 3u;
                         if (cb & DB_STOP) break;
                         if (code == 0u) break;
                         if (AFF && code != 1u) break;
                         const int px = cx - (code != 2u ? 1 : 0), py = cy - (code != 3u ? 1 : 0);
                         if (px < 0 || py < 0) break;
-                        int po;
-                        if (window_read<false>(w, dirs, nullptr, px, py, po) & DB_MARK) break;
+                        const uint32_t pb = window_read<false>(w, dirs, nullptr, px, py, o, cbyte);
+                        if (pb & DB_MARK) break;
+                        cb = pb;
                         cx = px;
                         cy = py;
                     }
@@ -1727,7 +1772,8 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             int s_start = s_end + 1, t_start = t_end + 1;
             int matches = 0, mismatches = 0, len = 0;
             int idx;
-            int direction = eff_dir<AFF>(window_read<true>(w, dirs, res, x, y, idx));
+            uint32_t byte;
+            int direction = eff_dir<AFF>(window_read<true>(w, dirs, res, x, y, idx, byte));
             bool show = true;
             if (it.same_id) {
                 direction = 64;
@@ -1738,15 +1784,16 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             bool term = false;
             while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
                 int o;
-                const uint32_t b = window_read<true>(w, dirs, res, x, y, o);
+                const uint32_t b = window_read<true>(w, dirs, res, x, y, o, byte);
+                // the window holds the residues of the cell just read (read
+                // beside its nibble, used on diagonal moves)
+                const int qa = __builtin_amdgcn_readfirstlane((int)w.qw[x - w.s * WAVE * w.R]);
+                const int tb = __builtin_amdgcn_readfirstlane((int)w.tw[y - (w.t0 - (WAVE - 1))]);
                 direction = eff_dir<AFF>(b);
-                window_write(w, dirs, o, nib_walked(b));
+                window_write(w, dirs, o, nib_walked(b), byte);
                 if (direction == 96) {
                     show = false;
                 } else if (direction == 1 || direction == 64) {
-                    // the window holds the residues of the cell just read
-                    const int qa = __builtin_amdgcn_readfirstlane((int)w.qw[x - w.s * WAVE * w.R]);
-                    const int tb = __builtin_amdgcn_readfirstlane((int)w.tw[y - (w.t0 - (WAVE - 1))]);
                     if (qa == tb) ++matches; else ++mismatches;
                     if (lead) op[nops] = 'D';
                     ++nops;
