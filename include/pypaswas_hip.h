@@ -67,7 +67,7 @@ typedef struct {
     double ms_total;                     /* wall time of the call, host clock */
     int64_t score_launches;
     double ms_readback;                  /* host: reading tasks and records back (incl. copies) */
-    double ms_tail;                      /* host: tail rounds of pairs that did not fit the work areas */
+    double ms_tail;                      /* further rounds of pairs that did not fit the work areas (on the device, then on the host) */
     double ms_assemble;                  /* host: records into pair order */
     double ms_plan;                      /* host: encoding and work list of a one-shot call (pw_align_pairs) */
     double ms_upload;                    /* host: staging and copy of the plan to the device (one-shot call) */

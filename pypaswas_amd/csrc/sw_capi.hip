@@ -89,8 +89,9 @@ struct Limits {
 // read-back.  want_* are the sizes the next execution allocates.
 struct Workspace {
     DevBuf ctl, maxima, gated, tasks, touts, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch,
-        dscratch, units, dlinks, dprog;
+        dscratch, units, dlinks, dprog, rq_slots, rq_needs;
     int64_t want_dir = 0, want_cand = 0, want_ops = 0;
+    int64_t ops_kept = 0;  // ops bytes carried over when the area grew between device rounds
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
     int64_t units_cap = 0, dlinks_cap = 0, dprog_cap = 0;
@@ -426,7 +427,7 @@ void free_pinned(PinBuf &b) {
 void free_workspace(pw_ctx *ctx, Workspace &W) {
     DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.touts, &W.dirs, &W.cands, &W.spslot, &W.keys,
                       &W.out_sps, &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks,
-                      &W.dprog};
+                      &W.dprog, &W.rq_slots, &W.rq_needs};
     for (DevBuf *b : bufs) free_buf(ctx, *b);
     PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_touts, &W.h_sps, &W.h_hits, &W.h_ops};
     for (PinBuf *b : pins) free_pinned(*b);
@@ -1960,6 +1961,8 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     ENSURE(ctx, W.tasks, (size_t)n * sizeof(PairTask));
     ENSURE(ctx, W.touts, (size_t)n * sizeof(TaskOut));
     ENSURE(ctx, W.gated, (size_t)n * sizeof(int));
+    ENSURE(ctx, W.rq_slots, (size_t)n * sizeof(int));
+    ENSURE(ctx, W.rq_needs, (size_t)n * sizeof(int));
     if (W.dir_ext) {
         W.dir_cap = W.dir_ext_cap;
     } else {
@@ -2149,13 +2152,14 @@ int launch_walk(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
     auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
     hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                        (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
-                       (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p);
+                       (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p, W.cand_cap);
     HIPCHK(ctx, hipGetLastError());
     if (g.wave_grid > 0) {
         auto ww = aff ? walk_wave_kernel<true> : walk_wave_kernel<false>;
         hipLaunchKernelGGL(ww, dim3(g.wave_grid), dim3(64), WALK_LDS, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                            (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
-                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p);
+                           (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p,
+                           W.cand_cap);
         HIPCHK(ctx, hipGetLastError());
     }
     return PW_OK;
@@ -2207,6 +2211,21 @@ int launch_clears(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L) {
     return rc;
 }
 
+// Task kernel over the gated list (first round) or over the list of requeued
+// tasks with their read-back slots and candidate needs (requeued).
+int launch_tasks(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, bool requeued) {
+    hipLaunchKernelGGL(task_kernel, dim3(g.build_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
+                       (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
+                       (const SeqRef *)P->seqs.p, P->nq, P->sc.ll, (Ctl *)W.ctl.p, (const int *)W.gated.p,
+                       (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
+                       W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax,
+                       (const CkPair *)P->d_ckpairs.p, (int)P->ckpairs.size(), (const float *)P->d_ck.p,
+                       P->cross ? (int)P->nt : 0, requeued ? (const int *)W.rq_slots.p : (const int *)nullptr,
+                       requeued ? (const int *)W.rq_needs.p : (const int *)nullptr);
+    HIPCHK(ctx, hipGetLastError());
+    return PW_OK;
+}
+
 // The whole device sequence of one execution (capturable: kernels, event
 // fork/join and one device-to-host copy into pinned memory).
 int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, bool events,
@@ -2233,14 +2252,8 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
         hipLaunchKernelGGL(gate_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
                            (const float *)P->d_mss.p, P->n_pairs, P->sc.minimum, ctl, (int *)W.gated.p);
         HIPCHK(ctx, hipGetLastError());
-        hipLaunchKernelGGL(task_kernel, dim3(g.build_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
-                           (const float *)P->d_mss.p, (const int2 *)P->d_pairs.p, (const uint8_t *)P->d_same.p,
-                           (const SeqRef *)P->seqs.p, P->nq, P->sc.ll, ctl, (const int *)W.gated.p,
-                           (PairTask *)W.tasks.p, W.dir_cap, W.cand_cap, W.cand0, (DirUnit *)W.units.p, W.units_cap,
-                           W.dlinks_cap, (unsigned *)W.dprog.p, W.dprog_cap, P->dir_rmax,
-                           (const CkPair *)P->d_ckpairs.p, (int)P->ckpairs.size(), (const float *)P->d_ck.p,
-                           P->cross ? (int)P->nt : 0);
-        HIPCHK(ctx, hipGetLastError());
+        int rc2 = launch_tasks(ctx, P, W, L, g, false);
+        if (rc2) return rc2;
     }
     if (events) HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
     rc = launch_traceback(ctx, P, W, L, g, minimum, events);
@@ -2314,8 +2327,12 @@ int take_maxima_buffer(pw_ctx *ctx, int64_t n, HostResult *R) {
 
 // Reads the tasks and compacted records of a finished device sequence whose
 // control block is in W.h_ctl; finished pairs go to outs, the others to pending.
+// After device rounds (n_last >= 0) the summaries cover every slot of the
+// first round, and the tasks still unfinished are those of the last round
+// (tasks[0, n_last)) that did not finish; otherwise they are tasks[i] for the
+// summaries i that did not finish.
 int read_tasks(pw_ctx *ctx, Workspace &W, Lane &L, int ntasks, std::vector<PairOut> &outs,
-               std::vector<PairTask> &pending) {
+               std::vector<PairTask> &pending, int n_last = -1) {
     const Ctl &c = *(const Ctl *)W.h_ctl.p;
     if (ntasks <= 0) return PW_OK;
     const int64_t nout = std::min<int64_t>((int64_t)c.out_used, W.cand_cap);
@@ -2340,16 +2357,20 @@ int read_tasks(pw_ctx *ctx, Workspace &W, Lane &L, int ntasks, std::vector<PairO
     for (int i = 0; i < ntasks && !unfinished; ++i) unfinished = T[i].status != TASK_OK;
     const PairTask *full = nullptr;
     if (unfinished) {
-        ENSURE_PINNED(ctx, W.h_tasks, (size_t)ntasks * sizeof(PairTask));
-        HIPCHK(ctx, hipMemcpyAsync(W.h_tasks.p, W.tasks.p, (size_t)ntasks * sizeof(PairTask), hipMemcpyDeviceToHost,
+        const int nfull = n_last >= 0 ? n_last : ntasks;
+        ENSURE_PINNED(ctx, W.h_tasks, (size_t)std::max(1, nfull) * sizeof(PairTask));
+        HIPCHK(ctx, hipMemcpyAsync(W.h_tasks.p, W.tasks.p, (size_t)nfull * sizeof(PairTask), hipMemcpyDeviceToHost,
                                    L.s));
         HIPCHK(ctx, hipStreamSynchronize(L.s));
         full = (const PairTask *)W.h_tasks.p;
+        if (n_last >= 0)
+            for (int i = 0; i < n_last; ++i)
+                if (full[i].status != TASK_OK) pending.push_back(full[i]);
     }
     for (int i = 0; i < ntasks; ++i) {
         const TaskOut &t = T[i];
         if (t.status != TASK_OK) {
-            pending.push_back(full[i]);
+            if (n_last < 0) pending.push_back(full[i]);
             continue;
         }
         if ((int64_t)t.out_off + t.nsp > nout || t.ops_off + t.nops > nops) {
@@ -2467,6 +2488,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         std::vector<DirUnit> units((size_t)nc);
         for (int u = 0; u < nc; ++u) {
             chunk[u].split = 0;  // tail rounds run each pair's stripes in one unit
+            chunk[u].gslot = u;  // read back by this round's summaries
             units[u] = DirUnit{u, -1, -1};
         }
         Ctl c;
@@ -2540,8 +2562,12 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
 }
 
 // Sizes for the next execution after one that overflowed its work areas.
-void adapt_workspace(pw_ctx *ctx, Workspace &W, const Ctl &c, const std::vector<PairTask> &pending) {
-    if (!ctx->lim.grow || pending.empty()) return;
+// c: the first round's control block (what the direction and candidate
+// areas were asked for); last: that of the last device round (ops asked for
+// over every round).
+void adapt_workspace(pw_ctx *ctx, Workspace &W, const Ctl &c, const Ctl &last,
+                     const std::vector<PairTask> &pending) {
+    if (!ctx->lim.grow) return;
     W.want_dir = std::min<int64_t>(std::max<int64_t>(W.want_dir, (int64_t)c.dir_used), dir_limit(ctx));
     int32_t maxc = W.cand0;
     for (const PairTask &t : pending)
@@ -2549,7 +2575,7 @@ void adapt_workspace(pw_ctx *ctx, Workspace &W, const Ctl &c, const std::vector<
     W.cand0 = std::min<int32_t>(maxc, 1024);
     W.want_cand = std::min(std::max<int64_t>({W.want_cand, (int64_t)c.cand_used, (int64_t)c.ngated * W.cand0}),
                            cand_slot_limit(ctx));
-    W.want_ops = std::max<int64_t>(W.want_ops, (int64_t)c.ops_used);
+    W.want_ops = std::max<int64_t>({W.want_ops, (int64_t)c.ops_used, (int64_t)last.ops_used});
 }
 
 // Per-pair records in pair order into the host result.
@@ -2743,6 +2769,7 @@ struct Finished {
     int ng = 0;
     double ms_score = 0, ms_dir = 0, ms_tb = 0, ms_walk = 0;
     double ms_readback = 0, ms_tail = 0, ms_assemble = 0;  // host phases
+    double ms_device_rounds = 0;                           // further rounds on the device (wall)
 };
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -2809,6 +2836,86 @@ int execute_launch(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimu
 
 // Waits for the launched execution, reads its tasks and records back and runs
 // the tail rounds of the pairs that did not fit.
+// Rounds after the first on the device: requeue_kernel lists the unfinished
+// tasks, round_reset_kernel restarts the allocators, then the task kernel
+// (over that list), direction pass, start selection and walk run as in the
+// first round.  Stops when every task finished, when a round finished none
+// (n_last: that round's task count, its unfinished tasks left for the host),
+// or after MAX_DEVICE_ROUNDS.
+constexpr int MAX_DEVICE_ROUNDS = 64;
+
+// Replaces the ops area by one sized for `used` bytes plus as much again
+// beyond the old capacity (what the failed allocations asked for), the old
+// area's contents copied over; W.ops_kept = the old capacity, where the
+// allocator restarts.
+int grow_ops(pw_ctx *ctx, Workspace &W, Lane &L, int64_t used) {
+    const int64_t keep = W.ops_cap;
+    const int64_t need = std::min<int64_t>(used + (used - keep), std::max<int64_t>(used, dir_limit(ctx)));
+    DevBuf old = W.ops;
+    W.ops = DevBuf{};
+    const int rc = ensure(ctx, W.ops, (size_t)need);
+    if (rc) {
+        W.ops = old;
+        return rc;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(W.ops.p, old.p, (size_t)keep, hipMemcpyDeviceToDevice, L.s));
+    HIPCHK(ctx, hipStreamSynchronize(L.s));
+    if (ctx->defer_free) ctx->dead_dev.emplace_back(old.p, old.cap);
+    else dev_release(ctx, old.p, old.cap);
+    W.ops_cap = (int64_t)W.ops.cap;
+    W.ops_kept = keep;
+    W.want_ops = std::max<int64_t>(W.want_ops, need);
+    return PW_OK;
+}
+int device_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, Finished &F,
+                  int &n_last) {
+    Ctl *dctl = (Ctl *)W.ctl.p;
+    Ctl c = *(const Ctl *)W.h_ctl.p;
+    int prev = c.ngated;
+    n_last = prev;
+    const int rq_grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)prev + 255) / 256, (int64_t)ctx->num_cus * 4));
+    for (int round = 0; round < MAX_DEVICE_ROUNDS && c.unfinished > 0; ++round) {
+        hipLaunchKernelGGL(requeue_kernel, dim3(rq_grid), dim3(256), 0, L.s, (const PairTask *)W.tasks.p, dctl,
+                           (int *)W.gated.p, (int *)W.rq_slots.p, (int *)W.rq_needs.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, dctl, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        c = *(const Ctl *)W.h_ctl.p;
+        const int next = c.next_ngated;
+        if (next <= 0 || next >= prev) break;  // the round just walked finished nothing
+        // the record area full: no later round can place records
+        if ((int64_t)c.out_used > W.cand_cap) break;
+        // the ops area full: grown (its records copied over) where growth is on
+        long long ops_keep = -1;
+        if ((int64_t)c.ops_used > W.ops_cap) {
+            if (!ctx->lim.grow) break;
+            const int rc = grow_ops(ctx, W, L, (int64_t)c.ops_used);
+            if (rc) return rc;
+            ops_keep = (long long)W.ops_kept;
+        }
+        hipLaunchKernelGGL(round_reset_kernel, dim3(1), dim3(64), 0, L.s, dctl, next, ops_keep);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
+        int rc = launch_tasks(ctx, P, W, L, g, true);
+        if (!rc) rc = launch_traceback(ctx, P, W, L, g, minimum, true);
+        if (rc) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, dctl, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        F.ms_dir += elapsed(L.ev[2], L.ev[3]);
+        F.ms_tb += elapsed(L.ev[3], L.ev[4]);
+        F.ms_walk += elapsed(L.ev[4], L.ev[5]);
+        c = *(const Ctl *)W.h_ctl.p;
+        if (c.abort_wait) {
+            ctx->err = "a stripe or chunk hand-over wait on the device gave up after " +
+                       std::to_string(ctx->wait_polls) + " polls; the execution's results are discarded";
+            return PW_EDEVICE;
+        }
+        prev = next;
+        n_last = next;
+    }
+    return PW_OK;
+}
+
 int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X, Finished &F) {
     HIPCHK(ctx, hipStreamSynchronize(L.s));
     if (X.graph) {
@@ -2830,20 +2937,31 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
         ctx->err = "gate count out of range";
         return PW_EDEVICE;
     }
-    std::vector<PairTask> pending;
+    // pairs that did not fit (directions, candidates, records) run further
+    // rounds on the device, the records collecting in the read-back areas;
+    // what a round cannot finish goes to the host's tail rounds
+    int n_last = -1;
     auto t0 = std::chrono::steady_clock::now();
-    int rc = read_tasks(ctx, W, L, F.ng, F.outs, pending);
+    if (c.unfinished > 0) {
+        int rc = device_rounds(ctx, P, W, L, X.g, X.minimum, F, n_last);
+        if (rc) return rc;
+    }
+    F.ms_device_rounds = ms_since(t0);
+    std::vector<PairTask> pending;
+    t0 = std::chrono::steady_clock::now();
+    int rc = read_tasks(ctx, W, L, F.ng, F.outs, pending, n_last);
     if (rc) return rc;
     F.ms_readback = ms_since(t0);
     t0 = std::chrono::steady_clock::now();
+    // areas sized for what this execution asked for, for the next one
+    if (c.unfinished > 0) adapt_workspace(ctx, W, c, *(const Ctl *)W.h_ctl.p, pending);
     if (!pending.empty()) {
         // the tail rounds reuse the staging buffers: keep copies of what they hold
         F.store.keep(F.outs, 0);
-        adapt_workspace(ctx, W, c, pending);
         rc = tail_rounds(ctx, P, W, L, X.g, X.minimum, pending, F.outs, F.store, &F.ms_dir, &F.ms_tb, &F.ms_walk);
         if (rc) return rc;
     }
-    F.ms_tail = ms_since(t0);
+    F.ms_tail = ms_since(t0) + F.ms_device_rounds;  // further rounds: on the device, then on the host
     t0 = std::chrono::steady_clock::now();
     sort_by_pair(F.outs);
     F.ms_assemble = ms_since(t0);

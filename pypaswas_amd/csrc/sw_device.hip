@@ -1110,7 +1110,9 @@ __device__ __forceinline__ int ck_find(const CkPair *__restrict__ ckp, int nck, 
 // room it runs unsplit; for a checkpointed pair, one per block it needs).  A
 // pair that does not fit is marked TASK_DEFERRED for a later round.  With
 // cross_nt > 0 the pairs are every query with every target, query-major, and
-// pairs is not read.
+// pairs is not read.  A requeued round (requeue_kernel) passes each pair's
+// read-back slot and candidate need in slots and needs; the first round takes
+// slot g and cand0 candidate slots per pair.
 // Allocations are summed over the wave: one atomic per wave and counter.
 __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
                                                    const int2 *__restrict__ pairs,
@@ -1122,7 +1124,8 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
                                                    int64_t units_cap, int64_t link_cap,
                                                    unsigned *__restrict__ prog, int64_t prog_cap, int rmax,
                                                    const CkPair *__restrict__ ckpairs, int nck,
-                                                   const float *__restrict__ ckarea, int cross_nt) {
+                                                   const float *__restrict__ ckarea, int cross_nt,
+                                                   const int *__restrict__ slots, const int *__restrict__ needs) {
     const int ng = __hip_atomic_load(&ctl->ngated, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int stride = gridDim.x * blockDim.x;
     // whole waves iterate together (the scans need every lane)
@@ -1152,7 +1155,7 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
             t.t1 = ll * m;
             t.mp = mss[i];
             t.maxv = m;
-            t.cand_cap = cand0;
+            t.cand_cap = needs ? max(cand0, needs[g]) : cand0;
             t.ncand = 0;
             t.nsp = 0;
             t.same_id = same_id[i];
@@ -1163,14 +1166,15 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
             t.rows_off = 0;
             t.prog_off = 0;
             t.ck = ck;
-            t.pad = 0;
+            t.gslot = slots ? slots[g] : g;
             if (ck >= 0)
                 ckn = ck_units(ckpairs[ck], ckarea, (ql + WAVE * CK_R - 1) / (WAVE * CK_R),
                                __builtin_fmaxf(t.t1, t.mp), nullptr, g);
         }
         const unsigned long long doff = wave_alloc(&ctl->dir_used, on ? (uint64_t)area : 0ull);
-        const unsigned long long coff = wave_alloc(&ctl->cand_used, on ? (uint64_t)cand0 : 0ull);
-        const bool ok = on && (int64_t)doff + area <= dir_cap && (int64_t)coff + cand0 <= cand_cap_total;
+        const int cneed = on ? t.cand_cap : 0;
+        const unsigned long long coff = wave_alloc(&ctl->cand_used, (uint64_t)cneed);
+        const bool ok = on && (int64_t)doff + area <= dir_cap && (int64_t)coff + cneed <= cand_cap_total;
         const int64_t rows = (ok && ns > 0) ? (int64_t)(ns - 1) * link_row_floats(tl) : 0;
         const unsigned long long lo = wave_alloc(&ctl->link_used, (uint64_t)rows);
         const unsigned long long po = wave_alloc(&ctl->prog_used, (ok && ns > 0) ? (uint64_t)ns : 0ull);
@@ -1204,6 +1208,62 @@ __global__ void __launch_bounds__(256) task_kernel(const float *__restrict__ max
             }
         }
         tasks[g] = t;
+    }
+}
+
+// Next round of an execution: the tasks of the round just walked that did not
+// finish (no room for their directions or candidates, a candidate count
+// beyond their slots, no room for their records) are listed for task_kernel,
+// each with its read-back slot and the candidate slots it needs (its exact
+// count after a candidate overflow).  The list replaces the gated list; the
+// count goes to ctl->next_ngated (one atomic per wave).
+__global__ void __launch_bounds__(256) requeue_kernel(const PairTask *__restrict__ tasks, Ctl *__restrict__ ctl,
+                                                      int *__restrict__ gated, int *__restrict__ slots,
+                                                      int *__restrict__ needs) {
+    const int n = task_count(ctl);
+    const int lane = threadIdx.x & 63;
+    const int stride = gridDim.x * blockDim.x;
+    for (int g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); g0 < n; g0 += stride) {
+        const int g = g0 + lane;
+        const bool redo = g < n && tasks[g].status != TASK_OK;
+        const uint64_t mask = __ballot(redo);
+        if (!mask) continue;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->next_ngated, __popcll(mask));
+        base = __shfl(base, 0, 64);
+        if (redo) {
+            const PairTask &t = tasks[g];
+            const int k = base + __popcll(mask & ((1ull << lane) - 1ull));
+            gated[k] = t.pair;
+            slots[k] = t.gslot;
+            needs[k] = t.status == TASK_CAND_OVERFLOW ? max(t.ncand, t.cand_cap) : t.cand_cap;
+        }
+    }
+}
+
+// Starts the next round (one thread): the ntasks requeued tasks (the count the
+// host read back) become the round's tasks and the traceback areas' allocators
+// restart; the record areas (out, ops) keep what earlier rounds wrote (after
+// the ops area grew, ops_keep >= 0 restarts its allocator there: every
+// allocation that succeeded ends at or below the old capacity).  The
+// control block is written with device-scope atomic stores, as every kernel
+// reads it with device-scope atomics: a plain access can be served by this
+// XCD's L2 from a line cached before the requeue's atomics (seen on MI355X:
+// a plain load of next_ngated returned 0 after the host had read the count).
+__global__ void round_reset_kernel(Ctl *__restrict__ ctl, int ntasks, long long ops_keep) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (ops_keep >= 0)
+            __hip_atomic_store(&ctl->ops_used, (unsigned long long)ops_keep, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->ngated, ntasks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->next_ngated, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->unfinished, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->dir_queue, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->dir_used, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->cand_used, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->link_used, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->prog_used, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->units_used, 0ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1615,12 +1675,17 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
                                                   uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
                                                   uint64_t *__restrict__ keys, SpRec *__restrict__ out_sps,
                                                   HitRec *__restrict__ out_hits, char *__restrict__ ops,
-                                                  int64_t ops_cap, double minimum, TaskOut *__restrict__ touts) {
+                                                  int64_t ops_cap, double minimum, TaskOut *__restrict__ touts,
+                                                  int64_t out_cap) {
     const int n = task_count(ctl);
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
         const PairTask it = tasks[g];
         if (it.status != TASK_OK) {
-            touts[g] = TaskOut{it.pair, it.status, 0, 0, 0, 0};
+            // (deferred or candidate overflow: counted once per wave for the next round)
+            touts[it.gslot] = TaskOut{it.pair, it.status, 0, 0, 0, 0};
+            const uint64_t m = __ballot(1);
+            if ((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1)
+                atomicAdd(&ctl->unfinished, (unsigned)__popcll(m));
             continue;
         }
         if (long_path_pair(seqs, it)) continue;
@@ -1629,12 +1694,16 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
         const int64_t ql = q.len, tl = t.len;
         const int64_t cap = nsp > 0 ? min((int64_t)nsp * (ql + tl), ql * tl + nsp) : 0;
         const unsigned long long ooff = cap > 0 ? atomicAdd(&ctl->ops_used, (unsigned long long)cap) : 0ull;
-        if ((int64_t)ooff + cap > ops_cap) {
+        const unsigned long long out0 =
+            ((int64_t)ooff + cap <= ops_cap && nsp > 0) ? atomicAdd(&ctl->out_used, (unsigned long long)nsp) : 0ull;
+        // no room for its op characters or records (the record areas collect
+        // every round of an execution): the pair goes to a later round
+        if ((int64_t)ooff + cap > ops_cap || (int64_t)out0 + nsp > out_cap) {
             tasks[g].status = TASK_OPS_OVERFLOW;
-            touts[g] = TaskOut{it.pair, TASK_OPS_OVERFLOW, 0, 0, 0, 0};
+            touts[it.gslot] = TaskOut{it.pair, TASK_OPS_OVERFLOW, 0, 0, 0, 0};
+            atomicAdd(&ctl->unfinished, 1u);
             continue;
         }
-        const unsigned long long out0 = nsp > 0 ? atomicAdd(&ctl->out_used, (unsigned long long)nsp) : 0ull;
         const SpRec *sp = sps + it.cand_off;
         uint64_t *key = keys + it.cand_off;
         for (int i = 0; i < nsp; ++i) {
@@ -1713,7 +1782,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint8_t *__restrict__ re
         tasks[g].ops_off = (int64_t)ooff;
         tasks[g].nops = nops;
         tasks[g].out_off = (int32_t)out0;
-        touts[g] = TaskOut{it.pair, TASK_OK, nsp, (int32_t)out0, (int64_t)ooff, nops};
+        touts[it.gslot] = TaskOut{it.pair, TASK_OK, nsp, (int32_t)out0, (int64_t)ooff, nops};
     }
 }
 
@@ -1725,7 +1794,8 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
                                                        uint8_t *__restrict__ dirs, const SpRec *__restrict__ sps,
                                                        uint64_t *__restrict__ keys, SpRec *__restrict__ out_sps,
                                                        HitRec *__restrict__ out_hits, char *__restrict__ ops,
-                                                       int64_t ops_cap, double minimum, TaskOut *__restrict__ touts) {
+                                                       int64_t ops_cap, double minimum, TaskOut *__restrict__ touts,
+                                                       int64_t out_cap) {
     extern __shared__ uint8_t wlds[];
     const int n = task_count(ctl);
     const int lane = threadIdx.x & 63;
@@ -1740,15 +1810,17 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
         unsigned long long ooff = 0, out0 = 0;
         if (lead && cap > 0) ooff = atomicAdd(&ctl->ops_used, (unsigned long long)cap);
         ooff = __shfl(ooff, 0, 64);
-        if ((int64_t)ooff + cap > ops_cap) {
+        if (lead && (int64_t)ooff + cap <= ops_cap && nsp > 0) out0 = atomicAdd(&ctl->out_used, (unsigned long long)nsp);
+        out0 = __shfl(out0, 0, 64);
+        // no room for its op characters or records: a later round
+        if ((int64_t)ooff + cap > ops_cap || (int64_t)out0 + nsp > out_cap) {
             if (lead) {
                 tasks[g].status = TASK_OPS_OVERFLOW;
-                touts[g] = TaskOut{it.pair, TASK_OPS_OVERFLOW, 0, 0, 0, 0};
+                touts[it.gslot] = TaskOut{it.pair, TASK_OPS_OVERFLOW, 0, 0, 0, 0};
+                atomicAdd(&ctl->unfinished, 1u);
             }
             continue;
         }
-        if (lead && nsp > 0) out0 = atomicAdd(&ctl->out_used, (unsigned long long)nsp);
-        out0 = __shfl(out0, 0, 64);
         const SpRec *sp = sps + it.cand_off;
         uint64_t *key = keys + it.cand_off;
         for (int i = lane; i < nsp; i += WAVE) out_sps[out0 + i] = sp[i];
@@ -1843,7 +1915,7 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             tasks[g].ops_off = (int64_t)ooff;
             tasks[g].nops = nops;
             tasks[g].out_off = (int32_t)out0;
-            touts[g] = TaskOut{it.pair, TASK_OK, nsp, (int32_t)out0, (int64_t)ooff, nops};
+            touts[it.gslot] = TaskOut{it.pair, TASK_OK, nsp, (int32_t)out0, (int64_t)ooff, nops};
         }
     }
 }

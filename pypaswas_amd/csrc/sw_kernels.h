@@ -130,7 +130,7 @@ struct PairTask {
     int64_t rows_off;    // split: stripe-boundary rows in the link area (floats)
     int64_t prog_off;    // split: progress counters in the counter area
     int32_t ck;          // checkpointed pair: its CkPair index (-1: none)
-    int32_t pad;
+    int32_t gslot;       // its slot in the read-back summaries (its index in the first round's gated list)
 };
 
 // What the host reads back per task: where the pair's records sit (written
@@ -162,6 +162,8 @@ struct Ctl {
     unsigned long long link_used, prog_used;  // split-pair areas of the direction pass
     long long units_used;                     // work units of the direction pass
     unsigned abort_wait;  // set by a wave whose stripe or chunk hand-over wait gave up (see WAIT_POLLS)
+    unsigned unfinished;  // tasks of the round left unfinished (no room, or a candidate or record overflow)
+    int next_ngated;      // tasks requeued for the next round (requeue_kernel)
 };
 
 // Polls a wave makes on a hand-over counter before it gives up, sets
