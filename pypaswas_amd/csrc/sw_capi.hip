@@ -94,6 +94,7 @@ struct Workspace {
     int64_t ops_kept = 0;  // ops bytes carried over when the area grew between device rounds
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
+    int64_t out_cap = 0;                             // records the read-back areas (out_sps, out_hits) hold
     int64_t units_cap = 0, dlinks_cap = 0, dprog_cap = 0;
     PinBuf h_ctl, h_tasks, h_touts, h_sps, h_hits, h_ops;
     // direction area lent by a caller (a range of a serving arena) instead of dirs
@@ -102,6 +103,15 @@ struct Workspace {
 };
 
 uint8_t *dir_ptr(const Workspace &W) { return W.dir_ext ? W.dir_ext : (uint8_t *)W.dirs.p; }
+
+// Candidate slots (cands, spslot, keys: one round's) and record slots
+// (out_sps, out_hits: every round's) of the current buffers.
+void set_slot_caps(Workspace &W) {
+    W.cand_cap = std::min<int64_t>({(int64_t)(W.cands.cap / sizeof(Cand)), (int64_t)(W.spslot.cap / sizeof(SpRec)),
+                                    (int64_t)(W.keys.cap / sizeof(uint64_t)), (int64_t)INT32_MAX});
+    W.out_cap = std::min<int64_t>({(int64_t)(W.out_sps.cap / sizeof(SpRec)),
+                                   (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
+}
 
 }  // namespace
 
@@ -1980,10 +1990,7 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     ENSURE(ctx, W.keys, (size_t)ccap * sizeof(uint64_t));
     ENSURE(ctx, W.out_sps, (size_t)ccap * sizeof(SpRec));
     ENSURE(ctx, W.out_hits, (size_t)ccap * sizeof(HitRec));
-    W.cand_cap = std::min<int64_t>({(int64_t)(W.cands.cap / sizeof(Cand)), (int64_t)(W.spslot.cap / sizeof(SpRec)),
-                                    (int64_t)(W.keys.cap / sizeof(uint64_t)),
-                                    (int64_t)(W.out_sps.cap / sizeof(SpRec)),
-                                    (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
+    set_slot_caps(W);
     ENSURE(ctx, W.ops, (size_t)std::max<int64_t>(16, W.want_ops));
     W.ops_cap = (int64_t)W.ops.cap;
     // direction-pass work units: sized for every pair passing the gate; the
@@ -2152,14 +2159,14 @@ int launch_walk(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
     auto wk = aff ? walk_kernel<true> : walk_kernel<false>;
     hipLaunchKernelGGL(wk, dim3(g.task_grid), dim3(64), 0, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                        (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p, (HitRec *)W.out_hits.p,
-                       (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p, W.cand_cap);
+                       (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p, W.out_cap);
     HIPCHK(ctx, hipGetLastError());
     if (g.wave_grid > 0) {
         auto ww = aff ? walk_wave_kernel<true> : walk_wave_kernel<false>;
         hipLaunchKernelGGL(ww, dim3(g.wave_grid), dim3(64), WALK_LDS, L.s, res, seqs, tasks, ctl, dir_ptr(W),
                            (const SpRec *)W.spslot.p, (uint64_t *)W.keys.p, (SpRec *)W.out_sps.p,
                            (HitRec *)W.out_hits.p, (char *)W.ops.p, W.ops_cap, minimum, (TaskOut *)W.touts.p,
-                           W.cand_cap);
+                           W.out_cap);
         HIPCHK(ctx, hipGetLastError());
     }
     return PW_OK;
@@ -2335,7 +2342,7 @@ int read_tasks(pw_ctx *ctx, Workspace &W, Lane &L, int ntasks, std::vector<PairO
                std::vector<PairTask> &pending, int n_last = -1) {
     const Ctl &c = *(const Ctl *)W.h_ctl.p;
     if (ntasks <= 0) return PW_OK;
-    const int64_t nout = std::min<int64_t>((int64_t)c.out_used, W.cand_cap);
+    const int64_t nout = std::min<int64_t>((int64_t)c.out_used, W.out_cap);
     const int64_t nops = std::min<int64_t>((int64_t)c.ops_used, W.ops_cap);
     // per-task summaries (32 bytes; the whole 120-byte tasks only when some
     // did not finish and go to a tail round)
@@ -2443,20 +2450,18 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
                 W.dir_cap = (int64_t)W.dirs.cap;
             }
         }
+        // a round's records are at most its candidates: both areas take cwant
         const int64_t cwant = std::max<int64_t>((int64_t)f.cand_cap, std::min(cneed, cand_limit));
-        if (cwant > W.cand_cap) {
+        if (cwant > std::min(W.cand_cap, W.out_cap)) {
             const size_t c = (size_t)cwant;
             ENSURE(ctx, W.cands, c * sizeof(Cand));
             ENSURE(ctx, W.spslot, c * sizeof(SpRec));
             ENSURE(ctx, W.keys, c * sizeof(uint64_t));
             ENSURE(ctx, W.out_sps, c * sizeof(SpRec));
             ENSURE(ctx, W.out_hits, c * sizeof(HitRec));
-            W.cand_cap = std::min<int64_t>({(int64_t)(W.cands.cap / sizeof(Cand)),
-                                            (int64_t)(W.spslot.cap / sizeof(SpRec)),
-                                            (int64_t)(W.keys.cap / sizeof(uint64_t)),
-                                            (int64_t)(W.out_sps.cap / sizeof(SpRec)),
-                                            (int64_t)(W.out_hits.cap / sizeof(HitRec)), (int64_t)INT32_MAX});
+            set_slot_caps(W);
         }
+        const int64_t slot_cap = std::min(W.cand_cap, W.out_cap);
         std::vector<PairTask> chunk;
         int64_t dused = 0, cused = 0;
         const int64_t max_take = std::min<int64_t>(P->n_pairs, W.units_cap);
@@ -2464,7 +2469,7 @@ int tail_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &
         for (; k < avail && (int64_t)chunk.size() < max_take; ++k) {
             PairTask t = at(k);
             const int64_t a = area_of(t);
-            if (dused + a > W.dir_cap || cused + t.cand_cap > W.cand_cap) break;
+            if (dused + a > W.dir_cap || cused + t.cand_cap > slot_cap) break;
             t.dir_off = dused;
             t.cand_off = cused;
             t.status = TASK_OK;
@@ -2734,6 +2739,7 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     add((uint64_t)(uintptr_t)dir_ptr(W));
     add((uint64_t)W.dir_cap);
     add((uint64_t)W.cand_cap);
+    add((uint64_t)W.out_cap);
     add((uint64_t)W.ops_cap);
     add((uint64_t)W.cand0);
     add((uint64_t)W.units_cap);
@@ -2844,6 +2850,25 @@ int execute_launch(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimu
 // or after MAX_DEVICE_ROUNDS.
 constexpr int MAX_DEVICE_ROUNDS = 64;
 
+// Direction and candidate areas for the next device round: what the last
+// round asked for (its allocators count every request, met or not), within
+// the limits.
+int grow_round_areas(pw_ctx *ctx, Workspace &W, const Ctl &c) {
+    if (!W.dir_ext && (int64_t)c.dir_used > W.dir_cap && W.dir_cap < dir_limit(ctx)) {
+        ENSURE(ctx, W.dirs, (size_t)std::min<int64_t>((int64_t)c.dir_used, dir_limit(ctx)));
+        W.dir_cap = (int64_t)W.dirs.cap;
+    }
+    const int64_t cl = cand_slot_limit(ctx);
+    if ((int64_t)c.cand_used > W.cand_cap && W.cand_cap < cl) {
+        const size_t n = (size_t)std::min<int64_t>((int64_t)c.cand_used, cl);
+        ENSURE(ctx, W.cands, n * sizeof(Cand));
+        ENSURE(ctx, W.spslot, n * sizeof(SpRec));
+        ENSURE(ctx, W.keys, n * sizeof(uint64_t));
+        set_slot_caps(W);
+    }
+    return PW_OK;
+}
+
 // Replaces the ops area by one sized for `used` bytes plus as much again
 // beyond the old capacity (what the failed allocations asked for), the old
 // area's contents copied over; W.ops_kept = the old capacity, where the
@@ -2884,7 +2909,13 @@ int device_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry
         const int next = c.next_ngated;
         if (next <= 0 || next >= prev) break;  // the round just walked finished nothing
         // the record area full: no later round can place records
-        if ((int64_t)c.out_used > W.cand_cap) break;
+        if ((int64_t)c.out_used > W.out_cap) break;
+        // the direction and candidate areas hold one round's work: grown (no
+        // contents to keep) toward what the round just walked asked for
+        if (ctx->lim.grow) {
+            const int rc = grow_round_areas(ctx, W, c);
+            if (rc) return rc;
+        }
         // the ops area full: grown (its records copied over) where growth is on
         long long ops_keep = -1;
         if ((int64_t)c.ops_used > W.ops_cap) {
