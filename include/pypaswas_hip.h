@@ -175,6 +175,14 @@ int pw_set_limits(pw_ctx *ctx, int64_t max_direction_bytes, int32_t candidates_p
  * or 8 (0: chosen per plan so that the pairs give enough work units). */
 int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_share, int32_t dir_residues);
 
+/* Where the records of executions afterwards are put into pair order (no
+ * counterpart in pyPaSWAS, which assembles its hits on the host; results never
+ * depend on it).  on_device != 0 (default): when every pair of an execution
+ * finished on the device, its records, op characters and alignment rows are laid
+ * out in pair order there and copied once into the result's pinned arrays;
+ * 0: they are read back per pair and assembled on the host. */
+int pw_set_assembly(pw_ctx *ctx, int32_t on_device);
+
 /* Bound of the device-side waits of the stripe and chunk hand-over of split
  * pairs and split lane-per-pair jobs (no counterpart in pyPaSWAS, whose
  * traceback spins on semaphores without a bound, default_main_cpu.cl:457-483):

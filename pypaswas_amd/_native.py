@@ -29,7 +29,8 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # exported entry points, in the order include/pypaswas_hip.h declares them
 EXPORTS = ("pw_init", "pw_destroy", "pw_device_mem_bytes", "pw_set_scoring", "pw_align_pairs", "pw_prepare",
-           "pw_execute", "pw_release_plan", "pw_plan_info", "pw_execute_graph", "pw_set_limits", "pw_set_routing", "pw_set_wait_limit", "pw_score_pairs",
+           "pw_execute", "pw_release_plan", "pw_plan_info", "pw_execute_graph", "pw_set_limits", "pw_set_routing", "pw_set_assembly",
+           "pw_set_wait_limit", "pw_score_pairs",
            "pw_score_pairs_noalloc", "pw_autotune", "pw_get_tuning", "pw_set_tuning", "pw_tune_plan", "pw_server_create", "pw_server_set_scoring", "pw_server_submit", "pw_server_poll", "pw_server_take",
            "pw_server_stats", "pw_server_destroy", "pw_free_result", "pw_last_error")
 
@@ -99,6 +100,7 @@ def load() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.POINTER(ResultRec))]
         L.pw_set_limits.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
         L.pw_set_routing.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]
+        L.pw_set_assembly.argtypes = [vp, ctypes.c_int32]
         L.pw_set_wait_limit.argtypes = [vp, ctypes.c_int64]
         L.pw_score_pairs.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, vp,
                                      ctypes.c_int64, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double)]
@@ -359,6 +361,10 @@ class Device:
         """Work-area sizing of the traceback stages (pw_set_limits); 0 keeps a setting."""
         self._check(self.lib.pw_set_limits(self.ctx, int(max_direction_bytes), int(candidates_per_pair),
                                            int(ops_bytes), 1 if grow else 0), "pw_set_limits")
+
+    def set_assembly(self, on_device: bool = True):
+        """Result layout in pair order on the device (default) or on the host (pw_set_assembly)."""
+        self._check(self.lib.pw_set_assembly(self.ctx, 1 if on_device else 0), "pw_set_assembly")
 
     def set_routing(self, orient: int = -1, group_max_share: float = 0.0, dir_residues: int = 0):
         """Routing rules of plans prepared afterwards (pw_set_routing); defaults restore them."""

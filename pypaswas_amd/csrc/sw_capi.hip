@@ -25,7 +25,9 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -90,6 +92,9 @@ struct Limits {
 struct Workspace {
     DevBuf ctl, maxima, gated, tasks, touts, dirs, cands, spslot, keys, out_sps, out_hits, ops, gscratch, tscratch,
         dscratch, units, dlinks, dprog, rq_slots, rq_needs;
+    // the gate's block counts; the result layout's block sums, per-slot
+    // offsets and the laid-out records, op characters and alignment rows
+    DevBuf gsums, lsums, lay_off, f_sps, f_hits, f_ops, f_aln;
     int64_t want_dir = 0, want_cand = 0, want_ops = 0;
     int64_t ops_kept = 0;  // ops bytes carried over when the area grew between device rounds
     int32_t cand0 = 0;
@@ -166,6 +171,52 @@ int tune_bucket(int v, const int *rep, int n) {
 
 // A result as the library owns it until pw_free_result; pub is its public
 // view (first member, so a pw_result pointer converts back).
+// Host array of a result that only grows (a recycled result fills it without
+// touching or zeroing new memory): pinned when the device copies into it,
+// else ordinary memory.  A pinned block replaced while frees are deferred goes
+// to *dead (hipHostFree waits for the whole device).
+template <class T>
+struct ResultArray {
+    T *p = nullptr;
+    size_t cap = 0;  // elements
+    bool pinned = false;
+    ResultArray() = default;
+    ResultArray(const ResultArray &) = delete;
+    ResultArray &operator=(const ResultArray &) = delete;
+    ~ResultArray() { drop(nullptr); }
+    void drop(std::vector<void *> *dead) {
+        if (p) {
+            if (!pinned) std::free(p);
+            else if (dead) dead->push_back(p);
+            else (void)hipHostFree(p);
+        }
+        p = nullptr;
+        cap = 0;
+        pinned = false;
+    }
+    // room for n elements (pinned if pin); false when no memory
+    bool reserve(size_t n, bool pin, std::vector<void *> *dead) {
+        if (n <= cap && (pinned || !pin)) return true;
+        drop(dead);
+        const size_t want = std::max<size_t>(n + n / 8, 16);
+        void *q = nullptr;
+        if (pin && hipHostMalloc(&q, want * sizeof(T), hipHostMallocDefault) == hipSuccess) {
+            pinned = true;
+        } else {
+            if (pin) (void)hipGetLastError();
+            q = std::malloc(want * sizeof(T));
+            if (!q) return false;
+        }
+        p = static_cast<T *>(q);
+        cap = want;
+        return true;
+    }
+    T *data() { return p; }
+    const T *data() const { return p; }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+};
+
 struct HostResult {
     pw_result pub;
     float *maxima = nullptr;  // pinned host memory of maxima_cap floats
@@ -173,10 +224,10 @@ struct HostResult {
     std::vector<float> maxima_vec;  // maxima of a served job (when maxima is NULL)
     // records; the vectors only grow (a recycled result fills them without
     // touching or zeroing new memory), n_sps and n_ops are the sizes in use
-    std::vector<pw_starting_point> sps;
-    std::vector<pw_hit> hits;
-    std::vector<char> ops;
-    std::vector<char> aligned;  // 3 chars per op: the three alignment rows of each shown hit
+    ResultArray<pw_starting_point> sps;
+    ResultArray<pw_hit> hits;
+    ResultArray<char> ops;
+    ResultArray<char> aligned;  // 3 chars per op: the three alignment rows of each shown hit
     int64_t n_sps = 0, n_ops = 0;
 };
 
@@ -207,6 +258,7 @@ struct pw_ctx {
     Tuning tune[2];  // linear, affine
     int route_orient = -1;        // pw_set_routing: orientation of lane-per-pair jobs (-1: the cheaper)
     double route_share = 0.25;    // pw_set_routing: longest lane-per-pair job (share of cells per lane)
+    bool layout_on_device = true;  // pw_set_assembly: lay results out on the device when every pair finished there
     int route_dir_rmax = 0;       // pw_set_routing: residues per lane of the direction pass (0: by plan)
     std::vector<HostResult *> result_pool;  // freed results kept for reuse
     // this device's copy of pw_static_scratch (hipGetSymbolAddress at pw_init),
@@ -296,6 +348,8 @@ struct pw_plan {
     // copy from pinned staging
     DevBuf res, seqs, tables, d_items, d_trans, d_gitems, d_gents, d_mss, d_pairs, d_same;
     DevBuf d_ckpairs;
+    DevBuf d_chars;  // 512 bytes: the residue character of each code, then its lower case
+    std::vector<uint8_t> chartab;
     DevBuf blob, d_links, d_prog, d_ck;
     PinBuf h_blob;
     Workspace own;
@@ -437,7 +491,8 @@ void free_pinned(PinBuf &b) {
 void free_workspace(pw_ctx *ctx, Workspace &W) {
     DevBuf *bufs[] = {&W.ctl, &W.maxima, &W.gated, &W.tasks, &W.touts, &W.dirs, &W.cands, &W.spslot, &W.keys,
                       &W.out_sps, &W.out_hits, &W.ops, &W.gscratch, &W.tscratch, &W.dscratch, &W.units, &W.dlinks,
-                      &W.dprog, &W.rq_slots, &W.rq_needs};
+                      &W.dprog, &W.rq_slots, &W.rq_needs, &W.gsums, &W.lsums, &W.lay_off, &W.f_sps, &W.f_hits,
+                      &W.f_ops, &W.f_aln};
     for (DevBuf *b : bufs) free_buf(ctx, *b);
     PinBuf *pins[] = {&W.h_ctl, &W.h_tasks, &W.h_touts, &W.h_sps, &W.h_hits, &W.h_ops};
     for (PinBuf *b : pins) free_pinned(*b);
@@ -670,6 +725,12 @@ extern "C" int pw_set_routing(pw_ctx *ctx, int32_t orient, double group_max_shar
     ctx->route_orient = orient;
     ctx->route_share = group_max_share > 0 ? group_max_share : 0.25;
     ctx->route_dir_rmax = dir_residues;
+    return PW_OK;
+}
+
+extern "C" int pw_set_assembly(pw_ctx *ctx, int32_t on_device) {
+    if (!ctx) return PW_EINVAL;
+    ctx->layout_on_device = on_device != 0;
     return PW_OK;
 }
 
@@ -1803,6 +1864,14 @@ int plan_worklist(const PlanOpts &opt, pw_plan *P) {
 int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync) {
     const Encoded &E = P->E;
     const int64_t n = P->n_pairs;
+    // residue characters of the alignment rows, by code ('?' beyond the
+    // alphabet), and their lower-case forms
+    P->chartab.assign(512, 0);
+    for (int c = 0; c < 256; ++c) {
+        const uint8_t ch = c < (int)E.chars.size() ? E.chars[c] : (uint8_t)'?';
+        P->chartab[c] = ch;
+        P->chartab[256 + c] = (uint8_t)std::tolower(ch);
+    }
     struct Part {
         DevBuf *d;
         const void *h;  // nullptr: the pair list, packed from pq and pt
@@ -1819,7 +1888,8 @@ int plan_upload(pw_ctx *ctx, pw_plan *P, hipStream_t s, PinBuf &stage, bool sync
                     {&P->d_mss, P->mss.data(), (size_t)n * sizeof(float), 0},
                     {&P->d_pairs, nullptr, P->cross ? 0 : (size_t)n * sizeof(int2), 0},
                     {&P->d_same, P->same.data(), (size_t)n, 0},
-                    {&P->d_ckpairs, P->ckpairs.data(), P->ckpairs.size() * sizeof(CkPair), 0}};
+                    {&P->d_ckpairs, P->ckpairs.data(), P->ckpairs.size() * sizeof(CkPair), 0},
+                    {&P->d_chars, P->chartab.data(), P->chartab.size(), 0}};
     size_t total = 0;
     for (Part &q : parts) {
         q.off = total;
@@ -1973,6 +2043,7 @@ int size_workspace(pw_ctx *ctx, pw_plan *P, Workspace &W, const Geometry &g) {
     ENSURE(ctx, W.gated, (size_t)n * sizeof(int));
     ENSURE(ctx, W.rq_slots, (size_t)n * sizeof(int));
     ENSURE(ctx, W.rq_needs, (size_t)n * sizeof(int));
+    ENSURE(ctx, W.gsums, (size_t)std::max(1, g.gate_grid) * sizeof(int));
     if (W.dir_ext) {
         W.dir_cap = W.dir_ext_cap;
     } else {
@@ -2256,8 +2327,12 @@ int launch_pipeline(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geomet
                                    hipMemcpyDeviceToHost, L.s));
     }
     if (P->n_pairs > 0) {
-        hipLaunchKernelGGL(gate_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
-                           (const float *)P->d_mss.p, P->n_pairs, P->sc.minimum, ctl, (int *)W.gated.p);
+        hipLaunchKernelGGL(gate_count_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
+                           (const float *)P->d_mss.p, P->n_pairs, P->sc.minimum, ctl, (int *)W.gsums.p);
+        HIPCHK(ctx, hipGetLastError());
+        hipLaunchKernelGGL(gate_scatter_kernel, dim3(g.gate_grid), dim3(256), 0, L.s, (const float *)W.maxima.p,
+                           (const float *)P->d_mss.p, P->n_pairs, P->sc.minimum, (const int *)W.gsums.p,
+                           (int *)W.gated.p);
         HIPCHK(ctx, hipGetLastError());
         int rc2 = launch_tasks(ctx, P, W, L, g, false);
         if (rc2) return rc2;
@@ -2609,17 +2684,29 @@ void sort_by_pair(std::vector<PairOut> &outs) {
     outs.swap(out);
 }
 
-void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const pw_plan *P, HostResult *R) {
+// Reports how many starting points the reference's per-pair cap would drop:
+// it keeps at most sp_cap starting points per pair of a batch in total
+// (SmithWaterman.py:683-685).
+int64_t sp_over_cap(int64_t nsp, int32_t sp_cap, int64_t n_pairs) {
+    if (sp_cap <= 0) return 0;
+    const int64_t cap_total = (int64_t)sp_cap * n_pairs;
+    return nsp >= cap_total ? nsp - cap_total : 0;
+}
+
+int assemble(pw_ctx *ctx, std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const pw_plan *P,
+             HostResult *R) {
     sort_by_pair(outs);
     int64_t nsp = 0, nops = 0;
     for (const PairOut &o : outs) {
         nsp += o.nsp;
         nops += o.nops;
     }
-    if ((int64_t)R->sps.size() < nsp) R->sps.resize((size_t)nsp);
-    if ((int64_t)R->hits.size() < nsp) R->hits.resize((size_t)nsp);
-    if ((int64_t)R->ops.size() < nops) R->ops.resize((size_t)nops);
-    if ((int64_t)R->aligned.size() < 3 * nops) R->aligned.resize((size_t)(3 * nops));
+    std::vector<void *> *dead = ctx->defer_free ? &ctx->dead_pinned : nullptr;
+    if (!R->sps.reserve((size_t)nsp, false, dead) || !R->hits.reserve((size_t)nsp, false, dead) ||
+        !R->ops.reserve((size_t)nops, false, dead) || !R->aligned.reserve((size_t)(3 * nops), false, dead)) {
+        ctx->err = "no host memory for the result records";
+        return PW_ENOMEM;
+    }
     R->n_sps = nsp;
     R->n_ops = nops;
     static_assert(sizeof(SpRec) == sizeof(pw_starting_point), "starting point layout");
@@ -2711,14 +2798,8 @@ void assemble(std::vector<PairOut> &outs, int32_t sp_cap, int64_t n_pairs, const
             if (e > b) fill(b, e, sp_at[b], op_at[b]);
         });
     }
-    // the reference keeps at most sp_cap starting points per pair of a batch in
-    // total (SmithWaterman.py:683-685); report how many a cap would drop
-    int64_t truncated = 0;
-    if (sp_cap > 0) {
-        const int64_t cap_total = (int64_t)sp_cap * n_pairs;
-        if (nsp >= cap_total) truncated = nsp - cap_total;
-    }
-    R->pub.sp_truncated = truncated;
+    R->pub.sp_truncated = sp_over_cap(nsp, sp_cap, n_pairs);
+    return PW_OK;
 }
 
 // Launch parameters a captured graph depends on.
@@ -2730,7 +2811,7 @@ std::vector<uint64_t> graph_signature(const pw_plan *P, const Workspace &W, cons
     const DevBuf *bufs[] = {&W.ctl,      &W.maxima,   &W.gated,    &W.tasks,    &W.touts,    &W.dirs,
                             &W.cands,    &W.spslot,   &W.keys,     &W.out_sps,  &W.out_hits, &W.ops,
                             &W.gscratch, &W.tscratch, &W.dscratch, &W.units,    &W.dlinks,   &W.dprog,
-                            &P->d_links, &P->d_prog,   &P->d_ck,    &P->d_ckpairs};
+                            &P->d_links, &P->d_prog,   &P->d_ck,    &P->d_ckpairs, &W.gsums};
     for (const DevBuf *b : bufs) {
         add((uint64_t)(uintptr_t)b->p);
         add(b->cap);
@@ -2776,6 +2857,7 @@ struct Finished {
     double ms_score = 0, ms_dir = 0, ms_tb = 0, ms_walk = 0;
     double ms_readback = 0, ms_tail = 0, ms_assemble = 0;  // host phases
     double ms_device_rounds = 0;                           // further rounds on the device (wall)
+    bool on_device = false;  // every pair finished on the device: the result is laid out there (outs empty)
 };
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -2893,7 +2975,7 @@ int grow_ops(pw_ctx *ctx, Workspace &W, Lane &L, int64_t used) {
     return PW_OK;
 }
 int device_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, Finished &F,
-                  int &n_last) {
+                  int &n_last, bool &done) {
     Ctl *dctl = (Ctl *)W.ctl.p;
     Ctl c = *(const Ctl *)W.h_ctl.p;
     int prev = c.ngated;
@@ -2944,10 +3026,14 @@ int device_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry
         prev = next;
         n_last = next;
     }
+    done = c.unfinished == 0;
     return PW_OK;
 }
 
-int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X, Finished &F) {
+// layout_on_device: when every pair finishes on the device, leave the records
+// there for device_assemble (F.on_device) instead of reading them back.
+int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X, Finished &F,
+                   bool layout_on_device = false) {
     HIPCHK(ctx, hipStreamSynchronize(L.s));
     if (X.graph) {
         F.ms_score = elapsed(L.ev[0], L.ev[5]);
@@ -2972,12 +3058,19 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
     // rounds on the device, the records collecting in the read-back areas;
     // what a round cannot finish goes to the host's tail rounds
     int n_last = -1;
+    bool done = c.unfinished == 0;
     auto t0 = std::chrono::steady_clock::now();
-    if (c.unfinished > 0) {
-        int rc = device_rounds(ctx, P, W, L, X.g, X.minimum, F, n_last);
+    if (!done) {
+        int rc = device_rounds(ctx, P, W, L, X.g, X.minimum, F, n_last, done);
         if (rc) return rc;
     }
     F.ms_device_rounds = ms_since(t0);
+    if (done && layout_on_device) {
+        if (c.unfinished > 0) adapt_workspace(ctx, W, c, *(const Ctl *)W.h_ctl.p, {});
+        F.ms_tail = F.ms_device_rounds;
+        F.on_device = true;
+        return PW_OK;
+    }
     std::vector<PairTask> pending;
     t0 = std::chrono::steady_clock::now();
     int rc = read_tasks(ctx, W, L, F.ng, F.outs, pending, n_last);
@@ -2996,6 +3089,83 @@ int execute_finish(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Exec &X
     t0 = std::chrono::steady_clock::now();
     sort_by_pair(F.outs);
     F.ms_assemble = ms_since(t0);
+    return PW_OK;
+}
+
+static_assert(sizeof(HitRec) == sizeof(pw_hit) && offsetof(HitRec, score) == offsetof(pw_hit, score) &&
+                  offsetof(HitRec, sp_index) == offsetof(pw_hit, sp_index) &&
+                  offsetof(HitRec, ops_offset) == offsetof(pw_hit, ops_offset),
+              "the device writes the public hit records");
+
+// The result of an execution whose pairs all finished on the device: the
+// records, op characters and alignment rows laid out in pair order there, then
+// copied into the result's pinned arrays.
+int device_assemble(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, int ng, int32_t sp_cap, HostResult *R,
+                    Finished &F) {
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t nsp = 0, nops = 0;
+    Ctl *dctl = (Ctl *)W.ctl.p;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)ng + 255) / 256, (int64_t)ctx->num_cus * 4));
+    if (ng > 0) {
+        ENSURE(ctx, W.lsums, (size_t)grid * 2 * sizeof(long long));
+        ENSURE(ctx, W.lay_off, (size_t)ng * 2 * sizeof(long long));
+        hipLaunchKernelGGL(layout_sums_kernel, dim3(grid), dim3(256), 0, L.s, (const TaskOut *)W.touts.p, ng, dctl,
+                           (long long *)W.lsums.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(W.h_ctl.p, dctl, sizeof(Ctl), hipMemcpyDeviceToHost, L.s));
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        const Ctl &c = *(const Ctl *)W.h_ctl.p;
+        nsp = c.lay_sps;
+        nops = c.lay_ops;
+        if (nsp < 0 || nops < 0 || nsp > (int64_t)INT32_MAX) {
+            ctx->err = "result sizes out of range";
+            return PW_EDEVICE;
+        }
+    }
+    std::vector<void *> *dead = ctx->defer_free ? &ctx->dead_pinned : nullptr;
+    if (!R->sps.reserve((size_t)nsp, true, dead) || !R->hits.reserve((size_t)nsp, true, dead) ||
+        !R->ops.reserve((size_t)nops, true, dead) || !R->aligned.reserve((size_t)(3 * nops), true, dead)) {
+        ctx->err = "no host memory for the result records";
+        return PW_ENOMEM;
+    }
+    if (nsp > 0) {
+        if (!P->d_chars.p || P->d_chars.cap < 512) {
+            ctx->err = "the plan's residue characters are not on the device";
+            return PW_EDEVICE;
+        }
+        ENSURE(ctx, W.f_sps, (size_t)nsp * sizeof(SpRec));
+        ENSURE(ctx, W.f_hits, (size_t)nsp * sizeof(HitRec));
+        ENSURE(ctx, W.f_ops, (size_t)std::max<int64_t>(1, nops));
+        ENSURE(ctx, W.f_aln, (size_t)std::max<int64_t>(1, 3 * nops));
+        long long *off = (long long *)W.lay_off.p;
+        hipLaunchKernelGGL(layout_offsets_kernel, dim3(grid), dim3(256), 0, L.s, (const TaskOut *)W.touts.p, ng,
+                           (const long long *)W.lsums.p, off, off + ng);
+        HIPCHK(ctx, hipGetLastError());
+        const int ggrid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)ng + 3) / 4, (int64_t)ctx->num_cus * 8));
+        hipLaunchKernelGGL(layout_gather_kernel, dim3(ggrid), dim3(256), 0, L.s, (const uint8_t *)P->res.p,
+                           (const SeqRef *)P->seqs.p, P->nq, (const TaskOut *)W.touts.p, ng, (const long long *)off,
+                           (const long long *)(off + ng), (const SpRec *)W.out_sps.p, (const HitRec *)W.out_hits.p,
+                           (const char *)W.ops.p, (const uint8_t *)P->d_chars.p, (SpRec *)W.f_sps.p,
+                           (HitRec *)W.f_hits.p, (char *)W.f_ops.p, (char *)W.f_aln.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        F.ms_assemble = ms_since(t0);
+        t0 = std::chrono::steady_clock::now();
+        HIPCHK(ctx, hipMemcpyAsync(R->sps.data(), W.f_sps.p, (size_t)nsp * sizeof(SpRec), hipMemcpyDeviceToHost, L.s));
+        HIPCHK(ctx, hipMemcpyAsync(R->hits.data(), W.f_hits.p, (size_t)nsp * sizeof(HitRec), hipMemcpyDeviceToHost,
+                                   L.s));
+        if (nops > 0) {
+            HIPCHK(ctx, hipMemcpyAsync(R->ops.data(), W.f_ops.p, (size_t)nops, hipMemcpyDeviceToHost, L.s));
+            HIPCHK(ctx, hipMemcpyAsync(R->aligned.data(), W.f_aln.p, (size_t)(3 * nops), hipMemcpyDeviceToHost, L.s));
+        }
+        HIPCHK(ctx, hipStreamSynchronize(L.s));
+        F.ms_readback = ms_since(t0);
+    } else {
+        F.ms_assemble = ms_since(t0);
+    }
+    R->n_sps = nsp;
+    R->n_ops = nops;
+    R->pub.sp_truncated = sp_over_cap(nsp, sp_cap, P->n_pairs);
     return PW_OK;
 }
 
@@ -3034,11 +3204,17 @@ int execute(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, double minimum, int3
     rc = execute_launch(ctx, P, W, L, minimum, use_graph, R->maxima, X);
     if (rc) return rc;
     Finished F;
-    rc = execute_finish(ctx, P, W, L, X, F);
+    rc = execute_finish(ctx, P, W, L, X, F, ctx->layout_on_device);
     if (rc) return rc;
-    const auto t0 = std::chrono::steady_clock::now();
-    assemble(F.outs, sp_cap, P->n_pairs, P, R);
-    F.ms_assemble += ms_since(t0);
+    if (F.on_device) {
+        rc = device_assemble(ctx, P, W, L, F.ng, sp_cap, R, F);
+        if (rc) return rc;
+    } else {
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = assemble(ctx, F.outs, sp_cap, P->n_pairs, P, R);
+        if (rc) return rc;
+        F.ms_assemble += ms_since(t0);
+    }
     fill_public(R, R->maxima, P->n_pairs, P->cells, F.ng, F, X);
     return PW_OK;
 }

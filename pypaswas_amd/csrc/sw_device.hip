@@ -1004,42 +1004,241 @@ __device__ __host__ __forceinline__ int split_stripes(int alen, int blen, int R)
     return (ns >= 2 && blen >= SPLIT_MIN_COLS) ? ns : 0;
 }
 
-// Traceback gate of default_main_cpu.cl:408-410 (maximum >= MINIMUM_SCORE)
-// plus the pair's minimum start score, without which no cell can start
-// (:427): the indices of the passing pairs are compacted into `gated`.
-// Passing pairs are sparse, so each workgroup counts its contiguous range
-// first, takes its output range with one atomic, then writes (one LDS atomic
-// per wave for the slots inside the range).
-__global__ void __launch_bounds__(256) gate_kernel(const float *__restrict__ maxima, const float *__restrict__ mss,
-                                                   int64_t n, float minimum, Ctl *__restrict__ ctl,
-                                                   int *__restrict__ gated) {
-    __shared__ int count, base;
-    const int lane = threadIdx.x & 63;
-    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
-    if (threadIdx.x == 0) count = 0;
-    __syncthreads();
-    for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
-        const int64_t i = i0 + threadIdx.x;
-        const bool pass = i < hi && maxima[i] >= minimum && maxima[i] >= mss[i];
-        const uint64_t mask = __ballot(pass);
-        if (lane == 0 && mask) atomicAdd(&count, __popcll(mask));
+// Exclusive prefix sum over a 256-thread block of v, and the block total (lds:
+// four entries; every thread of the block calls it).
+template <class T>
+__device__ __forceinline__ T block_excl_scan(T v, T &total, T *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    T before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const T t = lds[w];
+        if (w < wave) before += t;
+        total += t;
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+// Called by every thread of each block of a grid whose blocks stored their
+// sums (device-scope atomic stores) before: true in the block that finished
+// last, which then sees every block's sum.
+__device__ __forceinline__ bool last_block(unsigned *done) {
+    __shared__ bool last;
     __syncthreads();
     if (threadIdx.x == 0) {
-        base = count ? atomicAdd(&ctl->ngated, count) : 0;
-        count = 0;
+        __threadfence();
+        last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
     __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// Exclusive prefix sums, in place, of the n sums at a[0], a[stride], ... (one
+// 256-thread block, device-scope atomic accesses); returns the total.
+template <class T>
+__device__ T scan_sums(T *a, int n, int stride, T *lds) {
+    T run = 0;
+    for (int b0 = 0; b0 < n; b0 += 256) {
+        const int b = b0 + threadIdx.x;
+        const T v = b < n ? __hip_atomic_load(&a[(int64_t)b * stride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        T tot;
+        const T ex = block_excl_scan<T>(v, tot, lds);
+        if (b < n) __hip_atomic_store(&a[(int64_t)b * stride], run + ex, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        run += tot;
+    }
+    return run;
+}
+
+// Traceback gate of default_main_cpu.cl:408-410 (maximum >= MINIMUM_SCORE)
+// plus the pair's minimum start score, without which no cell can start
+// (:427): the indices of the passing pairs are compacted into `gated` in
+// ascending order, so that the read-back slots of an execution are in pair
+// order.  Block b of both passes takes the contiguous range [b*per, (b+1)*per).
+__device__ __forceinline__ bool gate_pass(const float *__restrict__ maxima, const float *__restrict__ mss, int64_t i,
+                                          float minimum) {
+    const float m = maxima[i];
+    return m >= minimum && m >= mss[i];
+}
+
+// Pass 1: each block counts its passing pairs into bsum[b]; the block that
+// finishes last turns the counts into exclusive offsets and sets ctl->ngated.
+__global__ void __launch_bounds__(256) gate_count_kernel(const float *__restrict__ maxima,
+                                                         const float *__restrict__ mss, int64_t n, float minimum,
+                                                         Ctl *__restrict__ ctl, int *__restrict__ bsum) {
+    __shared__ int lds[4];
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+    int cnt = 0;
     for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
         const int64_t i = i0 + threadIdx.x;
-        const bool pass = i < hi && maxima[i] >= minimum && maxima[i] >= mss[i];
-        const uint64_t mask = __ballot(pass);
-        if (!mask) continue;
-        int first = 0;
-        if (lane == 0) first = atomicAdd(&count, __popcll(mask));
-        first = __shfl(first, 0, 64);
-        if (pass) gated[base + first + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+        cnt += (i < hi && gate_pass(maxima, mss, i, minimum)) ? 1 : 0;
+    }
+    int tot;
+    (void)block_excl_scan<int>(cnt, tot, lds);
+    if (threadIdx.x == 0) __hip_atomic_store(&bsum[blockIdx.x], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!last_block(&ctl->gate_done)) return;
+    const int all = scan_sums<int>(bsum, gridDim.x, 1, lds);
+    if (threadIdx.x == 0) __hip_atomic_store(&ctl->ngated, all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pass 2: each block writes its passing pairs in order from its offset.
+__global__ void __launch_bounds__(256) gate_scatter_kernel(const float *__restrict__ maxima,
+                                                           const float *__restrict__ mss, int64_t n, float minimum,
+                                                           const int *__restrict__ boff, int *__restrict__ gated) {
+    __shared__ int lds[4];
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+    int run = __hip_atomic_load(&boff[blockIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool pass = i < hi && gate_pass(maxima, mss, i, minimum);
+        int tot;
+        const int k = block_excl_scan<int>(pass ? 1 : 0, tot, lds);
+        if (pass) gated[run + k] = (int)i;
+        run += tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Result layout on the device: every pair's records, op characters and the
+// three alignment rows of its shown hits, in pair order (the read-back slots
+// are in pair order), at the offsets of exclusive sums over the slots.
+// ---------------------------------------------------------------------------
+
+// Pass 1: per-block sums of the slots' record and op counts (bsum[2b],
+// bsum[2b + 1]); the block finishing last scans them and stores the totals in
+// ctl->lay_sps / ctl->lay_ops.
+__global__ void __launch_bounds__(256) layout_sums_kernel(const TaskOut *__restrict__ touts, int ng,
+                                                          Ctl *__restrict__ ctl, long long *__restrict__ bsum) {
+    __shared__ long long lds[4];
+    const int per = (ng + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * per, hi = min(ng, lo + per);
+    long long ns = 0, no = 0;
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        ns += touts[i].nsp;
+        no += touts[i].nops;
+    }
+    long long ts, to;
+    (void)block_excl_scan<long long>(ns, ts, lds);
+    (void)block_excl_scan<long long>(no, to, lds);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&bsum[2 * blockIdx.x], ts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bsum[2 * blockIdx.x + 1], to, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!last_block(&ctl->lay_done)) return;
+    const long long as = scan_sums<long long>(bsum, gridDim.x, 2, lds);
+    const long long ao = scan_sums<long long>(bsum + 1, gridDim.x, 2, lds);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&ctl->lay_sps, as, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->lay_ops, ao, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Pass 2: each slot's first record and first op character in the result.
+__global__ void __launch_bounds__(256) layout_offsets_kernel(const TaskOut *__restrict__ touts, int ng,
+                                                             const long long *__restrict__ bsum,
+                                                             long long *__restrict__ sp_off,
+                                                             long long *__restrict__ op_off) {
+    __shared__ long long lds[4];
+    const int per = (ng + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * per, hi = min(ng, lo + per);
+    long long rs = __hip_atomic_load(&bsum[2 * blockIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long long ro = __hip_atomic_load(&bsum[2 * blockIdx.x + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i0 = lo; i0 < hi; i0 += blockDim.x) {
+        const int i = i0 + threadIdx.x;
+        const bool on = i < hi;
+        long long ts, to;
+        const long long es = block_excl_scan<long long>(on ? (long long)touts[i].nsp : 0ll, ts, lds);
+        const long long eo = block_excl_scan<long long>(on ? (long long)touts[i].nops : 0ll, to, lds);
+        if (on) {
+            sp_off[i] = rs + es;
+            op_off[i] = ro + eo;
+        }
+        rs += ts;
+        ro += to;
+    }
+}
+
+// Pass 3, one wave per slot: starting points and op characters copied, hits
+// with their indices moved to the result's, and each shown hit's query,
+// alignment and target rows (SmithWaterman.py:777-830) written start to end
+// at three times its op offset.  The ops of a hit run from its end back; a
+// column's query (target) position is the hit's start plus the columns
+// before it that consume a query (target) residue: a ballot prefix count
+// per 64 columns.  ctab: the residue character of each code, then its lower
+// case (the match test compares lower-cased, :794-799).
+__global__ void __launch_bounds__(256) layout_gather_kernel(
+    const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs, int nq, const TaskOut *__restrict__ touts,
+    int ng, const long long *__restrict__ sp_off, const long long *__restrict__ op_off,
+    const SpRec *__restrict__ out_sps, const HitRec *__restrict__ out_hits, const char *__restrict__ ops,
+    const uint8_t *__restrict__ ctab, SpRec *__restrict__ dsp, HitRec *__restrict__ dhit, char *__restrict__ dops,
+    char *__restrict__ daln) {
+    __shared__ uint8_t tab[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) tab[i] = ctab[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < ng; s += waves) {
+        const TaskOut t = touts[s];
+        const long long sp0 = sp_off[s], op0 = op_off[s];
+        const uint32_t *ssrc = reinterpret_cast<const uint32_t *>(out_sps + t.out_off);
+        uint32_t *sdst = reinterpret_cast<uint32_t *>(dsp + sp0);
+        constexpr int SPW = (int)(sizeof(SpRec) / 4);
+        for (int w = lane; w < t.nsp * SPW; w += WAVE) sdst[w] = ssrc[w];
+        for (int k = lane; k < t.nsp; k += WAVE) {
+            HitRec h = out_hits[t.out_off + k];
+            h.sp_index = (int32_t)(sp0 + h.sp_index);
+            h.ops_offset = op0 + h.ops_offset;
+            dhit[sp0 + k] = h;
+        }
+        const char *osrc = ops + t.ops_off;
+        char *odst = dops + op0;
+        for (int64_t i = lane; i < t.nops; i += WAVE) odst[i] = osrc[i];
+        for (int k = 0; k < t.nsp; ++k) {
+            const HitRec h = out_hits[t.out_off + k];
+            if (!h.shown || h.length <= 0) continue;
+            const uint8_t *qc = res + seqs[h.query].off, *tc = res + seqs[nq + h.target].off;
+            const char *op = osrc + h.ops_offset + h.length - 1;
+            char *qrow = daln + 3 * (op0 + h.ops_offset), *arow = qrow + h.length, *trow = arow + h.length;
+            int x = h.s_start, y = h.t_start;
+            for (int c0 = 0; c0 < h.length; c0 += WAVE) {
+                const int c = c0 + lane;
+                const bool on = c < h.length;
+                const char v = on ? op[-c] : 'D';
+                const uint64_t bq = __ballot(on && v != 'U'), bt = __ballot(on && v != 'L');
+                const int xi = x + __popcll(bq & below), yi = y + __popcll(bt & below);
+                if (on) {
+                    uint8_t qch = '-', tch = '-', ach = ' ';
+                    if (v == 'D') {
+                        const uint8_t a = qc[xi], b = tc[yi];
+                        qch = tab[a];
+                        tch = tab[b];
+                        ach = tab[256 + a] == tab[256 + b] ? '|' : '.';
+                    } else if (v == 'L') {
+                        qch = tab[qc[xi]];
+                    } else {
+                        tch = tab[tc[yi]];
+                    }
+                    qrow[c] = (char)qch;
+                    arow[c] = (char)ach;
+                    trow[c] = (char)tch;
+                }
+                x += __popcll(bq);
+                y += __popcll(bt);
+            }
+        }
     }
 }
 

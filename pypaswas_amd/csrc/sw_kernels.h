@@ -164,6 +164,9 @@ struct Ctl {
     unsigned abort_wait;  // set by a wave whose stripe or chunk hand-over wait gave up (see WAIT_POLLS)
     unsigned unfinished;  // tasks of the round left unfinished (no room, or a candidate or record overflow)
     int next_ngated;      // tasks requeued for the next round (requeue_kernel)
+    unsigned gate_done;   // blocks of gate_count_kernel finished
+    unsigned lay_done;    // blocks of layout_sums_kernel finished
+    long long lay_sps, lay_ops;  // records and op characters of the execution (layout_sums_kernel)
 };
 
 // Polls a wave makes on a hand-over counter before it gives up, sets

@@ -215,7 +215,7 @@ int launch_batch(pw_server *S, ServeLane &ln, std::vector<ServeJob *> &batch, in
 }
 
 // One job's records out of its batch's (pair-sorted) records.
-void split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j, size_t &cursor) {
+int split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j, size_t &cursor) {
     HostResult *R = nullptr;
     new_result(ctx, &R);
     const int64_t p1 = j->p0 + (int64_t)j->pq.size();
@@ -225,7 +225,11 @@ void split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j,
         ++cursor;
     }
     for (PairOut &o : mine) o.pair -= (int32_t)j->p0;
-    assemble(mine, j->sp_cap, (int64_t)j->pq.size(), ln.P, R);
+    const int rc = assemble(ctx, mine, j->sp_cap, (int64_t)j->pq.size(), ln.P, R);
+    if (rc) {
+        (void)pw_free_result(ctx, &R->pub);
+        return rc;
+    }
     for (int64_t k = 0; k < R->n_sps; ++k) {
         pw_starting_point &sp = R->sps[(size_t)k];
         sp.sequence -= (uint32_t)j->q0;
@@ -246,6 +250,7 @@ void split_job(pw_ctx *ctx, const Finished &F, const ServeLane &ln, ServeJob *j,
     fill_public(R, R->maxima_vec.data(), (int64_t)j->pq.size(), j->cells, (int64_t)mine.size(), stats, ln.X);
     j->result = R;
     j->done = true;
+    return PW_OK;
 }
 
 int finish_batch(pw_server *S, ServeLane &ln) {
@@ -257,7 +262,8 @@ int finish_batch(pw_server *S, ServeLane &ln) {
     S->us_device += 1e3 * elapsed(ln.L.ev[0], ln.L.ev[5]);
     size_t cursor = 0;
     for (ServeJob *j : ln.jobs) {
-        split_job(ctx, F, ln, j, cursor);
+        rc = split_job(ctx, F, ln, j, cursor);
+        if (rc) return rc;
         S->completed.push_back(j->id);
         S->n_completed += 1;
     }
