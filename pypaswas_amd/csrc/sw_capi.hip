@@ -1005,8 +1005,10 @@ int group_nb(int clen, const Tuning *tu) {
 // two transposed substitution tables, the swept-residue window, the staging
 // of the stripe boundary rows (up to 64 columns in, 64 out) and, for the direction
 // pass, the candidate buffer
+// LDS of the wave-per-pair kernels: the scoring pass holds both orientations
+// of the substitution table, the direction pass (query along the lanes) one.
 size_t lds_bytes(int KP, bool dirs = false) {
-    return (size_t)2 * KP * KP * sizeof(float) + BWIN + WAVE + 2 * WAVE * sizeof(float4) +
+    return (size_t)(dirs ? 1 : 2) * KP * KP * sizeof(float) + BWIN + WAVE + 2 * WAVE * sizeof(float4) +
            (dirs ? CAND_BUF * sizeof(Cand) : 0);
 }
 
