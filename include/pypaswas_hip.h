@@ -66,9 +66,9 @@ typedef struct {
     double ms_walk;                      /* walk */
     double ms_total;                     /* wall time of the call, host clock */
     int64_t score_launches;
-    double ms_readback;                  /* host: reading tasks and records back (incl. copies) */
+    double ms_readback;                  /* copy of the records to the host (or, host assembly: reading tasks and records back) */
     double ms_tail;                      /* further rounds of pairs that did not fit the work areas (on the device, then on the host) */
-    double ms_assemble;                  /* host: records into pair order */
+    double ms_assemble;                  /* records into pair order (the device layout kernels, or the host assembly) */
     double ms_plan;                      /* host: encoding and work list of a one-shot call (pw_align_pairs) */
     double ms_upload;                    /* host: staging and copy of the plan to the device (one-shot call) */
     const char *aligned;                 /* [3 * n_ops] for each shown hit at 3 * ops_offset: its query row,
