@@ -96,7 +96,6 @@ struct Workspace {
     // offsets and the laid-out records, op characters and alignment rows
     DevBuf gsums, lsums, lay_off, f_sps, f_hits, f_ops, f_aln;
     int64_t want_dir = 0, want_cand = 0, want_ops = 0;
-    int64_t ops_kept = 0;  // ops bytes carried over when the area grew between device rounds
     int32_t cand0 = 0;
     int64_t dir_cap = 0, cand_cap = 0, ops_cap = 0;  // usable sizes of the current buffers
     int64_t out_cap = 0;                             // records the read-back areas (out_sps, out_hits) hold
@@ -2937,45 +2936,75 @@ constexpr int MAX_DEVICE_ROUNDS = 64;
 // Direction and candidate areas for the next device round: what the last
 // round asked for (its allocators count every request, met or not), within
 // the limits.
-int grow_round_areas(pw_ctx *ctx, Workspace &W, const Ctl &c) {
+// (the capacities always describe the buffers, also after a failed growth)
+int grow_round_areas(pw_ctx *ctx, Workspace &W, const Ctl &c, bool &grew) {
     if (!W.dir_ext && (int64_t)c.dir_used > W.dir_cap && W.dir_cap < dir_limit(ctx)) {
-        ENSURE(ctx, W.dirs, (size_t)std::min<int64_t>((int64_t)c.dir_used, dir_limit(ctx)));
+        const int64_t before = W.dir_cap;
+        const int rc = ensure(ctx, W.dirs, (size_t)std::min<int64_t>((int64_t)c.dir_used, dir_limit(ctx)));
         W.dir_cap = (int64_t)W.dirs.cap;
+        if (rc) return rc;
+        grew = grew || W.dir_cap > before;
     }
     const int64_t cl = cand_slot_limit(ctx);
     if ((int64_t)c.cand_used > W.cand_cap && W.cand_cap < cl) {
+        const int64_t before = W.cand_cap;
         const size_t n = (size_t)std::min<int64_t>((int64_t)c.cand_used, cl);
-        ENSURE(ctx, W.cands, n * sizeof(Cand));
-        ENSURE(ctx, W.spslot, n * sizeof(SpRec));
-        ENSURE(ctx, W.keys, n * sizeof(uint64_t));
+        int rc = ensure(ctx, W.cands, n * sizeof(Cand));
+        if (!rc) rc = ensure(ctx, W.spslot, n * sizeof(SpRec));
+        if (!rc) rc = ensure(ctx, W.keys, n * sizeof(uint64_t));
         set_slot_caps(W);
+        if (rc) return rc;
+        grew = grew || W.cand_cap > before;
     }
     return PW_OK;
 }
 
-// Replaces the ops area by one sized for `used` bytes plus as much again
-// beyond the old capacity (what the failed allocations asked for), the old
-// area's contents copied over; W.ops_kept = the old capacity, where the
-// allocator restarts.
-int grow_ops(pw_ctx *ctx, Workspace &W, Lane &L, int64_t used) {
-    const int64_t keep = W.ops_cap;
-    const int64_t need = std::min<int64_t>(used + (used - keep), std::max<int64_t>(used, dir_limit(ctx)));
-    DevBuf old = W.ops;
-    W.ops = DevBuf{};
-    const int rc = ensure(ctx, W.ops, (size_t)need);
+// Replaces b by a buffer of at least need bytes whose first keep bytes are
+// b's (a device-to-device copy); the old buffer goes back to the cache.
+int regrow(pw_ctx *ctx, Lane &L, DevBuf &b, size_t keep, size_t need) {
+    DevBuf old = b;
+    b = DevBuf{};
+    const int rc = ensure(ctx, b, need);
     if (rc) {
-        W.ops = old;
+        b = old;
         return rc;
     }
-    HIPCHK(ctx, hipMemcpyAsync(W.ops.p, old.p, (size_t)keep, hipMemcpyDeviceToDevice, L.s));
+    if (keep > 0) HIPCHK(ctx, hipMemcpyAsync(b.p, old.p, keep, hipMemcpyDeviceToDevice, L.s));
     HIPCHK(ctx, hipStreamSynchronize(L.s));
     if (ctx->defer_free) ctx->dead_dev.emplace_back(old.p, old.cap);
     else dev_release(ctx, old.p, old.cap);
-    W.ops_cap = (int64_t)W.ops.cap;
-    W.ops_kept = keep;
-    W.want_ops = std::max<int64_t>(W.want_ops, need);
     return PW_OK;
 }
+
+// The record areas filled up during a round: every allocation that succeeded
+// ends at or below their capacity, the allocator having counted the failed
+// ones beyond it.  Each area is replaced by one sized for what was asked
+// (`used`) plus as much again beyond the old capacity, the old contents
+// copied over; *kept = the old capacity, where the allocator restarts.
+int grow_ops(pw_ctx *ctx, Workspace &W, Lane &L, int64_t used, long long *kept) {
+    const int64_t keep = W.ops_cap;
+    const int64_t need = std::min<int64_t>(used + (used - keep), std::max<int64_t>(used, dir_limit(ctx)));
+    const int rc = regrow(ctx, L, W.ops, (size_t)keep, (size_t)need);
+    if (rc) return rc;
+    W.ops_cap = (int64_t)W.ops.cap;
+    W.want_ops = std::max<int64_t>(W.want_ops, need);
+    *kept = keep;
+    return PW_OK;
+}
+
+int grow_records(pw_ctx *ctx, Workspace &W, Lane &L, int64_t used, long long *kept) {
+    const int64_t keep = W.out_cap;
+    const int64_t need = std::min<int64_t>(std::max<int64_t>(2 * keep, used + (used - keep)), (int64_t)INT32_MAX);
+    if (need <= keep) return PW_ENOMEM;
+    int rc = regrow(ctx, L, W.out_sps, (size_t)keep * sizeof(SpRec), (size_t)need * sizeof(SpRec));
+    if (!rc) rc = regrow(ctx, L, W.out_hits, (size_t)keep * sizeof(HitRec), (size_t)need * sizeof(HitRec));
+    set_slot_caps(W);
+    if (rc) return rc;
+    W.want_cand = std::max<int64_t>(W.want_cand, std::min<int64_t>(need, cand_slot_limit(ctx)));
+    *kept = keep;
+    return PW_OK;
+}
+
 int device_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry &g, double minimum, Finished &F,
                   int &n_last, bool &done) {
     Ctl *dctl = (Ctl *)W.ctl.p;
@@ -2991,24 +3020,32 @@ int device_rounds(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry
         HIPCHK(ctx, hipStreamSynchronize(L.s));
         c = *(const Ctl *)W.h_ctl.p;
         const int next = c.next_ngated;
-        if (next <= 0 || next >= prev) break;  // the round just walked finished nothing
-        // the record area full: no later round can place records
-        if ((int64_t)c.out_used > W.out_cap) break;
-        // the direction and candidate areas hold one round's work: grown (no
-        // contents to keep) toward what the round just walked asked for
+        if (next <= 0) break;
+        // where growth is on: the direction and candidate areas, which hold one
+        // round's work, toward what the round just walked asked for (nothing to
+        // keep); the record and ops areas, full, with their contents copied.
+        // Without growth a full record or ops area ends the rounds.
+        bool grew = false;
+        long long ops_keep = -1, out_keep = -1;
+        const bool full = (int64_t)c.out_used > W.out_cap || (int64_t)c.ops_used > W.ops_cap;
         if (ctx->lim.grow) {
-            const int rc = grow_round_areas(ctx, W, c);
+            int rc = grow_round_areas(ctx, W, c, grew);
+            if (!rc && (int64_t)c.out_used > W.out_cap) rc = grow_records(ctx, W, L, (int64_t)c.out_used, &out_keep);
+            if (!rc && (int64_t)c.ops_used > W.ops_cap) rc = grow_ops(ctx, W, L, (int64_t)c.ops_used, &ops_keep);
+            if (rc == PW_ENOMEM) {
+                // no room to grow: what is left goes to the host's tail rounds
+                (void)hipGetLastError();
+                ctx->err.clear();
+                break;
+            }
             if (rc) return rc;
+            grew = grew || out_keep >= 0 || ops_keep >= 0;
+        } else if (full) {
+            break;
         }
-        // the ops area full: grown (its records copied over) where growth is on
-        long long ops_keep = -1;
-        if ((int64_t)c.ops_used > W.ops_cap) {
-            if (!ctx->lim.grow) break;
-            const int rc = grow_ops(ctx, W, L, (int64_t)c.ops_used);
-            if (rc) return rc;
-            ops_keep = (long long)W.ops_kept;
-        }
-        hipLaunchKernelGGL(round_reset_kernel, dim3(1), dim3(64), 0, L.s, dctl, next, ops_keep);
+        // the round just walked finished nothing and no area grew
+        if (next >= prev && !grew) break;
+        hipLaunchKernelGGL(round_reset_kernel, dim3(1), dim3(64), 0, L.s, dctl, next, ops_keep, out_keep);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(L.ev[2], L.s));
         int rc = launch_tasks(ctx, P, W, L, g, true);

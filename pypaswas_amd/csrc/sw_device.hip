@@ -1449,16 +1449,19 @@ __global__ void __launch_bounds__(256) requeue_kernel(const PairTask *__restrict
 // Starts the next round (one thread): the ntasks requeued tasks (the count the
 // host read back) become the round's tasks and the traceback areas' allocators
 // restart; the record areas (out, ops) keep what earlier rounds wrote (after
-// the ops area grew, ops_keep >= 0 restarts its allocator there: every
+// an area grew, ops_keep / out_keep >= 0 restarts its allocator there: every
 // allocation that succeeded ends at or below the old capacity).  The
 // control block is written with device-scope atomic stores, as every kernel
 // reads it with device-scope atomics: a plain access can be served by this
 // XCD's L2 from a line cached before the requeue's atomics (seen on MI355X:
 // a plain load of next_ngated returned 0 after the host had read the count).
-__global__ void round_reset_kernel(Ctl *__restrict__ ctl, int ntasks, long long ops_keep) {
+__global__ void round_reset_kernel(Ctl *__restrict__ ctl, int ntasks, long long ops_keep, long long out_keep) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (ops_keep >= 0)
             __hip_atomic_store(&ctl->ops_used, (unsigned long long)ops_keep, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (out_keep >= 0)
+            __hip_atomic_store(&ctl->out_used, (unsigned long long)out_keep, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl->ngated, ntasks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl->next_ngated, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -114,3 +114,29 @@ def test_nothing_gated_and_graph_replays(dev, oracle):
     finally:
         dev.set_assembly(True)
         plan.release()
+
+
+def test_areas_grow_between_device_rounds(dev, oracle):
+    """A context whose first areas are tiny (one candidate slot per pair, a
+    64-byte ops area): the candidate, record and ops areas grow between device
+    rounds, the records written before each growth carried over."""
+    from pypaswas_amd import workloads as W
+    from pypaswas_amd._native import Device
+    qr, tr, st = W.config3(n_queries=60, n_targets=1000)
+    q, t = W.reference_order(qr, tr, False)
+    print("config3 60 x 1000, areas growing between device rounds", flush=True)
+    p = oracle_params(oracle, st)
+    qs, ts = [s for _, s in q], [s for _, s in t]
+    qids, tids = [i for i, _ in q], [i for i, _ in t]
+    pq, pt = all_pairs(len(qs), len(ts))
+    roomy = run(dev, oracle, p, qs, ts, pq, pt, qids, tids, mode=1)
+    d = Device(0)
+    try:
+        d.set_limits(candidates_per_pair=1, ops_bytes=64, grow=True)
+        small = run(d, oracle, p, qs, ts, pq, pt, qids, tids, mode=1)
+        same_result(roomy, small)
+        assert shown_rows(roomy) == shown_rows(small)
+        assert len(roomy.hits) > 5000
+        del small
+    finally:
+        d.close()
