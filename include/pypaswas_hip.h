@@ -249,10 +249,10 @@ int pw_set_tuning(pw_ctx *ctx, int32_t affine, const float *rates, int32_t group
 /* Plan-level tuning of a batched plan (pw_prepare, mode 1): its scoring pass
  * is timed under each combination of three routing choices -- the longest
  * lane-per-pair job allowed (1/8, 1/4 or 1/2 of the plan's cells per device
- * lane), the chunk width of lane-per-pair jobs whose target exceeds 96
- * residues (64 or 96), and tile shapes from the pw_autotune table or from the
- * cost model (when a table exists) -- and the fastest work list is kept for
- * the plan's executions; the defaults (1/4, 64, cost model) stay unless
+ * lane), the chunk width of lane-per-pair jobs whose column sequence exceeds
+ * 96 residues (80, 64 or 96), and tile shapes from the pw_autotune table or
+ * from the cost model (when a table exists) -- and the fastest work list is
+ * kept for the plan's executions; the defaults (1/4, 80, cost model) stay unless
  * another combination is at least 3% faster.  Results never depend on these
  * choices.
  * *seconds: tuning time; *score_ms: the kept scoring pass; choice[3]: share x

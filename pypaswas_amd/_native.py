@@ -339,7 +339,7 @@ class Device:
                                            ctypes.byref(nb), ctypes.byref(valid)), "pw_get_tuning")
         return {"valid": bool(valid.value), "rates": rates, "group_rates": grates, "group_nb_long": int(nb.value)}
 
-    def set_tuning(self, affine: bool, rates: Optional[np.ndarray], group_nb_long: int = 64):
+    def set_tuning(self, affine: bool, rates: Optional[np.ndarray], group_nb_long: int = 80):
         """Installs a saved table (pw_set_tuning); rates=None restores the cost model."""
         r = None if rates is None else np.ascontiguousarray(rates, dtype=np.float32).reshape(7, 7, 4)
         self._check(self.lib.pw_set_tuning(self.ctx, 1 if affine else 0, None if r is None else r.ctypes.data,

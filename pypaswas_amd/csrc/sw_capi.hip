@@ -132,10 +132,17 @@ constexpr int TUNE_A = 7, TUNE_B = 7, TUNE_R = 4;
 constexpr int TUNE_RS[TUNE_R] = {1, 2, 4, 8};
 constexpr int TUNE_REP_A[TUNE_A] = {48, 96, 192, 384, 768, 1536, 3072};
 constexpr int TUNE_REP_B[TUNE_B] = {24, 48, 96, 192, 384, 768, 1536};
+// Chunk width of multi-chunk lane-per-pair jobs (column sequence longer than 96
+// residues).  80 rather than 64: fewer chunk edges through the scratch column
+// per cell outweigh the wider chunk's padding and its register use (two waves
+// per SIMD instead of three); measured on the 2,000 x 20,000 protein set
+// (columns the queries, ~350 residues): score pass 1071 ms at 64, 993 at 80,
+// 1008 at 96, 1433 at 48.
+constexpr int GROUP_NB_LONG = 80;
 struct Tuning {
     bool valid = false;
     float rate[TUNE_A][TUNE_B][TUNE_R] = {};
-    int group_nb_long = 64;  // chunk width of lane-per-pair jobs whose target exceeds 96 residues
+    int group_nb_long = GROUP_NB_LONG;  // chunk width of lane-per-pair jobs whose column sequence exceeds 96 residues
     float group_rate[5] = {};  // measured for widths 32, 48, 64, 80, 96
 };
 
@@ -994,10 +1001,10 @@ constexpr double GROUP_SPLIT_MAX_WAIT = 2.0;
 constexpr int GROUP_MAX_ROWS = 1 << 15;
 
 // columns per chunk of a lane-per-pair job: one chunk up to 96 columns,
-// chunks of 64 (or the tuned width) beyond
+// chunks of GROUP_NB_LONG (or the tuned width) beyond
 int group_nb(int clen, const Tuning *tu) {
     if (clen <= 96) return std::max(16, (clen + 7) / 8 * 8);
-    return (tu && tu->valid) ? tu->group_nb_long : 64;
+    return (tu && tu->valid) ? tu->group_nb_long : GROUP_NB_LONG;
 }
 
 
@@ -3400,11 +3407,12 @@ int autotune(pw_ctx *ctx, double *seconds) {
             if (rc) return rc;
             tu.group_rate[w] = (float)((double)NT * 64 * A * B / (ms * 1e3));
         }
-        // the built-in width 64 unless another is clearly faster
-        float best = tu.group_rate[2];
-        tu.group_nb_long = 64;
+        // the built-in width unless another is clearly faster
+        const int wdef = (int)(std::find(widths, widths + 5, GROUP_NB_LONG) - widths);
+        float best = tu.group_rate[wdef];
+        tu.group_nb_long = GROUP_NB_LONG;
         for (int w = 0; w < 5; ++w)
-            if (tu.group_rate[w] > TUNE_MARGIN * tu.group_rate[2] && tu.group_rate[w] > best) {
+            if (tu.group_rate[w] > TUNE_MARGIN * tu.group_rate[wdef] && tu.group_rate[w] > best) {
                 best = tu.group_rate[w];
                 tu.group_nb_long = widths[w];
             }
@@ -3436,7 +3444,7 @@ int tune_plan(pw_ctx *ctx, pw_plan *P, double *seconds, double *best_ms, int32_t
     const Tuning *tu = &ctx->tune[P->sc.affine ? 1 : 0];
     std::vector<Cand> cands;
     for (double share : {0.125, 0.25, 0.5})
-        for (int nb : {64, 96})
+        for (int nb : {GROUP_NB_LONG, 64, 96})
             for (int table = 0; table < (tu->valid ? 2 : 1); ++table) cands.push_back(Cand{share, nb, table != 0});
     auto opts_of = [&](const Cand &c) {
         PlanOpts o = plan_opts(ctx, P->sc);
@@ -3448,7 +3456,8 @@ int tune_plan(pw_ctx *ctx, pw_plan *P, double *seconds, double *best_ms, int32_t
     // the default choices are timed first and kept unless another candidate
     // is clearly faster (differences within a few per cent are noise)
     std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) {
-        const bool da = a.share == 0.25 && a.nb == 64 && !a.table, db = b.share == 0.25 && b.nb == 64 && !b.table;
+        const bool da = a.share == 0.25 && a.nb == GROUP_NB_LONG && !a.table,
+                   db = b.share == 0.25 && b.nb == GROUP_NB_LONG && !b.table;
         return da && !db;
     });
     constexpr double PLAN_TUNE_MARGIN = 0.97;
