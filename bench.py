@@ -161,6 +161,20 @@ def main():
     ms_step = elapsed * 1000 / args.steps
     log("batched: %.1f ms/step, %.1f GCUPS, score kernel %.2f ms" % (ms_step, gcups, np.mean(score_ms)))
     last = r
+    # the same plan through its captured HIP graph (pw_execute_graph): one
+    # capture (untimed, with a warmup replay), then as many timed replays
+    r = dev.execute(plan, minimum, graph=True)
+    r = dev.execute(plan, minimum, graph=True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = dev.execute(plan, minimum, graph=True)
+    el_g = time.perf_counter() - t0
+    graph = {"ms_per_step": round(el_g * 1000 / args.steps, 3), "gcups": round(cells * args.steps / el_g / 1e9, 2),
+             "identical_results": bool(np.array_equal(r.maxima, last.maxima) and np.array_equal(r.hits, last.hits)
+                                       and r.ops == last.ops)}
+    log("graph replays: %.2f ms/step, %.1f GCUPS, identical=%s" % (graph["ms_per_step"], graph["gcups"],
+                                                                   graph["identical_results"]))
+    del r
     plan.release()
 
     plain = None
@@ -233,6 +247,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_launch": plain,
+        "graph_replay": graph,
         "end_to_end": engine,
         "batched_over_plain": round(gcups / plain["gcups"], 2) if plain else None,
         "protein_uneven": protein,
