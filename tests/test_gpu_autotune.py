@@ -92,7 +92,8 @@ def test_plan_tuning_keeps_results(tuned, oracle):
         before = tuned.execute(plan, p.minimum)
         info = tuned.tune_plan(plan)
         print("plan tuning", info, flush=True)
-        assert info["score_ms"] > 0 and info["group_chunk_width"] in (64, 96)
+        # the widths plan tuning times: the default long-job width (80) and 64, 96
+        assert info["score_ms"] > 0 and info["group_chunk_width"] in (64, 80, 96)
         after = tuned.execute(plan, p.minimum)
         same_result(after, before)
         same_result(tuned.execute(plan, p.minimum, graph=True), before)
