@@ -1987,6 +1987,19 @@ int persistent_grid(pw_ctx *ctx, const void *kernel, size_t lds, int64_t work, d
     return (int)std::max<int64_t>(1, std::min<int64_t>(work, g));
 }
 
+// Per-lane score profile of the checkpointed pairs' scoring pass (small
+// alphabets only: nres + 1 slots of 64 lanes x CK_R floats after the
+// wave-per-pair kernel's LDS, 10-12 KiB for DNA, so that a CU still holds
+// ten or more of its waves); 0: the pass reads the substitution table.
+constexpr int PROF_MAX_SLOTS = 6;
+int prof_nres(const pw_plan *P) {
+    return (!P->ckpairs.empty() && P->E.K + 1 <= PROF_MAX_SLOTS) ? P->E.K : 0;
+}
+size_t score_lds(const pw_plan *P) {
+    const int n = prof_nres(P);
+    return lds_bytes(P->E.KP) + (n > 0 ? (size_t)(n + 1) * WAVE * CK_R * sizeof(float) : 0);
+}
+
 size_t group_lds(const pw_plan *P, int w) { return (size_t)P->E.KP * (NB_WIDTHS[w] + 4) * sizeof(float); }
 
 Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
@@ -2003,7 +2016,7 @@ Geometry geometry(pw_ctx *ctx, const pw_plan *P) {
     }
     if (P->n_tile > 0) {
         g.tile_grid = persistent_grid(ctx, (const void *)(aff ? score_kernel<true> : score_kernel<false>),
-                                      lds_bytes(P->E.KP), P->n_tile, P->opts.grid_share);
+                                      score_lds(P), P->n_tile, P->opts.grid_share);
         g.tscr_stride = 3 * (P->max_b + 64);
     }
     const int64_t n = P->n_pairs;
@@ -2169,7 +2182,8 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
     }
     if (P->n_tile > 0) {
         auto kern = aff ? score_kernel<true> : score_kernel<false>;
-        const size_t lds = lds_bytes(P->E.KP);
+        const size_t lds = score_lds(P);
+        const int nres = prof_nres(P);
         const ScoreItem *items = (const ScoreItem *)P->d_items.p;
         const uint8_t *trans = (const uint8_t *)P->d_trans.p;
         if (P->mode == 0) {
@@ -2180,7 +2194,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
                 hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, L.s, res, seqs, items + b.first, (int)n,
                                    trans + b.first, tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext,
                                    (float *)W.tscratch.p, g.tscr_stride, (unsigned *)nullptr, mx, (float *)nullptr,
-                                   (unsigned *)nullptr, &ctl->abort_wait, ctx->wait_polls, (float *)P->d_ck.p);
+                                   (unsigned *)nullptr, &ctl->abort_wait, ctx->wait_polls, (float *)P->d_ck.p, nres);
                 HIPCHK(ctx, hipGetLastError());
                 if (launches) *launches += 1;
             }
@@ -2192,7 +2206,7 @@ int launch_score(pw_ctx *ctx, pw_plan *P, Workspace &W, Lane &L, const Geometry 
             hipLaunchKernelGGL(kern, dim3(g.tile_grid), dim3(64), lds, st, res, seqs, items, (int)P->n_tile, trans,
                                tabs, P->E.KP, P->E.lkp, P->sc.gap, P->sc.ext, (float *)W.tscratch.p, g.tscr_stride,
                                ctl->queue, mx, (float *)P->d_links.p, (unsigned *)P->d_prog.p, &ctl->abort_wait,
-                               ctx->wait_polls, (float *)P->d_ck.p);
+                               ctx->wait_polls, (float *)P->d_ck.p, nres);
             HIPCHK(ctx, hipGetLastError());
             if (launches) *launches += 1;
         }

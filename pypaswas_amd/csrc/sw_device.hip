@@ -141,6 +141,10 @@ struct CkArgs {
     float *ck = nullptr;
     float *bmax = nullptr;
     int t0 = 0, t1 = -1;
+    // scoring pass with a per-lane profile (PROF): this wave's profile in LDS
+    // and the residue codes it covers (slot c < nres: code c, slot nres: FILL)
+    float *prof = nullptr;
+    int nres = 0;
 };
 // The block arguments for the tile instance of R residues per lane: only the
 // CK_R instance runs checkpoint blocks.
@@ -163,13 +167,14 @@ __device__ __forceinline__ CkArgs ck_for(const CkArgs &c) {
 // from boundary s-1 of links and the bottom row goes to boundary s, chunk by
 // chunk, synchronised through the progress counters prog.
 // ---------------------------------------------------------------------------
-template <bool AFF, int R, bool DIRS, bool CK = false>
+template <bool AFF, int R, bool DIRS, bool CK = false, bool PROF = false>
 __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int alen_it, int blen_it,
                       const float *subT, int lkp, uint8_t *bwin, float4 *stg, float *__restrict__ scr, float G,
                       float E, int fill, uint8_t *__restrict__ dir, float t1, float mp, int *__restrict__ cand_count,
                       Cand *__restrict__ cands, int cand_cap, int s_first, int s_last, float *__restrict__ links,
                       unsigned *__restrict__ prog, unsigned *abort, unsigned wlimit, CkArgs ckv = CkArgs()) {
     static_assert(!CK || (!DIRS && !AFF && R == CK_R), "checkpoints: the linear scoring pass at CK_R residues per lane");
+    static_assert(!PROF || (CK && R % 4 == 0), "per-lane profile: the checkpointing scoring pass");
     const int lane = threadIdx.x & 63;
     const int rows = WAVE * R;
     const int nstripes = (alen_it + rows - 1) / rows;
@@ -233,12 +238,41 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         nbuf = 0;
     };
 
+    // Score reads.  Table form: the entry of the swept residue's row (byte
+    // offset boff = code << lkp) at this lane's residue of row r (a4[r]), one
+    // address add and one LDS read per cell.  Profile form (PROF, small
+    // alphabets): the lane's R scores against every residue code sit
+    // contiguously in LDS, [slot][lane][r], and boff is the slot's byte offset;
+    // one add and R/4 wide reads per step.
+    constexpr int PSHIFT = PROF ? 6 + (R == 8 ? 3 : 2) + 2 : 0;  // bytes per slot: 64 lanes x R floats
+    const char *profl = PROF ? reinterpret_cast<const char *>(ckv.prof) + lane * (R * 4) : nullptr;
+    auto off_of = [&](int code) {
+        if constexpr (PROF) return min(code, ckv.nres) << PSHIFT;
+        else return code << lkp;
+    };
     for (int s = s_first; s < s_last; ++s) {
         int a4[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = s * rows + lane * R + r;
             a4[r] = ((i < A.len) ? (int)res[A.off + i] : fill) * 4;
+        }
+        if constexpr (PROF) {
+            // this stripe's profile; each lane writes and reads only its own
+            // entries, so LDS order within the lane is all it needs
+            float *pw = ckv.prof + lane * R;
+            for (int c = 0; c <= ckv.nres; ++c) {
+                const int b = c < ckv.nres ? c : fill;
+#pragma unroll
+                for (int r4 = 0; r4 < R; r4 += 4) {
+                    float4 v;
+                    v.x = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4]);
+                    v.y = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4 + 1]);
+                    v.z = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4 + 2]);
+                    v.w = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4 + 3]);
+                    *reinterpret_cast<float4 *>(pw + (c << (PSHIFT - 2)) + r4) = v;
+                }
+            }
         }
         // state of this lane's rows at the previous column: T, I and, for
         // affine gaps, d + G with d = T[i-1][j-1] + s (and, for the direction
@@ -307,12 +341,26 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         asm volatile("" : "+s"(wbase));
         load_window(wbase);
         const int w0 = tb0 & (BWIN - 1);
-        int fB = ((int)bwin[w0 + (WAVE - 1) + lane]) << lkp;  // columns tb0 .. tb0 + 63
+        int fB = off_of((int)bwin[w0 + (WAVE - 1) + lane]);  // columns tb0 .. tb0 + 63
         // column tb0 - lane (< 0 for lanes 1..63 at step 0)
-        int boff = tb0 > 0 ? ((int)bwin[w0 + (WAVE - 1) - lane]) << lkp : (lane == 0 ? fB : (fill << lkp));
-        float scn[R];
+        int boff = tb0 > 0 ? off_of((int)bwin[w0 + (WAVE - 1) - lane]) : (lane == 0 ? fB : off_of(fill));
+        auto read_scores = [&](float (&o)[R]) {
+            if constexpr (PROF) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) scn[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+                for (int r4 = 0; r4 < R; r4 += 4) {
+                    const float4 v = *reinterpret_cast<const float4 *>(profl + boff + r4 * 4);
+                    o[r4] = v.x;
+                    o[r4 + 1] = v.y;
+                    o[r4 + 2] = v.z;
+                    o[r4 + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) o[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+            }
+        };
+        float scn[R];
+        read_scores(scn);
         // lane 63's bottom-row values of the latest steps, the newest in lane 63
         // (lane L holds the one of the step 63 - L steps back)
         float oT = 0.0f, oJ = 0.0f, oM = 0.0f;
@@ -341,13 +389,12 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             // scores of step t+1 (also at t+1 == nsteps: offsets always lie in the table)
             if ((tn & (WAVE - 1)) == 0) {
                 if ((tn & (BWIN - 1)) == 0) load_window(tn);
-                fB = ((int)bwin[(tn & (BWIN - 1)) + (WAVE - 1) + lane]) << lkp;
+                fB = off_of((int)bwin[(tn & (BWIN - 1)) + (WAVE - 1) + lane]);
             } else {
                 fB = shl1i(fB, fB);
             }
             boff = shr1i(fB, boff);
-#pragma unroll
-            for (int r = 0; r < R; ++r) nx[r] = *reinterpret_cast<const float *>(subB + boff + a4[r]);
+            read_scores(nx);
 
             float tup = upT, jin = upJ;
             uint32_t bits[R];
@@ -579,12 +626,13 @@ __device__ __forceinline__ void load_tables(float *lds, const float *__restrict_
     __syncthreads();
 }
 
-template <bool AFF, int R, bool CK = false>
+template <bool AFF, int R, bool CK = false, bool PROF = false>
 __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *seqs, const ScoreItem &it,
                                             const float *lds_tab, int kp2, int lkp, uint8_t *bwin, float4 *stg,
                                             float *scr,
                                             float G, float E, int fill, int trans, float *links, unsigned *prog,
-                                            unsigned *abort, unsigned wlimit, float *ckarea = nullptr) {
+                                            unsigned *abort, unsigned wlimit, float *ckarea = nullptr,
+                                            float *prof = nullptr, int nres = 0) {
     const float *tab = lds_tab + (trans ? kp2 : 0);
     float *lk = it.rows_off >= 0 ? links + it.rows_off : nullptr;
     unsigned *pg = it.rows_off >= 0 ? prog + it.prog_off : nullptr;
@@ -592,8 +640,10 @@ __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *se
     if (CK) {
         ckv.ck = ckarea + it.ck_off;
         ckv.bmax = ckarea + it.bmax_off;
+        ckv.prof = prof;
+        ckv.nres = nres;
     }
-    return tile<AFF, R, false, CK>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, stg, scr,
+    return tile<AFF, R, false, CK, PROF>(res, seqs[it.a_seq], seqs[it.b_seq], it.alen, it.blen, tab, lkp, bwin, stg, scr,
                                    G, E, fill, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0, it.s0, it.s1, lk, pg, abort,
                                    wlimit, ckv);
 }
@@ -606,7 +656,10 @@ __device__ __forceinline__ float score_item(const uint8_t *res, const SeqRef *se
 // split pair's maximum is the maximum over its stripes, combined with an
 // integer atomic max on the (non-negative) float bits into a zeroed slot.
 // The stripes of a checkpointed pair also write their checkpoints and block
-// maxima into ckarea.
+// maxima into ckarea.  nres > 0: they read their scores from a per-lane
+// profile of nres + 1 slots in LDS after the hand-over staging (the host
+// passes nres, the number of residue codes, only when the launch's LDS holds
+// that profile).
 template <bool AFF>
 __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ res, const SeqRef *__restrict__ seqs,
                                                    const ScoreItem *__restrict__ items, int nitems,
@@ -615,12 +668,14 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
                                                    float E, float *__restrict__ scratch, int64_t scr_stride,
                                                    unsigned *__restrict__ queue, float *__restrict__ maxima,
                                                    float *__restrict__ links, unsigned *__restrict__ prog,
-                                                   unsigned *abort, unsigned wlimit, float *__restrict__ ckarea) {
+                                                   unsigned *abort, unsigned wlimit, float *__restrict__ ckarea,
+                                                   int nres) {
     extern __shared__ float lds[];
     const int kp2 = kp * kp;
     load_tables(lds, tables, 2 * kp2);
     uint8_t *bwin = reinterpret_cast<uint8_t *>(lds + 2 * kp2);
     float4 *stg = reinterpret_cast<float4 *>(bwin + BWIN + WAVE);
+    float *prof = reinterpret_cast<float *>(stg + 2 * WAVE);
     float *scr = scratch + (int64_t)blockIdx.x * scr_stride;
     const int fill = kp - 1;
     int k = 0;
@@ -641,8 +696,12 @@ __global__ void __launch_bounds__(64) score_kernel(const uint8_t *__restrict__ r
         bool done = false;
         if constexpr (!AFF) {
             if (it.ck_off >= 0) {
-                m = score_item<AFF, CK_R, true>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr, links,
-                                                prog, abort, wlimit, ckarea);
+                if (nres > 0)
+                    m = score_item<AFF, CK_R, true, true>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr,
+                                                          links, prog, abort, wlimit, ckarea, prof, nres);
+                else
+                    m = score_item<AFF, CK_R, true>(res, seqs, it, lds, kp2, lkp, bwin, stg, scr, G, E, fill, tr,
+                                                    links, prog, abort, wlimit, ckarea);
                 done = true;
             }
         }
