@@ -241,11 +241,12 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
     // Score reads.  Table form: the entry of the swept residue's row (byte
     // offset boff = code << lkp) at this lane's residue of row r (a4[r]), one
     // address add and one LDS read per cell.  Profile form (PROF, small
-    // alphabets): the lane's R scores against every residue code sit
-    // contiguously in LDS, [slot][lane][r], and boff is the slot's byte offset;
-    // one add and R/4 wide reads per step.
+    // alphabets): the lane's R scores against every residue code, as float4s
+    // of rows 4g..4g+3 laid out [slot][g][lane] (the wave's 16-byte reads of
+    // one g are contiguous, free of bank conflicts), and boff is the slot's
+    // byte offset; one add and R/4 wide reads per step.
     constexpr int PSHIFT = PROF ? 6 + (R == 8 ? 3 : 2) + 2 : 0;  // bytes per slot: 64 lanes x R floats
-    const char *profl = PROF ? reinterpret_cast<const char *>(ckv.prof) + lane * (R * 4) : nullptr;
+    const char *profl = PROF ? reinterpret_cast<const char *>(ckv.prof) + lane * 16 : nullptr;
     auto off_of = [&](int code) {
         if constexpr (PROF) return min(code, ckv.nres) << PSHIFT;
         else return code << lkp;
@@ -260,7 +261,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
         if constexpr (PROF) {
             // this stripe's profile; each lane writes and reads only its own
             // entries, so LDS order within the lane is all it needs
-            float *pw = ckv.prof + lane * R;
+            float *pw = ckv.prof + lane * 4;
             for (int c = 0; c <= ckv.nres; ++c) {
                 const int b = c < ckv.nres ? c : fill;
 #pragma unroll
@@ -270,7 +271,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
                     v.y = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4 + 1]);
                     v.z = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4 + 2]);
                     v.w = *reinterpret_cast<const float *>(subB + (b << lkp) + a4[r4 + 3]);
-                    *reinterpret_cast<float4 *>(pw + (c << (PSHIFT - 2)) + r4) = v;
+                    *reinterpret_cast<float4 *>(pw + (c << (PSHIFT - 2)) + r4 * WAVE) = v;
                 }
             }
         }
@@ -348,7 +349,7 @@ __device__ float tile(const uint8_t *__restrict__ res, SeqRef A, SeqRef B, int a
             if constexpr (PROF) {
 #pragma unroll
                 for (int r4 = 0; r4 < R; r4 += 4) {
-                    const float4 v = *reinterpret_cast<const float4 *>(profl + boff + r4 * 4);
+                    const float4 v = *reinterpret_cast<const float4 *>(profl + boff + r4 * (WAVE * 4));
                     o[r4] = v.x;
                     o[r4 + 1] = v.y;
                     o[r4 + 2] = v.z;
