@@ -1723,6 +1723,46 @@ __device__ __forceinline__ int window_cell(PathWindow &w, const uint8_t *__restr
     return ((t - w.t0) * WAVE + k) * R + r;
 }
 
+// Window index of cell (x, y) (x, y >= 0) with no window load, per lane; ok
+// says whether the current window holds the cell.
+__device__ __forceinline__ int window_index(const PathWindow &w, int x, int y, bool &ok) {
+    const int R = w.R, rows = WAVE * R;
+    const int s = div_r(x >> 6, w.lgR), rem = x - s * rows, k = div_r(rem, w.lgR), r = rem - k * R;
+    const int t = y + k;
+    ok = s == w.s && t >= w.t0 && t < w.t0 + w.nt;
+    return ((t - w.t0) * WAVE + k) * R + r;
+}
+
+// Writes nibble b of window index idx (old: its byte as read) into the window
+// and memory, from the calling lane.
+__device__ __forceinline__ void window_write_lane(PathWindow &w, uint8_t *__restrict__ dirs, int idx, uint32_t b,
+                                                  uint32_t old) {
+    const uint32_t sh = (uint32_t)(idx & 1) << 2;
+    const uint8_t nb = (uint8_t)((old & ~(15u << sh)) | ((b & 15u) << sh));
+    w.blk[idx >> 1] = nb;
+    dirs[w.dir_off + ((int64_t)w.s * w.nsteps + w.t0) * (WAVE * w.R) / 2 + (idx >> 1)] = nb;
+}
+
+// Diagonal runs of a path, one cell per lane: lane i reads cell (x - i, y - i)
+// when the current window holds it (no load); its window index, byte and
+// nibble come back in o, byte and nib (valid when ok).  Two cells of one
+// diagonal never share a byte (their steps differ), so the lanes can write
+// their cells' nibbles at once.
+__device__ __forceinline__ void diag_cells(const PathWindow &w, int x, int y, bool &ok, int &o, uint32_t &byte,
+                                           uint32_t &nib) {
+    const int lane = threadIdx.x & 63;
+    const int xi = x - lane, yi = y - lane;
+    ok = xi >= 0 && yi >= 0;
+    o = 0;
+    byte = 0;
+    nib = 0;
+    if (ok) o = window_index(w, xi, yi, ok);
+    if (ok) {
+        byte = w.blk[o >> 1];
+        nib = (byte >> ((uint32_t)(o & 1) << 2)) & 15u;
+    }
+}
+
 // The direction nibble of cell (x, y), wave-uniform; idx is its nibble index
 // in the window.
 template <bool RES>
@@ -1874,6 +1914,32 @@ __global__ void __launch_bounds__(64) tb_wave_kernel(const SeqRef *__restrict__ 
                     int cx = x, cy = y, o = idx;
                     uint32_t cb = b, cbyte = byte;
                     for (;;) {
+                        if ((cb & 3u) == DB_MAIN && !(cb & DB_STOP)) {
+                            // a diagonal run, one cell per lane: cell i (i >= 1)
+                            // is reached when cells 0..i-1 go on diagonally and
+                            // cells 1..i are in the window and unmarked; every
+                            // reached cell is marked, then the last one goes on
+                            // below as the current cell (marked again, the same
+                            // byte)
+                            bool ok;
+                            int oi;
+                            uint32_t byi, ni;
+                            diag_cells(w, cx, cy, ok, oi, byi, ni);
+                            const int lane = threadIdx.x & 63;
+                            const int dc = (ok && (ni & 3u) == DB_MAIN && !(ni & DB_STOP)) ? 1 : 0;
+                            const int dprev = shr1i(0, dc);  // lane i: dc of lane i-1 (every lane takes part)
+                            const bool reach = ok && (lane == 0 || (!(ni & DB_MARK) && dprev != 0));
+                            const uint64_t miss = __builtin_amdgcn_ballot_w64(!reach);
+                            const int m = miss ? __builtin_ctzll(miss) : WAVE;  // cells 0..m-1 reached
+                            if (lane < m) window_write_lane(w, dirs, oi, ni | DB_MARK, byi);
+                            if (m > 1) {
+                                cx -= m - 1;
+                                cy -= m - 1;
+                                o = __builtin_amdgcn_readlane(oi, m - 1);
+                                cb = (uint32_t)__builtin_amdgcn_readlane((int)ni, m - 1);
+                                cbyte = (uint32_t)__builtin_amdgcn_readlane((int)byi, m - 1);
+                            }
+                        }
                         window_write(w, dirs, o, cb | DB_MARK, cbyte);
                         const uint32_t code = cb & 3u;
                         if (cb & DB_STOP) break;
@@ -2123,6 +2189,38 @@ __global__ void __launch_bounds__(64) walk_wave_kernel(const uint8_t *__restrict
             const int64_t start_ops = nops;
             bool term = false;
             while (show && !term && x >= 0 && y >= 0 && direction != 64 && direction != 32) {
+                if (direction == 1) {
+                    // a diagonal run, one cell per lane: the cells from (x, y)
+                    // on whose direction is 1 (up-left, not a stop), in the
+                    // window, taken at once (walked marks, op characters,
+                    // matches); the first other cell is taken below
+                    bool ok;
+                    int oi;
+                    uint32_t byi, ni;
+                    diag_cells(w, x, y, ok, oi, byi, ni);
+                    const int lane = threadIdx.x & 63;
+                    const bool dg = ok && eff_dir<AFF>(ni) == 1;
+                    const uint64_t miss = __builtin_amdgcn_ballot_w64(!dg);
+                    const int m = miss ? __builtin_ctzll(miss) : WAVE;
+                    if (m > 0) {
+                        bool same = false;
+                        if (lane < m) {
+                            same = w.qw[x - lane - w.s * WAVE * w.R] == w.tw[y - lane - (w.t0 - (WAVE - 1))];
+                            window_write_lane(w, dirs, oi, nib_walked(ni), byi);
+                            op[nops + lane] = 'D';
+                        }
+                        const int nm = __builtin_popcountll(__builtin_amdgcn_ballot_w64(lane < m && same));
+                        matches += nm;
+                        mismatches += m - nm;
+                        nops += m;
+                        len += m;
+                        s_start -= m;
+                        t_start -= m;
+                        x -= m;
+                        y -= m;
+                        continue;
+                    }
+                }
                 int o;
                 const uint32_t b = window_read<true>(w, dirs, res, x, y, o, byte);
                 // the window holds the residues of the cell just read (read
