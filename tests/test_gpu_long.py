@@ -104,3 +104,34 @@ def test_split_equals_unsplit(dev, nosplit_dev, oracle, affine):
     unsplit = hip_all_vs_all(nosplit_dev, oracle, qs, ts, p, mode=1)
     same_result(split, unsplit)
     assert split.n_traceback_pairs >= 3
+
+
+@pytest.mark.parametrize("affine", [False, True])
+def test_long_diagonal_runs_reach_the_edges(dev, oracle, affine):
+    """Wave-per-pair start selection and walk take diagonal runs one cell per
+    lane.  Long exact and near-exact diagonals here end at the matrix corner
+    (query = target prefix), at column 0 (the target's start inside the query)
+    and at row 0 (the query's start inside the target), crossing many LDS
+    windows and stripes; a low lower limit (0.5 linear, 0.97 affine: ~2,300 and
+    ~9,800 starting points) gives many starting points per pair whose paths run
+    into cells an earlier path marked or walked."""
+    print("long diagonal runs affine", affine, flush=True)
+    rng = np.random.default_rng(70 + int(affine))
+    t0 = rand_seq(rng, 4300, DNA)
+    q0 = t0[:4100] + rand_seq(rng, 200, DNA)                  # corner: path to (0, 0)
+    body = mutate(rng, rand_seq(rng, 3900, DNA), 0.03, 0.0, DNA)
+    q1 = rand_seq(rng, 300, DNA) + body                       # target from column 0
+    t1 = body[:3900] + rand_seq(rng, 600, DNA)
+    body2 = rand_seq(rng, 3800, DNA)
+    q2 = mutate(rng, body2, 0.05, 0.01, DNA) + rand_seq(rng, 500, DNA)  # query from row 0
+    t2 = rand_seq(rng, 500, DNA) + body2
+    qs, ts = [q0, q1, q2], [t0, t1, t2]
+    assert all(len(q) + len(t) >= 8192 for q, t in zip(qs, ts))
+    p = dna_params(oracle, affine, ll=(0.97 if affine else 0.5), ff=0.05)
+    res = hip_all_vs_all(dev, oracle, qs, ts, p, mode=1)
+    compare(oracle, qs, ts, p, res)
+    assert res.n_traceback_pairs >= 3
+    assert len(res.starting_points) > 1000
+    shown = res.hits[res.hits["shown"] != 0]
+    # the shown paths end at the corner, at column 0 and at row 0
+    assert {(int(h["s_start"]) == 0, int(h["t_start"]) == 0) for h in shown} >= {(True, True), (False, True), (True, False)}
